@@ -1,0 +1,168 @@
+"""Throughput of the SPPM pass loop on one MI355X.
+
+A step is one pass (m_display, glslppm.cpp:214-536) over synthetic input: photon trace of every
+photon slot, hash build, gather and progressive update of every hit point, draw, and the eye
+trace every max_path_length passes.  The default workload is the ~200k-triangle scene: the Cornell
+room plus 40 jittered icospheres of subdivision 4 (50% diffuse, 30% mirror, 20% glass, IOR 1.5),
+1024x1024 eye rays, 1024x1024 photons per pass, 1024x1024 hash slots, 10 bounces, alpha 0.7.
+
+Prints one JSON line.  value = photon segments per second (P x passes / s) with all state resident
+in HBM; eye rays/s, passes/s, emitted photons/s and the reference's "M paths/s" are reported beside
+it.  The roofline entry is the dominant kernel's algorithmic bytes per launch over its average
+launch time, measured with HIP events on the context stream inside the timed region.  The CPU
+baseline is the oracle (strict fp32 restatement, OpenMP) timed on this host on a bounded sample.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "photons/s + eye-rays/s per pass; p50/p99 time-to-target-passes under load"
+HBM_PEAK_GBS = 8000.0
+
+# Algorithmic bytes per unit of work, each array read or written once in its minimal payload:
+#   photon segment (k_photon): state in 72 B (pos, dir, flux, hit normal 4 x 12 B, generator 16 B,
+#     triangle 4 B, bounce 4 B) + state out 72 B + slot index out 4 B + count and representative
+#     updates 8 B = 156 B
+#   hit point per pass (k_gather): reads pos, normal, reflectance 36 B, triangle 4 B, (tau, R),
+#     (E, N) 32 B, writes (tau, R), (E, N) 32 B, RGBA8 4 B = 108 B; plus 8 slots x (count 4 B +
+#     representative 4 B) and up to 8 representative photons x 36 B, which land in L2 and are
+#     not counted
+#   eye ray (k_eye): reads generator 16 B and (E, N) 16 B, writes pos, refl, nrm 48 B, generator
+#     16 B, triangle 4 B, (material, t) 8 B, (E, N) 16 B = 124 B
+#   slot clear (k_clear_slots): 8 B per slot
+BYTES_PER = {"photon": 156.0, "gather": 108.0, "eye": 124.0, "clear": 8.0}
+
+WORKLOADS = {
+    "spheres-1024": dict(kind=1, seed=0x67736C70, count=40, detail=4, res=1024, L=10, camera=((0, 0, 1.667), (0, 0, 0), 45.0),
+                         desc="Cornell room + 40 icospheres (subdiv 4): Q=P=H=1024^2, 10 bounces, alpha 0.7"),
+    "cornell-256": dict(kind=0, seed=0, count=0, detail=0, res=256, L=8, camera=((0, 0, 1.667), (0, 0, 0), 45.0),
+                        fixed_radius=0.02,
+                        desc="Cornell room, 36 triangles: Q=P=H=256^2, 8 bounces, radius 1/50 of the room"),
+}
+
+
+def build_scene(w):
+    import glslppm_amd as g
+    m = g.Mesh.generate(w["kind"], seed=w["seed"], count=w["count"], detail=w["detail"])
+    (org, at, fov) = w["camera"]
+    m.set_camera(org, at, w["res"], w["res"], fov)
+    t = time.time()
+    m.build()
+    return m, time.time() - t
+
+
+def params_for(w):
+    import glslppm_amd as g
+    kw = dict(max_path_length=w["L"])
+    if w.get("fixed_radius"):
+        kw["fixed_radius"] = w["fixed_radius"]
+    return g.default_params(w["res"], **kw)
+
+
+def cpu_baseline(mesh, params, passes):
+    """Oracle on this host's cores over the first `passes` passes of the same workload."""
+    import oracle
+    o = oracle.OracleRenderer(mesh, params)
+    t = time.time()
+    o.run(passes)
+    dt = time.time() - t
+    P = params.photon_width * params.photon_height
+    return {"value": P * passes / dt, "unit": "photons/s", "cores": oracle.num_threads(), "kind": "port",
+            "sample": "first %d passes (1 eye pass + %d photon passes) of the same workload, %.1f s" % (
+                passes, passes, dt)}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=50)
+    ap.add_argument("--warmup", type=int, default=10)
+    ap.add_argument("--workload", default="spheres-1024", choices=sorted(WORKLOADS))
+    ap.add_argument("--cpu-passes", type=int, default=10, help="passes of the CPU baseline sample (0: skip)")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU: this build has no multi-GPU path (each box has one GPU); "
+              "run with --gpus 1", file=sys.stderr)
+        sys.exit(2)
+
+    import glslppm_amd as g
+    w = WORKLOADS[args.workload]
+    mesh, build_s = build_scene(w)
+    params = params_for(w)
+    r = g.Renderer(mesh, params)
+    Q = params.image_width * params.image_height
+    P = params.photon_width * params.photon_height
+    H = params.hash_width * params.hash_height
+    L = params.max_path_length
+
+    r.run(args.warmup)
+    r.sync()
+    c0 = r.counters()
+    r.profile(True)
+    t0 = time.perf_counter()
+    r.run(args.steps)
+    r.sync()
+    t1 = time.perf_counter()
+    prof = r.profile_read()
+    r.profile(False)
+    c1 = r.counters()
+
+    wall = t1 - t0
+    passes = args.steps
+    eye_passes = c1["eye_samples"] - c0["eye_samples"]
+    emitted = c1["photons_emitted"] - c0["photons_emitted"]
+    value = P * passes / wall
+    # dominant kernel by total device time in the timed region
+    dom = max(prof, key=lambda k: prof[k][0])
+    dom_ms, dom_n = prof[dom]
+    units = {"photon": P, "gather": Q, "eye": Q, "clear": H}[dom]
+    bytes_per_launch = BYTES_PER[dom] * units
+    avg_ms = dom_ms / max(dom_n, 1)
+    achieved = bytes_per_launch / (avg_ms * 1e-3) / 1e9
+    traffic = None
+    tf = os.path.join(ROOT, "profiles", "traffic_%s_%s.json" % (dom, args.workload))
+    if os.path.exists(tf):
+        try:
+            traffic = json.load(open(tf)).get("hbm_bytes_per_launch")
+        except Exception:
+            traffic = None
+
+    cpu = None
+    if args.cpu_passes > 0:
+        cpu = cpu_baseline(mesh, params, args.cpu_passes)
+
+    line = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "photons/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": wall * 1e3 / passes,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (seeded scene generator, xorshift128 seeds of the reference)",
+        "config": {"workload": w["desc"], "triangles": mesh.num_triangles, "eye_rays": Q, "photons_per_pass": P,
+                   "hash_slots": H, "max_path_length": L, "bvh_build_s": round(build_s, 3)},
+        "eye_rays_per_s": Q * eye_passes / wall,
+        "passes_per_s": passes / wall,
+        "emitted_photons_per_s": emitted / wall,
+        "m_paths_per_s": (emitted + Q * eye_passes) / wall / (1024.0 * 1024.0),
+        "kernel_ms_per_launch": {k: (v[0] / v[1] if v[1] else None) for k, v in prof.items()},
+        "roofline": {"bound": "hbm", "kernel": "k_" + dom, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
+                     "bytes_per_launch": bytes_per_launch, "avg_launch_ms": avg_ms},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

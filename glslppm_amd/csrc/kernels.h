@@ -1,0 +1,118 @@
+// Device-side data layout and kernels of the SPPM pass loop on gfx950 (wave64).
+//
+// HBM layout (all arrays 16-byte aligned, structure-of-arrays so that a wave's 64 lanes read
+// 1 KiB contiguous per field):
+//   scene:  nodes[6][N][2] float4  per cube face, in that face's pre-order:
+//                                  (min.xyz, miss position), (max.xyz, leaf triangle or -1);
+//                                  the hit link is the next position, so it is not stored.
+//           tri_isect[T][3] float4 (v0, brdf), (v1 - v0, material), (v2 - v0, 0)
+//           tri_shade[T][6] float4 (v0, n0.x) (v1, n0.y) (v2, n0.z) (n1, n2.x)
+//                                  (n2.y, n2.z, gn.x, gn.y) (gn.z, material, 0, 0)
+//           mats[M][2] float4      (rgb, eta), (brdf, g, 0, 0)
+//   hit points (Q): pos, refl, nrm, rng, (tau, R), (E, N) float4; tri int; (mat, t) float2
+//   photons (P):    pos, flux, dir, rng float4 (.w of pos/flux/dir = hit normal);
+//                   state int2 (triangle, bounce | emitted << 16); slot int
+//   hash (H):       count u32, representative u32 (0xffffffff = none)
+#pragma once
+
+#include "fmath.h"
+
+struct DevScene {
+    const float4* __restrict__ nodes;
+    const float4* __restrict__ tri_isect;
+    const float4* __restrict__ tri_shade;
+    const float4* __restrict__ mats;
+    const int* __restrict__ light_tri;
+    const float* __restrict__ light_cdf;
+    int num_nodes;
+    int num_lights;
+    float light_area;
+    float bsphere_x, bsphere_y, bsphere_z, bsphere_r;
+};
+
+struct DevCamera {
+    float ox, oy, oz;
+    float ux, uy, uz;
+    float vx, vy, vz;
+    float wx, wy, wz;
+    float half_w, half_h, dist;
+};
+
+// Grid scalars and pass counters kept in HBM so that no pass needs a host round trip.
+struct DevGlobals {
+    float bbmin[3];
+    float bbmax[3];
+    float initial_radius;
+    float grid_scale;
+    float hash_scale;
+    float num_photons_f;
+    unsigned long long photons_emitted;
+    unsigned int emitted_this_pass;
+    unsigned int ticket_eye;
+    unsigned int ticket_photon;
+    unsigned int pad[3];
+};
+
+struct HitState {
+    float4* pos;
+    float4* refl;
+    float4* nrm;
+    float4* rng;
+    float4* tau_r;
+    float4* e_n;
+    int* tri;
+    float2* mat_t;
+    uchar4* rgba8;
+    float* partial;  // [blocks][6] min xyz, max xyz of the eye kernel
+};
+
+struct PhotonState {
+    float4* pos;
+    float4* flux;
+    float4* dir;
+    float4* rng;
+    int2* state;
+    int* slot;
+    unsigned int* partial;  // [blocks] emitted counts
+};
+
+struct HashState {
+    unsigned int* count;
+    unsigned int* rep;
+    int size;          // H
+    int width;         // hash_width (power of two)
+    int height;
+    float num_f;       // H as float (HashNum)
+};
+
+struct EyeParams {
+    int width, height;
+    float aa_x, aa_y;
+    float focal_length, aperture;
+    int max_path;
+    int num_eye_samples;   // the NumEyeSamples uniform (count + 1)
+    float fixed_radius;
+    float footprint;
+    float footprint_width;
+};
+
+struct PhotonParams {
+    int count;         // P
+    int width;         // photon_width
+    int max_path;
+    int rep_limit;     // photons with index >= rep_limit are never representatives
+};
+
+struct GatherParams {
+    int count;         // Q
+    float alpha;
+    int first_pass;
+    int draw;
+};
+
+__global__ void k_clear_slots(HashState h);
+__global__ void k_eye(DevScene s, DevCamera cam, EyeParams ep, HitState hs, DevGlobals* g);
+__global__ void k_photon(DevScene s, PhotonParams pp, PhotonState ps, HashState h, DevGlobals* g);
+__global__ void k_gather(GatherParams gp, HitState hs, PhotonState ps, HashState h, DevGlobals* g);
+__global__ void k_resolve(int count, HitState hs, const DevGlobals* g, float4* linear, float4* mapped);
+__global__ void k_init_state(int q, int p, HitState hs, PhotonState ps);

@@ -1,0 +1,713 @@
+// Kernels of the SPPM pass loop for gfx950.  One pass (m_display, glslppm.cpp:214-536) is:
+//   k_eye      every max_path passes: eyeraytrace.fs + the max.fs/min.fs reductions + the grid
+//              uniforms of glslppm.cpp:292-311, computed by the last block to finish
+//   k_clear_slots
+//   k_photon   photontrace.fs + sum.fs + hash.fs + scatter (atomic min of the photon index)
+//              + correction (atomic count), all in the photon's own thread
+//   k_gather   progressive.fs (8-cell gather and the N/R/tau update) + re.fs draw to RGBA8
+// Arithmetic follows the GLSL statement by statement in strict fp32 (see fmath.h).
+
+#include "kernels.h"
+
+namespace {
+
+constexpr float kInf = __builtin_huge_valf();
+
+PPM_DEV int cube_face(f3 d) {
+    const float ax = fabsf(d.x), ay = fabsf(d.y), az = fabsf(d.z);
+    if (ax >= ay && ax >= az) return d.x >= 0.0f ? 0 : 1;
+    if (ay >= az) return d.y >= 0.0f ? 2 : 3;
+    return d.z >= 0.0f ? 4 : 5;
+}
+
+struct Hit {
+    int tri;    // -1: no hit
+    float t;
+    float bx, by, bz;  // Barycentric = uvt.ywx
+};
+
+// Closest hit over the threaded BVH of the ray's cube face (eyeraytrace.fs:67-254,
+// photontrace.fs:123-218).  Eye rays cull back faces on their first segment; photon rays skip
+// light triangles and start with isect.w = 1e30 instead of 1e20.
+template <bool PHOTON>
+PPM_DEV Hit trace(const DevScene& s, f3 org, f3 dir, bool cull) {
+    const int n_nodes = s.num_nodes;
+    const float4* __restrict__ nd = s.nodes + (size_t)cube_face(dir) * (size_t)n_nodes * 2;
+    const f3 rcp = mk3(1.0f / dir.x, 1.0f / dir.y, 1.0f / dir.z);
+    const f3 t0 = mk3(-rcp.x * org.x, -rcp.y * org.y, -rcp.z * org.z);
+    Hit h;
+    h.tri = -1;
+    h.t = PHOTON ? 1.0e30f : 1.0e20f;
+    h.bx = h.by = h.bz = 0.0f;
+    int i = 0;
+    while (i < n_nodes) {
+        const float4 a = nd[2 * i];
+        const float4 b = nd[2 * i + 1];
+        const float lx = rcp.x * a.x + t0.x, ly = rcp.y * a.y + t0.y, lz = rcp.z * a.z + t0.z;
+        const float hx = rcp.x * b.x + t0.x, hy = rcp.y * b.y + t0.y, hz = rcp.z * b.z + t0.z;
+        const float tmin = max1(max1(min1(lx, hx), min1(ly, hy)), min1(lz, hz));
+        const float tmax = min1(min1(max1(lx, hx), max1(ly, hy)), max1(lz, hz));
+        const bool box_hit = (tmin <= tmax) && (tmin <= h.t) && (tmax >= 0.0f);
+        const int tri = __float_as_int(b.w);
+        if (tri >= 0 && box_hit) {
+            const float4 q0 = s.tri_isect[3 * tri + 0];
+            const float4 q1 = s.tri_isect[3 * tri + 1];
+            const float4 q2 = s.tri_isect[3 * tri + 2];
+            const f3 e0 = xyz(q1), e1 = xyz(q2);
+            const f3 pv = cross3(dir, e1);
+            const float det = dot3(e0, pv);
+            if (!cull || det > 1e-10f) {
+                const f3 tv = org - xyz(q0);
+                const f3 qv = cross3(tv, e0);
+                const float u = dot3(tv, pv) / det;
+                const float v = dot3(dir, qv) / det;
+                const float t = dot3(e1, qv) / det;
+                const float w = (1.0f - u) - v;
+                bool ok = (u >= 0.0f) && (v >= 0.0f) && (t >= 0.0f) && (w >= 0.0f) && (t < h.t);
+                if (PHOTON) ok = ok && (__float_as_int(q0.w) != -1);
+                if (ok) {
+                    h.tri = tri;
+                    h.t = t;
+                    h.bx = v;
+                    h.by = w;
+                    h.bz = u;
+                }
+            }
+        }
+        i = box_hit ? i + 1 : __float_as_int(a.w);
+    }
+    return h;
+}
+
+struct Surface {
+    f3 pos, nrm, gnrm, col;
+    float eta, g;
+    int brdf, mat;
+};
+
+PPM_DEV int tri_material(const DevScene& s, int tri) { return __float_as_int(s.tri_isect[3 * tri + 1].w); }
+
+// Shading fetch after the walk (eyeraytrace.fs:256-287).
+PPM_DEV Surface shade(const DevScene& s, const Hit& h) {
+    Surface r;
+    const float4* t = s.tri_shade + (size_t)6 * h.tri;
+    const float4 a = t[0], b = t[1], c = t[2], d = t[3], e = t[4], f = t[5];
+    const f3 V0 = xyz(a), V1 = xyz(b), V2 = xyz(c);
+    const f3 N0 = mk3(a.w, b.w, c.w), N1 = xyz(d), N2 = mk3(d.w, e.x, e.y);
+    r.pos = (V2 * h.bx + V0 * h.by) + V1 * h.bz;
+    r.nrm = normalize3((N2 * h.bx + N0 * h.by) + N1 * h.bz);
+    r.gnrm = mk3(e.z, e.w, f.x);
+    r.mat = __float_as_int(f.y);
+    const float4 m0 = s.mats[2 * r.mat], m1 = s.mats[2 * r.mat + 1];
+    r.col = xyz(m0);
+    r.eta = m0.w;
+    r.brdf = (int)m1.x;
+    r.g = m1.y;
+    return r;
+}
+
+// onb (eyeraytrace.fs:293-322)
+PPM_DEV f3 onb(f3 x, f3 n) {
+    if (n.z < -0.9999999f) {
+        const f3 u = mk3(x.z, x.x, 0.0f);
+        return x.y * n - u;
+    }
+    const float a = 1.0f / (1.0f + n.z);
+    const float b = -n.x * n.y * a;
+    const f3 u = mk3(1.0f - n.x * n.x * a, b, -n.x);
+    const f3 w = mk3(b, 1.0f - n.y * n.y * a, -n.y);
+    return (x.x * u + x.y * n) + x.z * w;
+}
+
+// Fresnel (eyeraytrace.fs:350-367)
+PPM_DEV float fresnel(f3 incom, f3 normal, float ni, float ne) {
+    const float eta = ni / ne;
+    const float c1 = dot3(incom, normal);
+    float c2 = 1.0f - (eta * eta) * (1.0f - c1 * c1);
+    if (c2 < 0.0f) return 1.0f;
+    c2 = sqrtf(c2);
+    const float rs = (ni * c1 - ne * c2) / (ni * c1 + ne * c2);
+    const float rp = (ni * c2 - ne * c1) / (ni * c2 + ne * c1);
+    return (rs * rs + rp * rp) * 0.5f;
+}
+
+// glossy_reflect (eyeraytrace.fs:370-392)
+PPM_DEV f3 glossy_reflect(f3 d, f3 n, float g, float4& rndv) {
+    const f3 r = normalize3(((1.0f - g) - g) * reflect3(d, n) + (g + g) * n);
+    const float rnd1 = gpu_rnd(rndv);
+    const float rnd2 = gpu_rnd(rndv);
+    const float temp1 = 6.283184f * rnd1;
+    const float temp2_ = ppm_powf(rnd2, g);
+    const float temp2 = sqrtf(1.0f - temp2_ * temp2_);
+    const f3 v = mk3(ppm_sinf(temp1) * temp2, temp2_, ppm_cosf(temp1) * temp2);
+    f3 result = normalize3(onb(v, r));
+    if (dot3(result, n) < 0.0f) result = reflect3(result, n);
+    return result;
+}
+
+template <typename T>
+PPM_DEV T wave_min(T v) {
+    for (int o = 32; o > 0; o >>= 1) v = min1(v, __shfl_xor(v, o, 64));
+    return v;
+}
+template <typename T>
+PPM_DEV T wave_max(T v) {
+    for (int o = 32; o > 0; o >>= 1) v = max1(v, __shfl_xor(v, o, 64));
+    return v;
+}
+PPM_DEV unsigned int wave_sum(unsigned int v) {
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+
+constexpr int kBlock = 256;
+constexpr int kWaves = kBlock / 64;
+
+}  // namespace
+
+// ------------------------------------------------------------------------------------------
+
+__global__ void k_init_state(int q, int p, HitState hs, PhotonState ps) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < q) {
+        hs.e_n[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        hs.tau_r[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    }
+    if (i < p) {
+        ps.pos[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        ps.flux[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        ps.dir[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        ps.state[i] = make_int2(-1, 0);  // (-1, -1, 0, 1e30): every photon starts new
+        ps.slot[i] = -1;
+    }
+}
+
+__global__ void k_clear_slots(HashState h) {
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < h.size; i += gridDim.x * blockDim.x) {
+        h.count[i] = 0u;
+        h.rep[i] = 0xffffffffu;
+    }
+}
+
+// eyeraytrace.fs:396-554 for one pixel, then the min/max of hit positions.
+__global__ __launch_bounds__(kBlock) void k_eye(DevScene s, DevCamera cam, EyeParams ep, HitState hs,
+                                                DevGlobals* g) {
+    const int q = ep.width * ep.height;
+    const int p = blockIdx.x * kBlock + threadIdx.x;
+    f3 pos = mk3(0.0f, 0.0f, 0.0f);
+    const bool active = p < q;
+    if (active) {
+        const int px_i = p % ep.width, px_j = p / ep.width;
+        float4 rndv = hs.rng[p];
+        const f3 C = mk3(cam.ox, cam.oy, cam.oz);
+        const f3 U = mk3(cam.ux, cam.uy, cam.uz), V = mk3(cam.vx, cam.vy, cam.vz), W = mk3(cam.wx, cam.wy, cam.wz);
+        const float ppx = ((float)px_i + 0.5f + ep.aa_x) - cam.half_w;
+        const float ppy = ((float)px_j + 0.5f + ep.aa_y) - cam.half_h;
+        const f3 rel = (ppx * U + ppy * V) + cam.dist * W;
+        f3 dir = normalize3(rel);
+        const float filter = (1.0f - fabsf(ep.aa_x) / 1.5f) * (1.0f - fabsf(ep.aa_y) / 1.5f);
+        const f3 fp = C + dir * ep.focal_length;
+        const float radius = sqrtf(gpu_rnd(rndv)) * ep.aperture;
+        const float theta = 6.283184f * gpu_rnd(rndv);
+        const f3 lens = C + radius * (U * ppm_cosf(theta) + V * ppm_sinf(theta));
+        dir = normalize3(fp - lens);
+        f3 org = lens;
+        f3 col = mk3(filter, filter, filter);
+        f3 nrm = mk3(0.0f, 0.0f, 0.0f);
+        f3 emi = mk3(0.0f, 0.0f, 0.0f);
+        const float eps = 1e-5f;
+        int last_tri = -1, last_mat = -1;
+        float last_t = 1.0e20f;
+        for (int j = 0; j < ep.max_path; j++) {
+            const Hit h = trace<false>(s, org, dir, j == 0);
+            last_tri = h.tri;
+            last_t = h.t;
+            if (h.tri < 0) {
+                last_mat = -1;
+                nrm = mk3(0.0f, 0.0f, 0.0f);
+                if (s.light_area == 0.0f) emi = mk3(0.7f, 0.7f, 1.0f) * col;
+                break;
+            }
+            const Surface i = shade(s, h);
+            last_mat = i.mat;
+            pos = i.pos;
+            nrm = i.nrm;
+            if (i.brdf == -1) {
+                nrm = mk3(0.0f, 0.0f, 0.0f);
+                emi = col * i.col;
+                break;
+            } else if (i.brdf == 0) {
+                col = col * i.col;
+                break;
+            } else if (i.brdf == 1 || i.brdf == 4) {
+                org = pos + eps * i.gnrm;
+                if (i.brdf == 4) nrm = glossy_reflect(nrm, nrm, i.g, rndv);
+                dir = reflect3(dir, nrm);
+                if (dot3(dir, i.gnrm) < 0.0f) dir = -dir;
+                col = col * i.col;
+            } else if (i.brdf == 2 || i.brdf == 5) {
+                if (i.brdf == 5) nrm = glossy_reflect(nrm, nrm, i.g, rndv);
+                const float ln = dot3(nrm, dir);
+                const float eta = i.eta;
+                if (ln < 0.0f) {
+                    const float re = fresnel(-dir, nrm, 1.0f, eta);
+                    if (gpu_rnd(rndv) < re) {
+                        org = pos + eps * i.gnrm;
+                        dir = reflect3(dir, nrm);
+                        if (dot3(dir, i.gnrm) < 0.0f) dir = -dir;
+                    } else {
+                        col = col * i.col;
+                        org = pos - eps * i.gnrm;
+                        dir = refract3(dir, nrm, 1.0f / eta);
+                        if (dot3(dir, i.gnrm) > 0.0f) dir = -dir;
+                    }
+                } else {
+                    const float re = fresnel(-dir, -nrm, eta, 1.0f);
+                    col = col * i.col;
+                    if (gpu_rnd(rndv) < re) {
+                        org = pos - eps * i.gnrm;
+                        dir = reflect3(dir, -nrm);
+                        if (dot3(dir, -i.gnrm) < 0.0f) dir = -dir;
+                    } else {
+                        org = pos + eps * i.gnrm;
+                        dir = refract3(dir, -nrm, eta);
+                        if (dot3(dir, -i.gnrm) > 0.0f) dir = -dir;
+                    }
+                }
+            } else if (i.brdf == 3 || i.brdf == 6) {
+                if (i.brdf == 6) nrm = glossy_reflect(nrm, nrm, i.g, rndv);
+                const float re = fresnel(-dir, nrm, 1.0f, i.eta);
+                if (gpu_rnd(rndv) < re) {
+                    org = pos + eps * i.gnrm;
+                    dir = reflect3(dir, nrm);
+                    if (dot3(dir, i.gnrm) < 0.0f) dir = -dir;
+                } else {
+                    col = col * i.col;
+                    break;
+                }
+            }
+        }
+        float4 en = hs.e_n[p];
+        const float a = 1.0f / (float)ep.num_eye_samples;
+        en.x = en.x * (1.0f - a) + emi.x * a;
+        en.y = en.y * (1.0f - a) + emi.y * a;
+        en.z = en.z * (1.0f - a) + emi.z * a;
+        hs.pos[p] = make_float4(pos.x, pos.y, pos.z, dir.x);
+        hs.refl[p] = make_float4(col.x, col.y, col.z, dir.y);
+        hs.nrm[p] = make_float4(nrm.x, nrm.y, nrm.z, dir.z);
+        hs.rng[p] = rndv;
+        hs.tri[p] = last_tri;
+        hs.mat_t[p] = make_float2(last_mat < 0 ? -1.0f : (float)last_mat, last_t);
+        hs.e_n[p] = en;
+    }
+
+    // max.fs / min.fs over every pixel's position (misses contribute their last position)
+    __shared__ float red[kWaves][6];
+    __shared__ bool last_block;
+    float v[6] = {active ? pos.x : kInf, active ? pos.y : kInf, active ? pos.z : kInf,
+                  active ? pos.x : -kInf, active ? pos.y : -kInf, active ? pos.z : -kInf};
+    for (int k = 0; k < 3; k++) v[k] = wave_min(v[k]);
+    for (int k = 3; k < 6; k++) v[k] = wave_max(v[k]);
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    if (lane == 0)
+        for (int k = 0; k < 6; k++) red[wave][k] = v[k];
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        float r[6];
+        for (int k = 0; k < 6; k++) r[k] = red[0][k];
+        for (int w = 1; w < kWaves; w++) {
+            for (int k = 0; k < 3; k++) r[k] = min1(r[k], red[w][k]);
+            for (int k = 3; k < 6; k++) r[k] = max1(r[k], red[w][k]);
+        }
+        for (int k = 0; k < 6; k++) hs.partial[blockIdx.x * 6 + k] = r[k];
+        __threadfence();
+        const unsigned int t = atomicAdd(&g->ticket_eye, 1u);
+        last_block = (t == gridDim.x - 1);
+    }
+    __syncthreads();
+    if (!last_block) return;
+    __threadfence();
+    float r[6] = {kInf, kInf, kInf, -kInf, -kInf, -kInf};
+    for (int b = threadIdx.x; b < (int)gridDim.x; b += kBlock) {
+        const volatile float* pp = hs.partial + b * 6;
+        for (int k = 0; k < 3; k++) r[k] = min1(r[k], pp[k]);
+        for (int k = 3; k < 6; k++) r[k] = max1(r[k], pp[k]);
+    }
+    for (int k = 0; k < 3; k++) r[k] = wave_min(r[k]);
+    for (int k = 3; k < 6; k++) r[k] = wave_max(r[k]);
+    __syncthreads();
+    if (lane == 0)
+        for (int k = 0; k < 6; k++) red[wave][k] = r[k];
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        float bmin[3], bmax[3];
+        for (int k = 0; k < 3; k++) {
+            bmin[k] = red[0][k];
+            bmax[k] = red[0][k + 3];
+            for (int w = 1; w < kWaves; w++) {
+                bmin[k] = min1(bmin[k], red[w][k]);
+                bmax[k] = max1(bmax[k], red[w][k + 3]);
+            }
+        }
+        // glslppm.cpp:294-311
+        float bbsize = 0.0f;
+        for (int k = 0; k < 3; k++) bbsize += bmax[k] - bmin[k];
+        float ir = (bbsize / 3.0f) / ep.footprint_width * ep.footprint;
+        if (ep.fixed_radius > 0.0f) ir = ep.fixed_radius;
+        for (int k = 0; k < 3; k++) {
+            g->bbmin[k] = bmin[k] - ir;
+            g->bbmax[k] = bmax[k] + ir;
+        }
+        const float gs = 0.5f / ir;
+        g->initial_radius = ir;
+        g->grid_scale = gs;
+        g->hash_scale = 4194304.0f / gs;
+        g->ticket_eye = 0u;
+        __threadfence();
+    }
+}
+
+namespace {
+
+// GenerateLightSourceSample (photontrace.fs:345-440)
+PPM_DEV void light_sample(const DevScene& s, float4& rndv, f3& flux, f3& dir, f3& org) {
+    const float rnd = gpu_rnd(rndv);
+    // lower bound over the texels at (mid - 0.5) / n, i.e. cdf[max(mid - 1, 0)]
+    int low = 0, high = s.num_lights;
+    while (low < high) {
+        const int mid = low + ((high - low) / 2);
+        const float c = s.light_cdf[mid > 0 ? mid - 1 : 0];
+        if (c < rnd) low = mid + 1;
+        else high = mid;
+    }
+    const int li = s.light_tri[high > 0 ? high - 1 : 0];
+    const float rndv2 = gpu_rnd(rndv);
+    const float t = sqrtf(1.0f - rndv2);
+    const float sv = gpu_rnd(rndv);
+    const float a = 1.0f - t;
+    const float c = sv * t;
+    const float b = t - c;
+    const float4* tr = s.tri_shade + (size_t)6 * li;
+    const float4 q0 = tr[0], q1 = tr[1], q2 = tr[2], q3 = tr[3], q4 = tr[4], q5 = tr[5];
+    const f3 V0 = xyz(q0), V1 = xyz(q1), V2 = xyz(q2);
+    const f3 N0 = mk3(q0.w, q1.w, q2.w), N1 = xyz(q3), N2 = mk3(q3.w, q4.x, q4.y);
+    const int mat = __float_as_int(q5.y);
+    const float4 m0 = s.mats[2 * mat];
+    const float le = s.light_area * 3.141592f;
+    flux = mk3(m0.x * le, m0.y * le, m0.z * le);
+    org = (V0 * a + V1 * b) + V2 * c;
+    const f3 nrm = normalize3((N0 * a + N1 * b) + N2 * c);
+    const f3 gnrm = mk3(q4.z, q4.w, q5.x);
+    org = org + gnrm * 1.0e-5f;
+    float rx = gpu_rnd(rndv);
+    float ry = gpu_rnd(rndv);
+    rx = 6.283184f * rx;
+    ry = sqrtf(ry);
+    dir = onb(mk3(ppm_sinf(rx) * ry, sqrtf(1.0f - ry * ry), ppm_cosf(rx) * ry), nrm);
+}
+
+// GenerateIBLSample (photontrace.fs:283-341), used when the scene has no light triangle
+PPM_DEV void ibl_sample(const DevScene& s, float4& rndv, f3& flux, f3& dir, f3& org) {
+    const float area = 12.566368f * s.bsphere_r * s.bsphere_r;
+    const float temp2 = 6.283184f * gpu_rnd(rndv);
+    const float aaa = sqrtf(1.0f - gpu_rnd(rndv));
+    const float sin_a = ppm_sinf(aaa);
+    const float sin_t1 = (sin_a + sin_a) * ppm_cosf(aaa);
+    dir.x = sin_t1 * ppm_cosf(temp2);
+    dir.y = 1.0f - (sin_a + sin_a) * sin_a;
+    dir.z = sin_t1 * ppm_sinf(temp2);
+    if (gpu_rnd(rndv) > 0.5f) {
+        dir = mk3(1.0f, 1.0f, 1.0f);
+        dir.y = dir.y + dir.y;
+        dir.z = -dir.z;
+        dir = normalize3(dir);
+        flux = mk3(1.0f * area, 1.0f * area, 0.7f * area);
+    } else {
+        flux = mk3(0.7f * area, 0.7f * area, 1.0f * area);
+    }
+    const float radius = sqrtf(gpu_rnd(rndv));
+    const float theta = 6.283184f * gpu_rnd(rndv);
+    const f3 temp = mk3(radius * ppm_cosf(theta), 0.0f, radius * ppm_sinf(theta));
+    org = (onb(temp * s.bsphere_r, dir) + dir * s.bsphere_r) + mk3(s.bsphere_x, s.bsphere_y, s.bsphere_z);
+    dir = -dir;
+}
+
+}  // namespace
+
+// photontrace.fs:464-720 for one photon slot, then hash.fs and the scatter/correction draws.
+__global__ __launch_bounds__(kBlock) void k_photon(DevScene s, PhotonParams pp, PhotonState ps, HashState h,
+                                                   DevGlobals* g) {
+    const int p = blockIdx.x * kBlock + threadIdx.x;
+    unsigned int emitted = 0;
+    if (p < pp.count) {
+        float4 rndv = ps.rng[p];
+        const float4 ppos = ps.pos[p], pflux = ps.flux[p], pdir = ps.dir[p];
+        const f3 prev_nrm = mk3(ppos.w, pflux.w, pdir.w);
+        f3 flux = mk3(fabsf(pflux.x), fabsf(pflux.y), fabsf(pflux.z));
+        const int2 st = ps.state[p];
+        const int bounce = (st.y & 0xffff) + 1;
+        f3 org = mk3(0.0f, 0.0f, 0.0f), dir = mk3(0.0f, 0.0f, 0.0f);
+        bool cont;
+        int out_bounce = bounce;
+        if (st.x < 0 || bounce >= pp.max_path) {
+            if (s.light_area != 0.0f) light_sample(s, rndv, flux, dir, org);
+            else ibl_sample(s, rndv, flux, dir, org);
+            emitted = 1;
+            out_bounce = 0;
+            cont = true;
+        } else {
+            const float4* tr = s.tri_shade + (size_t)6 * st.x;
+            const float4 q4 = tr[4], q5 = tr[5];
+            const f3 gnrm = mk3(q4.z, q4.w, q5.x);
+            const int mat = __float_as_int(q5.y);
+            const float4 m0 = s.mats[2 * mat], m1 = s.mats[2 * mat + 1];
+            f3 col = xyz(m0);
+            const int brdf = (int)m1.x;
+            const float eta = m0.w, gl = m1.y;
+            const f3 ipos = xyz(ppos);
+            f3 inrm = prev_nrm;
+            const f3 pd = xyz(pdir);
+            const float eps = 1e-6f;
+            if (brdf == 0) {
+                const float r0 = 6.283184f * gpu_rnd(rndv);
+                const float r1 = sqrtf(gpu_rnd(rndv));
+                const f3 v = mk3(ppm_sinf(r0) * r1, sqrtf(1.0f - r1 * r1), ppm_cosf(r0) * r1);
+                org = ipos + eps * gnrm;
+                dir = onb(v, inrm);
+                if (dot3(dir, gnrm) < 0.0f) dir = -dir;
+            } else if (brdf == 1 || brdf == 4) {
+                org = ipos + eps * gnrm;
+                if (brdf == 4) inrm = glossy_reflect(inrm, inrm, gl, rndv);
+                dir = reflect3(pd, inrm);
+                if (dot3(dir, gnrm) < 0.0f) dir = -dir;
+            } else if (brdf == 2 || brdf == 5) {
+                if (brdf == 5) inrm = glossy_reflect(inrm, inrm, gl, rndv);
+                const float ln = dot3(inrm, pd);
+                if (ln < 0.0f) {
+                    const float re = fresnel(-pd, inrm, 1.0f, eta);
+                    if (gpu_rnd(rndv) < re) {
+                        org = ipos + eps * gnrm;
+                        dir = reflect3(pd, inrm);
+                        col = mk3(1.0f, 1.0f, 1.0f);
+                        if (dot3(dir, gnrm) < 0.0f) dir = -dir;
+                    } else {
+                        org = ipos - eps * gnrm;
+                        dir = refract3(pd, inrm, 1.0f / eta);
+                        if (dot3(dir, gnrm) > 0.0f) dir = -dir;
+                    }
+                } else {
+                    const float re = fresnel(-pd, -inrm, eta, 1.0f);
+                    if (gpu_rnd(rndv) < re) {
+                        org = ipos - eps * gnrm;
+                        dir = reflect3(pd, -inrm);
+                        if (dot3(dir, -gnrm) < 0.0f) dir = -dir;
+                    } else {
+                        org = ipos + eps * gnrm;
+                        dir = refract3(pd, -inrm, eta);
+                        if (dot3(dir, -gnrm) > 0.0f) dir = -dir;
+                    }
+                }
+            } else if (brdf == 3 || brdf == 6) {
+                if (brdf == 6) inrm = glossy_reflect(inrm, inrm, gl, rndv);
+                const float re = fresnel(-pd, inrm, 1.0f, eta);
+                if (gpu_rnd(rndv) < re) {
+                    org = ipos + eps * gnrm;
+                    dir = reflect3(pd, inrm);
+                    col = mk3(1.0f, 1.0f, 1.0f);
+                    if (dot3(dir, gnrm) < 0.0f) dir = -dir;
+                } else {
+                    const float r0 = 6.283184f * gpu_rnd(rndv);
+                    const float r1 = sqrtf(gpu_rnd(rndv));
+                    const f3 v = mk3(ppm_sinf(r0) * r1, sqrtf(1.0f - r1 * r1), ppm_cosf(r0) * r1);
+                    org = ipos + eps * gnrm;
+                    dir = onb(v, inrm);
+                    if (dot3(dir, gnrm) < 0.0f) dir = -dir;
+                }
+            }
+            // Russian roulette
+            const float pr = max1(max1(col.x, col.y), col.z);
+            if (pr < gpu_rnd(rndv)) {
+                cont = false;
+            } else {
+                cont = true;
+                col = col / pr;
+                flux = flux * col;
+            }
+        }
+
+        int slot = -1;
+        if (cont) {
+            const Hit hh = trace<true>(s, org, dir, false);
+            f3 hpos = mk3(0.0f, 0.0f, 0.0f), hnrm = mk3(0.0f, 0.0f, 0.0f), hgn = mk3(0.0f, 0.0f, 0.0f);
+            int hbrdf = 0;
+            if (hh.tri >= 0) {
+                const Surface i = shade(s, hh);
+                hpos = i.pos;
+                hnrm = i.nrm;
+                hgn = i.gnrm;
+                hbrdf = i.brdf;
+            }
+            if (hh.tri < 0 || (hbrdf != 0 && hbrdf != 6 && hbrdf != 3)) flux = -flux;
+            // a miss has no geometric normal: 0/0 is NaN and the NaN-rejecting min gives sqrt(5)
+            const float gf = min1(fabsf(dot3(dir, hnrm) / dot3(dir, hgn)), sqrtf(5.0f));
+            flux = flux * gf;
+            ps.pos[p] = make_float4(hpos.x, hpos.y, hpos.z, hnrm.x);
+            ps.flux[p] = make_float4(flux.x, flux.y, flux.z, hnrm.y);
+            ps.dir[p] = make_float4(dir.x, dir.y, dir.z, hnrm.z);
+            ps.state[p] = make_int2(hh.tri, out_bounce | (int)(emitted << 16));
+            // hash.fs: deposits with flux.x >= 0 (including -0) go to the grid
+            if (!(flux.x < 0.0f)) {
+                const f3 cell = mk3(floorf((hpos.x - g->bbmin[0]) * g->grid_scale),
+                                    floorf((hpos.y - g->bbmin[1]) * g->grid_scale),
+                                    floorf((hpos.z - g->bbmin[2]) * g->grid_scale));
+                const float sf = cell_hash(cell, g->grid_scale, g->hash_scale, h.num_f);
+                // a slot past the last row lands outside the hash viewport and is clipped
+                if (sf < h.num_f) {
+                    slot = (int)sf;
+                    atomicAdd(h.count + slot, 1u);
+                    if (p < pp.rep_limit) atomicMin(h.rep + slot, (unsigned int)p);
+                }
+            }
+        } else {
+            ps.pos[p] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+            ps.flux[p] = make_float4(-1.0f, -1.0f, -1.0f, 0.0f);
+            ps.dir[p] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+            ps.state[p] = make_int2(-1, (int)(emitted << 16));
+        }
+        ps.rng[p] = rndv;
+        ps.slot[p] = slot;
+    }
+
+    // sum.fs over the emitted flags, NumPhotons += int(sum) by the last block (glslppm.cpp:396-397)
+    __shared__ unsigned int red[kWaves];
+    __shared__ bool last_block;
+    const unsigned int ws = wave_sum(emitted);
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = ws;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        unsigned int tot = 0;
+        for (int w = 0; w < kWaves; w++) tot += red[w];
+        ps.partial[blockIdx.x] = tot;
+        __threadfence();
+        const unsigned int t = atomicAdd(&g->ticket_photon, 1u);
+        last_block = (t == gridDim.x - 1);
+    }
+    __syncthreads();
+    if (!last_block) return;
+    __threadfence();
+    unsigned int sum = 0;
+    for (int b = threadIdx.x; b < (int)gridDim.x; b += kBlock) sum += ((volatile unsigned int*)ps.partial)[b];
+    sum = wave_sum(sum);
+    __syncthreads();
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = sum;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        unsigned int tot = 0;
+        for (int w = 0; w < kWaves; w++) tot += red[w];
+        g->num_photons_f = g->num_photons_f + (float)(int)tot;
+        g->photons_emitted += tot;
+        g->emitted_this_pass = tot;
+        g->ticket_photon = 0u;
+        __threadfence();
+    }
+}
+
+namespace {
+
+// re.fs: tone map of one channel
+PPM_DEV float srgb(float c) {
+    if (c < 0.0031308f) return 12.92f * c;
+    const float a = 0.055f;
+    return (1.0f + a) * ppm_powf(c, 0.416666666667f) - a;
+}
+
+PPM_DEV void radiance(float4 tr, float4 en, float np, float4& lin, float4& mapped) {
+    const float den = tr.w * tr.w * 3.141592f * np;
+    lin = make_float4(tr.x / den + en.x, tr.y / den + en.y, tr.z / den + en.z, 1.0f + 0.0f);
+    const float ex = 60000.0f;
+    mapped.x = srgb(1.0f - ppm_expf(-lin.x * ex));
+    mapped.y = srgb(1.0f - ppm_expf(-lin.y * ex));
+    mapped.z = srgb(1.0f - ppm_expf(-lin.z * ex));
+    mapped.w = 1.0f - ppm_expf(-lin.w * ex);
+}
+
+PPM_DEV unsigned char unorm8(float c) {
+    c = c > 1.0f ? 1.0f : c;
+    c = c > 0.0f ? c : 0.0f;
+    return (unsigned char)(int)(c * 255.0f + 0.5f);
+}
+
+}  // namespace
+
+// progressive.fs:98-142 for one hit point, then the re.fs draw of that pixel.
+__global__ __launch_bounds__(kBlock) void k_gather(GatherParams gp, HitState hs, PhotonState ps, HashState h,
+                                                   DevGlobals* g) {
+    const int p = blockIdx.x * kBlock + threadIdx.x;
+    if (p >= gp.count) return;
+    float4 tr = gp.first_pass ? make_float4(0.0f, 0.0f, 0.0f, g->initial_radius) : hs.tau_r[p];
+    float4 en = hs.e_n[p];
+    if (hs.tri[p] >= 0) {
+        const f3 qpos = xyz(hs.pos[p]);
+        const f3 qnrm = xyz(hs.nrm[p]);
+        const f3 refl = xyz(hs.refl[p]);
+        const float R = tr.w;
+        const float N = en.w;
+        const f3 bbmin = mk3(g->bbmin[0], g->bbmin[1], g->bbmin[2]);
+        const float gs = g->grid_scale, hsc = g->hash_scale;
+        const f3 d = qpos - bbmin;
+        const f3 ii = mk3(floorf(fabsf(d.x) * gs - 0.5f), floorf(fabsf(d.y) * gs - 0.5f),
+                          floorf(fabsf(d.z) * gs - 0.5f));
+        f3 flux = mk3(0.0f, 0.0f, 0.0f);
+        float m = 0.0f;
+        const int clamp_slot = (h.height - 1) * h.width;
+        for (int iz = 0; iz <= 1; iz++)
+            for (int iy = 0; iy <= 1; iy++)
+                for (int ix = 0; ix <= 1; ix++) {
+                    const f3 cell = mk3(ii.x + (float)ix, ii.y + (float)iy, ii.z + (float)iz);
+                    const float sf = cell_hash(cell, gs, hsc, h.num_f);
+                    // texel fetch clamps a slot past the last row to the start of that row
+                    const int slot = sf < h.num_f ? (int)sf : clamp_slot;
+                    const unsigned int cnt = h.count[slot];
+                    if (cnt == 0u) continue;  // adds exactly +0 to flux and count
+                    unsigned int rep = h.rep[slot];
+                    if (rep == 0xffffffffu) rep = 0u;  // cleared slot reads photon texel (0, 0)
+                    const f3 ppos = xyz(ps.pos[rep]);
+                    const f3 rmin = cell / gs + bbmin;
+                    const f3 rmax = (cell + mk3(1.0f, 1.0f, 1.0f)) / gs + bbmin;
+                    if (ppos.x > rmin.x && ppos.y > rmin.y && ppos.z > rmin.z && ppos.x < rmax.x &&
+                        ppos.y < rmax.y && ppos.z < rmax.z) {
+                        const f3 diff = ppos - qpos;
+                        const f3 pdir = xyz(ps.dir[rep]);
+                        if (length3(diff) < R && dot3(qnrm, -pdir) > 0.0f) {
+                            const float corr = (float)cnt;
+                            flux = flux + xyz(ps.flux[rep]) * corr;
+                            m = m + corr;
+                        }
+                    }
+                }
+        flux = flux * (refl * 0.31830995240629591621063460818591f);
+        const float gg = min1((N + gp.alpha * m) / (N + m), 1.0f);
+        tr.w = R * sqrtf(gg);
+        en.w = N + m * gp.alpha;
+        tr.x = (tr.x + flux.x) * gg;
+        tr.y = (tr.y + flux.y) * gg;
+        tr.z = (tr.z + flux.z) * gg;
+    }
+    hs.tau_r[p] = tr;
+    hs.e_n[p] = en;
+    if (gp.draw) {
+        float4 lin, mapped;
+        radiance(tr, en, g->num_photons_f, lin, mapped);
+        hs.rgba8[p] = make_uchar4(unorm8(mapped.x), unorm8(mapped.y), unorm8(mapped.z), unorm8(mapped.w));
+    }
+}
+
+__global__ void k_resolve(int count, HitState hs, const DevGlobals* g, float4* linear, float4* mapped) {
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= count) return;
+    float4 lin, mp;
+    radiance(hs.tau_r[p], hs.e_n[p], g->num_photons_f, lin, mp);
+    linear[p] = lin;
+    mapped[p] = mp;
+    hs.rgba8[p] = make_uchar4(unorm8(mp.x), unorm8(mp.y), unorm8(mp.z), unorm8(mp.w));
+}

@@ -1,0 +1,674 @@
+// Context, HBM allocation, pass sequencing and readback behind include/ppm.h.
+//
+// ppm_create replaces the texture/FBO set-up of main() (glslppm.cpp:664-829) and the scene
+// packing of CGPURT::CreateTextures (gpurt.cpp:47-552); ppm_pass replaces m_display()
+// (glslppm.cpp:214-536); ppm_readback replaces glReadPixels and reading the state textures.
+// Everything a pass needs (grid scalars, photon counters) stays in HBM, so ppm_pass only queues
+// kernels on the context's stream and never waits for the device.
+
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <new>
+#include <string>
+#include <vector>
+
+#include "../../include/ppm.h"
+#include "kernels.h"
+
+struct ppm_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    bool timed = false;
+    ppm_params prm;
+    int Q = 0, P = 0, H = 0, N = 0, T = 0, M = 0, NL = 0;
+    int eye_blocks = 0, photon_blocks = 0;
+    DevScene scene;
+    DevCamera cam;
+    HitState hit;
+    PhotonState ph;
+    HashState hash;
+    DevGlobals* glob = nullptr;
+    float4* resolve_lin = nullptr;
+    float4* resolve_map = nullptr;
+    std::vector<void*> allocs;
+    // per-kernel event timing (ppm_profile_enable)
+    bool profiling = false;
+    std::vector<hipEvent_t> ev_pool;
+    std::vector<std::pair<int, std::pair<hipEvent_t, hipEvent_t>>> ev_pending;
+    double prof_ms[PPM_KERNEL_KINDS] = {0, 0, 0, 0};
+    uint64_t prof_n[PPM_KERNEL_KINDS] = {0, 0, 0, 0};
+    uint64_t frame = 0, eye_samples = 0;
+    uint32_t xs[4];
+    std::string err;
+};
+
+namespace {
+
+thread_local std::string g_create_error;
+
+int set_err(ppm_ctx* c, int code, const std::string& msg) {
+    if (c) c->err = msg;
+    else g_create_error = msg;
+    return code;
+}
+
+#define HIP_TRY(ctx, call)                                                                   \
+    do {                                                                                     \
+        hipError_t e_ = (call);                                                              \
+        if (e_ != hipSuccess)                                                                \
+            return set_err((ctx), e_ == hipErrorOutOfMemory ? PPM_E_OOM : PPM_E_HIP,         \
+                           std::string(#call) + ": " + hipGetErrorString(e_));               \
+    } while (0)
+
+template <typename T>
+int dev_alloc(ppm_ctx* c, T** out, size_t count) {
+    void* p = nullptr;
+    size_t bytes = count * sizeof(T);
+    if (bytes == 0) bytes = 16;
+    hipError_t e = hipMalloc(&p, bytes);
+    if (e != hipSuccess)
+        return set_err(c, e == hipErrorOutOfMemory ? PPM_E_OOM : PPM_E_HIP,
+                       std::string("hipMalloc(") + std::to_string(bytes) + "): " + hipGetErrorString(e));
+    c->allocs.push_back(p);
+    *out = (T*)p;
+    return PPM_OK;
+}
+
+template <typename T>
+int upload(ppm_ctx* c, T** out, const std::vector<T>& v) {
+    int rc = dev_alloc(c, out, v.size());
+    if (rc) return rc;
+    if (!v.empty()) HIP_TRY(c, hipMemcpy(*out, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice));
+    return PPM_OK;
+}
+
+inline float as_float(int v) {
+    float f;
+    std::memcpy(&f, &v, 4);
+    return f;
+}
+
+// XORShift::m_frand (glslppm.cpp:77-91)
+inline float xs_frand(uint32_t* s) {
+    const uint32_t t = s[0] ^ (s[0] << 11);
+    s[0] = s[1];
+    s[1] = s[2];
+    s[2] = s[3];
+    s[3] = (s[3] ^ (s[3] >> 19)) ^ (t ^ (t >> 8));
+    return (float)s[3] * (1.0f / 4294967295.0f);
+}
+
+// RandomizeTextures (glslppm.cpp:113-140): four draws per texel scaled by 2^22.  g++ evaluates
+// the TVector4 constructor arguments last to first, so w takes the first draw and x the fourth.
+void seed_grid(uint32_t* s, std::vector<float4>& out, size_t n) {
+    out.resize(n);
+    for (size_t i = 0; i < n; i++) {
+        const float w = xs_frand(s) * 4194304.0f;
+        const float z = xs_frand(s) * 4194304.0f;
+        const float y = xs_frand(s) * 4194304.0f;
+        const float x = xs_frand(s) * 4194304.0f;
+        out[i] = make_float4(x, y, z, w);
+    }
+}
+
+bool is_pow2(int v) { return v > 0 && (v & (v - 1)) == 0; }
+
+int check_scene(const ppm_scene* s) {
+    if (!s) return set_err(nullptr, PPM_E_INVAL, "null scene");
+    if (s->num_triangles <= 0 || !s->positions || !s->normals || !s->material)
+        return set_err(nullptr, PPM_E_INVAL, "scene has no triangles");
+    if (s->num_nodes != 2 * s->num_triangles - 1 || !s->node_bounds || !s->node_triangle || !s->node_links)
+        return set_err(nullptr, PPM_E_INVAL, "scene BVH must have 2T-1 nodes");
+    if (s->num_materials <= 0 || !s->materials) return set_err(nullptr, PPM_E_INVAL, "scene has no materials");
+    if (s->num_lights < 0 || (s->num_lights > 0 && (!s->light_triangle || !s->light_cdf)))
+        return set_err(nullptr, PPM_E_INVAL, "bad light table");
+    for (int i = 0; i < s->num_triangles; i++)
+        if (s->material[i] < 0 || s->material[i] >= s->num_materials)
+            return set_err(nullptr, PPM_E_INVAL, "triangle material out of range");
+    for (int k = 0; k < s->num_nodes; k++)
+        if (s->node_triangle[k] < -1 || s->node_triangle[k] >= s->num_triangles)
+            return set_err(nullptr, PPM_E_INVAL, "leaf triangle out of range");
+    for (size_t k = 0; k < (size_t)6 * s->num_nodes * 2; k++)
+        if (s->node_links[k] < -1 || s->node_links[k] >= s->num_nodes)
+            return set_err(nullptr, PPM_E_INVAL, "BVH link out of range");
+    for (int l = 0; l < s->num_lights; l++)
+        if (s->light_triangle[l] < 0 || s->light_triangle[l] >= s->num_triangles)
+            return set_err(nullptr, PPM_E_INVAL, "light triangle out of range");
+    return PPM_OK;
+}
+
+// Per-face pre-order node records; the hit link of position i is i + 1.
+int pack_nodes(const ppm_scene* s, std::vector<float4>& out) {
+    const int N = s->num_nodes;
+    out.assign((size_t)6 * N * 2, make_float4(0, 0, 0, 0));
+    std::vector<int> order(N), pos(N);
+    for (int f = 0; f < 6; f++) {
+        const int32_t* L = s->node_links + (size_t)f * N * 2;
+        std::fill(pos.begin(), pos.end(), -1);
+        int k = 0, c = 0;
+        while (k >= 0) {
+            if (c >= N || pos[k] >= 0) return set_err(nullptr, PPM_E_INVAL, "BVH hit links do not form one chain");
+            order[c] = k;
+            pos[k] = c++;
+            k = L[2 * k];
+        }
+        if (c != N) return set_err(nullptr, PPM_E_INVAL, "BVH hit chain misses nodes");
+        for (int i = 0; i < N; i++) {
+            const int nk = order[i];
+            const int miss = L[2 * nk + 1];
+            const int mpos = miss < 0 ? N : pos[miss];
+            if (miss >= 0 && mpos <= i) return set_err(nullptr, PPM_E_INVAL, "BVH miss link points backwards");
+            const float* b = s->node_bounds + (size_t)nk * 6;
+            out[((size_t)f * N + i) * 2 + 0] = make_float4(b[0], b[1], b[2], as_float(mpos));
+            out[((size_t)f * N + i) * 2 + 1] = make_float4(b[3], b[4], b[5], as_float(s->node_triangle[nk]));
+        }
+    }
+    return PPM_OK;
+}
+
+// Triangle records of textureTriangles (gpurt.cpp:115-162) in the form the kernels read:
+// normal z as the shaders rebuild it, sign(sgn(n.z)(mat+1)) sqrt(|1 - x^2 - y^2|)
+// (eyeraytrace.fs:268-270), and the geometric normal normalize(cross(v1-v0, v2-v0)).
+void pack_triangles(const ppm_scene* s, std::vector<float4>& isect, std::vector<float4>& shade) {
+    const int T = s->num_triangles;
+    isect.resize((size_t)T * 3);
+    shade.resize((size_t)T * 6);
+    for (int t = 0; t < T; t++) {
+        const float* p = s->positions + (size_t)t * 9;
+        const float* n = s->normals + (size_t)t * 9;
+        const int mat = s->material[t];
+        const int brdf = s->materials[mat].brdf;
+        float e0[3], e1[3], rn[3][3];
+        for (int k = 0; k < 3; k++) {
+            e0[k] = p[3 + k] - p[k];
+            e1[k] = p[6 + k] - p[k];
+        }
+        for (int v = 0; v < 3; v++) {
+            const float nx = n[v * 3], ny = n[v * 3 + 1], nz = n[v * 3 + 2];
+            rn[v][0] = nx;
+            rn[v][1] = ny;
+            rn[v][2] = (nz > 0.0f ? 1.0f : -1.0f) * std::sqrt(std::fabs(1.0f - nx * nx - ny * ny));
+        }
+        float c[3] = {e0[1] * e1[2] - e0[2] * e1[1], e0[2] * e1[0] - e0[0] * e1[2], e0[0] * e1[1] - e0[1] * e1[0]};
+        const float inv = 1.0f / std::sqrt(c[0] * c[0] + c[1] * c[1] + c[2] * c[2]);
+        const float gn[3] = {c[0] * inv, c[1] * inv, c[2] * inv};
+        isect[(size_t)t * 3 + 0] = make_float4(p[0], p[1], p[2], as_float(brdf));
+        isect[(size_t)t * 3 + 1] = make_float4(e0[0], e0[1], e0[2], as_float(mat));
+        isect[(size_t)t * 3 + 2] = make_float4(e1[0], e1[1], e1[2], 0.0f);
+        float4* o = &shade[(size_t)t * 6];
+        o[0] = make_float4(p[0], p[1], p[2], rn[0][0]);
+        o[1] = make_float4(p[3], p[4], p[5], rn[0][1]);
+        o[2] = make_float4(p[6], p[7], p[8], rn[0][2]);
+        o[3] = make_float4(rn[1][0], rn[1][1], rn[1][2], rn[2][0]);
+        o[4] = make_float4(rn[2][1], rn[2][2], gn[0], gn[1]);
+        o[5] = make_float4(gn[2], as_float(mat), 0.0f, 0.0f);
+    }
+}
+
+hipEvent_t pooled_event(ppm_ctx* c) {
+    if (!c->ev_pool.empty()) {
+        hipEvent_t e = c->ev_pool.back();
+        c->ev_pool.pop_back();
+        return e;
+    }
+    hipEvent_t e = nullptr;
+    if (hipEventCreate(&e) != hipSuccess) return nullptr;
+    return e;
+}
+
+// Records an event before a launch when profiling; returns it (or null).
+hipEvent_t prof_begin(ppm_ctx* c) {
+    if (!c->profiling) return nullptr;
+    hipEvent_t e = pooled_event(c);
+    if (e) (void)hipEventRecord(e, c->stream);
+    return e;
+}
+
+void prof_end(ppm_ctx* c, int kind, hipEvent_t start) {
+    if (!c->profiling || !start) return;
+    hipEvent_t e = pooled_event(c);
+    if (!e) return;
+    (void)hipEventRecord(e, c->stream);
+    c->ev_pending.push_back({kind, {start, e}});
+    // bound the number of outstanding events
+    if (c->ev_pending.size() >= 4096) {
+        (void)hipEventSynchronize(e);
+        for (auto& pe : c->ev_pending) {
+            float ms = 0.0f;
+            if (hipEventElapsedTime(&ms, pe.second.first, pe.second.second) == hipSuccess) {
+                c->prof_ms[pe.first] += ms;
+                c->prof_n[pe.first] += 1;
+            }
+            c->ev_pool.push_back(pe.second.first);
+            c->ev_pool.push_back(pe.second.second);
+        }
+        c->ev_pending.clear();
+    }
+}
+
+void free_all(ppm_ctx* c) {
+    for (auto& pe : c->ev_pending) {
+        (void)hipEventDestroy(pe.second.first);
+        (void)hipEventDestroy(pe.second.second);
+    }
+    c->ev_pending.clear();
+    for (hipEvent_t e : c->ev_pool) (void)hipEventDestroy(e);
+    c->ev_pool.clear();
+    for (void* p : c->allocs) (void)hipFree(p);
+    c->allocs.clear();
+    if (c->ev0) (void)hipEventDestroy(c->ev0);
+    if (c->ev1) (void)hipEventDestroy(c->ev1);
+    if (c->stream) (void)hipStreamDestroy(c->stream);
+    c->ev0 = c->ev1 = nullptr;
+    c->stream = nullptr;
+}
+
+int create_impl(ppm_ctx* c, const ppm_scene* s, const ppm_params* p) {
+    c->prm = *p;
+    c->Q = p->image_width * p->image_height;
+    c->P = p->photon_width * p->photon_height;
+    c->H = p->hash_width * p->hash_height;
+    c->T = s->num_triangles;
+    c->N = s->num_nodes;
+    c->M = s->num_materials;
+    c->NL = s->num_lights;
+    std::memcpy(c->xs, p->xorshift, sizeof(c->xs));
+
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) return set_err(c, PPM_E_NODEV, "no HIP device");
+    if (p->device < 0 || p->device >= ndev) return set_err(c, PPM_E_INVAL, "device ordinal out of range");
+    c->device = p->device;
+    HIP_TRY(c, hipSetDevice(c->device));
+    HIP_TRY(c, hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+    HIP_TRY(c, hipEventCreate(&c->ev0));
+    HIP_TRY(c, hipEventCreate(&c->ev1));
+
+    int rc;
+    std::vector<float4> nodes, tri_isect, tri_shade, mats((size_t)c->M * 2);
+    if ((rc = pack_nodes(s, nodes))) return set_err(c, rc, g_create_error);
+    pack_triangles(s, tri_isect, tri_shade);
+    for (int m = 0; m < c->M; m++) {
+        const ppm_material& mt = s->materials[m];
+        mats[2 * m] = make_float4(mt.color[0], mt.color[1], mt.color[2], mt.eta);
+        mats[2 * m + 1] = make_float4((float)mt.brdf, mt.g, 0.0f, 0.0f);
+    }
+    std::vector<int> ltri(s->light_triangle, s->light_triangle + c->NL);
+    std::vector<float> lcdf(s->light_cdf, s->light_cdf + c->NL);
+    float4 *d_nodes, *d_isect, *d_shade, *d_mats;
+    int* d_ltri;
+    float* d_lcdf;
+    if ((rc = upload(c, &d_nodes, nodes)) || (rc = upload(c, &d_isect, tri_isect)) ||
+        (rc = upload(c, &d_shade, tri_shade)) || (rc = upload(c, &d_mats, mats)) ||
+        (rc = upload(c, &d_ltri, ltri)) || (rc = upload(c, &d_lcdf, lcdf)))
+        return rc;
+    c->scene.nodes = d_nodes;
+    c->scene.tri_isect = d_isect;
+    c->scene.tri_shade = d_shade;
+    c->scene.mats = d_mats;
+    c->scene.light_tri = d_ltri;
+    c->scene.light_cdf = d_lcdf;
+    c->scene.num_nodes = c->N;
+    c->scene.num_lights = c->NL;
+    c->scene.light_area = s->light_area;
+    c->scene.bsphere_x = s->bsphere[0];
+    c->scene.bsphere_y = s->bsphere[1];
+    c->scene.bsphere_z = s->bsphere[2];
+    c->scene.bsphere_r = s->bsphere[3];
+    const ppm_camera& k = s->camera;
+    c->cam = DevCamera{k.origin[0], k.origin[1], k.origin[2], k.u[0], k.u[1], k.u[2], k.v[0], k.v[1], k.v[2],
+                       k.w[0],      k.w[1],      k.w[2],      k.half_width, k.half_height, k.distance};
+
+    c->eye_blocks = (c->Q + 255) / 256;
+    c->photon_blocks = (c->P + 255) / 256;
+    const size_t Q = c->Q, P = c->P, H = c->H;
+    HitState& hs = c->hit;
+    PhotonState& ps = c->ph;
+    if ((rc = dev_alloc(c, &hs.pos, Q)) || (rc = dev_alloc(c, &hs.refl, Q)) || (rc = dev_alloc(c, &hs.nrm, Q)) ||
+        (rc = dev_alloc(c, &hs.rng, Q)) || (rc = dev_alloc(c, &hs.tau_r, Q)) || (rc = dev_alloc(c, &hs.e_n, Q)) ||
+        (rc = dev_alloc(c, &hs.tri, Q)) || (rc = dev_alloc(c, &hs.mat_t, Q)) || (rc = dev_alloc(c, &hs.rgba8, Q)) ||
+        (rc = dev_alloc(c, &hs.partial, (size_t)c->eye_blocks * 6)) || (rc = dev_alloc(c, &ps.pos, P)) ||
+        (rc = dev_alloc(c, &ps.flux, P)) || (rc = dev_alloc(c, &ps.dir, P)) || (rc = dev_alloc(c, &ps.rng, P)) ||
+        (rc = dev_alloc(c, &ps.state, P)) || (rc = dev_alloc(c, &ps.slot, P)) ||
+        (rc = dev_alloc(c, &ps.partial, (size_t)c->photon_blocks)) || (rc = dev_alloc(c, &c->hash.count, H)) ||
+        (rc = dev_alloc(c, &c->hash.rep, H)) || (rc = dev_alloc(c, &c->glob, 1)) ||
+        (rc = dev_alloc(c, &c->resolve_lin, Q)) || (rc = dev_alloc(c, &c->resolve_map, Q)))
+        return rc;
+    c->hash.size = c->H;
+    c->hash.width = p->hash_width;
+    c->hash.height = p->hash_height;
+    c->hash.num_f = (float)c->H;
+
+    // per-pixel and per-photon generator states
+    std::vector<float4> seeds;
+    seed_grid(c->xs, seeds, Q);
+    HIP_TRY(c, hipMemcpy(hs.rng, seeds.data(), Q * sizeof(float4), hipMemcpyHostToDevice));
+    seed_grid(c->xs, seeds, P);
+    HIP_TRY(c, hipMemcpy(ps.rng, seeds.data(), P * sizeof(float4), hipMemcpyHostToDevice));
+    DevGlobals g0;
+    std::memset(&g0, 0, sizeof(g0));
+    HIP_TRY(c, hipMemcpy(c->glob, &g0, sizeof(g0), hipMemcpyHostToDevice));
+    const int n = (int)std::max(Q, P);
+    hipLaunchKernelGGL(k_init_state, dim3((n + 255) / 256), dim3(256), 0, c->stream, c->Q, c->P, hs, ps);
+    HIP_TRY(c, hipGetLastError());
+    hipLaunchKernelGGL(k_clear_slots, dim3(std::min<int>((c->H + 255) / 256, 65536)), dim3(256), 0, c->stream,
+                       c->hash);
+    HIP_TRY(c, hipGetLastError());
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    return PPM_OK;
+}
+
+int check_params(const ppm_params* p) {
+    if (!p) return set_err(nullptr, PPM_E_INVAL, "null params");
+    if (p->image_width <= 0 || p->image_height <= 0 || p->photon_width <= 0 || p->photon_height <= 0 ||
+        p->hash_width <= 0 || p->hash_height <= 0)
+        return set_err(nullptr, PPM_E_INVAL, "sizes must be positive");
+    const long long Q = (long long)p->image_width * p->image_height;
+    const long long P = (long long)p->photon_width * p->photon_height;
+    const long long H = (long long)p->hash_width * p->hash_height;
+    if (Q > (1ll << 26) || P > (1ll << 26)) return set_err(nullptr, PPM_E_INVAL, "Q and P are limited to 2^26");
+    if (P > 0x7fffffffll) return set_err(nullptr, PPM_E_INVAL, "P too large");
+    if (H > (1ll << 24)) return set_err(nullptr, PPM_E_INVAL, "H is limited to 2^24 (slot numbers are exact floats)");
+    if (!is_pow2(p->hash_width)) return set_err(nullptr, PPM_E_INVAL, "hash_width must be a power of two");
+    if (p->max_path_length <= 0 || p->max_path_length > 0xffff)
+        return set_err(nullptr, PPM_E_INVAL, "max_path_length out of range");
+    if (!(p->fixed_radius >= 0.0f) || !(p->initial_footprint > 0.0f) || !(p->alpha > 0.0f && p->alpha <= 1.0f))
+        return set_err(nullptr, PPM_E_INVAL, "bad radius or alpha");
+    return PPM_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+void ppm_params_default(ppm_params* p, int32_t res) {
+    if (!p) return;
+    std::memset(p, 0, sizeof(*p));
+    p->image_width = p->image_height = res;
+    p->photon_width = p->photon_height = res;
+    p->hash_width = p->hash_height = res;
+    p->max_path_length = 10;
+    p->alpha = 0.7f;
+    p->initial_footprint = 2.5f;
+    p->fixed_radius = 0.0f;
+    p->focal_length = 13.0f;
+    p->aperture = 0.0f;
+    p->xorshift[0] = 123456789u;
+    p->xorshift[1] = 362436069u;
+    p->xorshift[2] = 521288629u;
+    p->xorshift[3] = 88675123u;
+    p->device = 0;
+    p->flags = 0;
+}
+
+int ppm_create(const ppm_scene* scene, const ppm_params* params, ppm_ctx** out) {
+    if (!out) return set_err(nullptr, PPM_E_INVAL, "null output");
+    *out = nullptr;
+    int rc;
+    if ((rc = check_params(params)) || (rc = check_scene(scene))) return rc;
+    ppm_ctx* c = new (std::nothrow) ppm_ctx();
+    if (!c) return set_err(nullptr, PPM_E_OOM, "out of host memory");
+    rc = create_impl(c, scene, params);
+    if (rc != PPM_OK) {
+        g_create_error = c->err.empty() ? g_create_error : c->err;
+        free_all(c);
+        delete c;
+        return rc;
+    }
+    *out = c;
+    return PPM_OK;
+}
+
+int ppm_pass(ppm_ctx* c, uint32_t npasses) {
+    if (!c) return PPM_E_INVAL;
+    HIP_TRY(c, hipSetDevice(c->device));
+    const ppm_params& p = c->prm;
+    const unsigned L = (unsigned)p.max_path_length;
+    if (npasses == 0) return PPM_OK;
+    HIP_TRY(c, hipEventRecord(c->ev0, c->stream));
+    for (uint32_t it = 0; it < npasses; it++) {
+        if (c->frame % L == 0) {
+            EyeParams ep;
+            ep.width = p.image_width;
+            ep.height = p.image_height;
+            ep.aa_x = (xs_frand(c->xs) - 0.5f) * 1.25f;
+            ep.aa_y = (xs_frand(c->xs) - 0.5f) * 1.25f;
+            ep.focal_length = p.focal_length;
+            ep.aperture = p.aperture;
+            ep.max_path = p.max_path_length;
+            ep.num_eye_samples = (int)(c->eye_samples + 1);
+            ep.fixed_radius = p.fixed_radius;
+            ep.footprint = p.initial_footprint;
+            ep.footprint_width = (float)p.image_width;
+            c->eye_samples++;
+            hipEvent_t e0 = prof_begin(c);
+            hipLaunchKernelGGL(k_eye, dim3(c->eye_blocks), dim3(256), 0, c->stream, c->scene, c->cam, ep, c->hit,
+                               c->glob);
+            prof_end(c, PPM_KERNEL_EYE, e0);
+        }
+        hipEvent_t e1 = prof_begin(c);
+        hipLaunchKernelGGL(k_clear_slots, dim3(std::min<int>((c->H + 255) / 256, 65536)), dim3(256), 0, c->stream,
+                           c->hash);
+        prof_end(c, PPM_KERNEL_CLEAR, e1);
+        PhotonParams pp;
+        pp.count = c->P;
+        pp.width = p.photon_width;
+        pp.max_path = p.max_path_length;
+        // scatter.vs: depth z = s/Pw + t; points with z >= 1 fail the depth test, which removes
+        // the columns i >= Pw^2 / (2 Ph) - 1/2 of the last photon row
+        {
+            const double first = std::ceil((double)p.photon_width * p.photon_width / (2.0 * p.photon_height) - 0.5);
+            const long long ex = (long long)p.photon_width - (long long)std::max(0.0, first);
+            pp.rep_limit = (int)(c->P - std::max(0ll, ex));
+        }
+        hipEvent_t e2 = prof_begin(c);
+        hipLaunchKernelGGL(k_photon, dim3(c->photon_blocks), dim3(256), 0, c->stream, c->scene, pp, c->ph, c->hash,
+                           c->glob);
+        prof_end(c, PPM_KERNEL_PHOTON, e2);
+        GatherParams gp;
+        gp.count = c->Q;
+        gp.alpha = p.alpha;
+        gp.first_pass = c->frame == 0 ? 1 : 0;
+        gp.draw = (p.flags & PPM_FLAG_NO_DRAW) ? 0 : 1;
+        hipEvent_t e3 = prof_begin(c);
+        hipLaunchKernelGGL(k_gather, dim3(c->eye_blocks), dim3(256), 0, c->stream, gp, c->hit, c->ph, c->hash,
+                           c->glob);
+        prof_end(c, PPM_KERNEL_GATHER, e3);
+        c->frame++;
+    }
+    HIP_TRY(c, hipGetLastError());
+    HIP_TRY(c, hipEventRecord(c->ev1, c->stream));
+    c->timed = true;
+    return PPM_OK;
+}
+
+int ppm_sync(ppm_ctx* c) {
+    if (!c) return PPM_E_INVAL;
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    return PPM_OK;
+}
+
+float ppm_last_pass_ms(ppm_ctx* c) {
+    if (!c || !c->timed) return -1.0f;
+    if (hipEventSynchronize(c->ev1) != hipSuccess) return -1.0f;
+    float ms = -1.0f;
+    if (hipEventElapsedTime(&ms, c->ev0, c->ev1) != hipSuccess) return -1.0f;
+    return ms;
+}
+
+size_t ppm_buffer_bytes(const ppm_ctx* c, ppm_buf which) {
+    if (!c) return 0;
+    const size_t Q = c->Q, P = c->P, H = c->H;
+    switch (which) {
+        case PPM_BUF_RADIANCE:
+        case PPM_BUF_TONEMAPPED:
+        case PPM_BUF_HIT_POSITION:
+        case PPM_BUF_HIT_REFLECTANCE:
+        case PPM_BUF_HIT_NORMAL:
+        case PPM_BUF_EYE_RNG:
+        case PPM_BUF_FLUX_RADIUS:
+        case PPM_BUF_EMISSION_COUNT: return Q * 16;
+        case PPM_BUF_RGBA8: return Q * 4;
+        case PPM_BUF_HIT_TRIANGLE: return Q * 8;
+        case PPM_BUF_HIT_T: return Q * 4;
+        case PPM_BUF_PHOTON_POSITION:
+        case PPM_BUF_PHOTON_FLUX:
+        case PPM_BUF_PHOTON_DIRECTION:
+        case PPM_BUF_PHOTON_RNG: return P * 16;
+        case PPM_BUF_PHOTON_STATE: return P * 12;
+        case PPM_BUF_PHOTON_SLOT: return P * 4;
+        case PPM_BUF_SLOT_COUNT:
+        case PPM_BUF_SLOT_REP: return H * 4;
+        case PPM_BUF_COUNTERS: return sizeof(ppm_counters);
+        case PPM_BUF_GRID: return sizeof(ppm_grid);
+        default: return 0;
+    }
+}
+
+int ppm_readback(ppm_ctx* c, ppm_buf which, void* dst, size_t bytes) {
+    if (!c || !dst) return PPM_E_INVAL;
+    const size_t need = ppm_buffer_bytes(c, which);
+    if (need == 0) return set_err(c, PPM_E_INVAL, "unknown buffer");
+    if (bytes != need) return set_err(c, PPM_E_INVAL, "byte count does not match the buffer size");
+    HIP_TRY(c, hipSetDevice(c->device));
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    const size_t Q = c->Q, P = c->P, H = c->H;
+    auto d2h = [&](const void* src, size_t n) -> int {
+        HIP_TRY(c, hipMemcpy(dst, src, n, hipMemcpyDeviceToHost));
+        return PPM_OK;
+    };
+    switch (which) {
+        case PPM_BUF_RADIANCE:
+        case PPM_BUF_TONEMAPPED:
+        case PPM_BUF_RGBA8: {
+            if (c->frame == 0) return set_err(c, PPM_E_INVAL, "no pass has run");
+            hipLaunchKernelGGL(k_resolve, dim3((c->Q + 255) / 256), dim3(256), 0, c->stream, c->Q, c->hit, c->glob,
+                               c->resolve_lin, c->resolve_map);
+            HIP_TRY(c, hipGetLastError());
+            HIP_TRY(c, hipStreamSynchronize(c->stream));
+            if (which == PPM_BUF_RADIANCE) return d2h(c->resolve_lin, Q * 16);
+            if (which == PPM_BUF_TONEMAPPED) return d2h(c->resolve_map, Q * 16);
+            return d2h(c->hit.rgba8, Q * 4);
+        }
+        case PPM_BUF_HIT_POSITION: return d2h(c->hit.pos, Q * 16);
+        case PPM_BUF_HIT_REFLECTANCE: return d2h(c->hit.refl, Q * 16);
+        case PPM_BUF_HIT_NORMAL: return d2h(c->hit.nrm, Q * 16);
+        case PPM_BUF_EYE_RNG: return d2h(c->hit.rng, Q * 16);
+        case PPM_BUF_FLUX_RADIUS: return d2h(c->hit.tau_r, Q * 16);
+        case PPM_BUF_EMISSION_COUNT: return d2h(c->hit.e_n, Q * 16);
+        case PPM_BUF_HIT_TRIANGLE:
+        case PPM_BUF_HIT_T: {
+            std::vector<int> tri(Q);
+            std::vector<float2> mt(Q);
+            HIP_TRY(c, hipMemcpy(tri.data(), c->hit.tri, Q * 4, hipMemcpyDeviceToHost));
+            HIP_TRY(c, hipMemcpy(mt.data(), c->hit.mat_t, Q * 8, hipMemcpyDeviceToHost));
+            if (which == PPM_BUF_HIT_T) {
+                float* o = (float*)dst;
+                for (size_t i = 0; i < Q; i++) o[i] = mt[i].y;
+            } else {
+                int32_t* o = (int32_t*)dst;
+                for (size_t i = 0; i < Q; i++) {
+                    o[2 * i] = tri[i];
+                    o[2 * i + 1] = tri[i] < 0 ? -1 : (int32_t)mt[i].x;
+                }
+            }
+            return PPM_OK;
+        }
+        case PPM_BUF_PHOTON_POSITION: return d2h(c->ph.pos, P * 16);
+        case PPM_BUF_PHOTON_FLUX: return d2h(c->ph.flux, P * 16);
+        case PPM_BUF_PHOTON_DIRECTION: return d2h(c->ph.dir, P * 16);
+        case PPM_BUF_PHOTON_RNG: return d2h(c->ph.rng, P * 16);
+        case PPM_BUF_PHOTON_STATE: {
+            std::vector<int2> st(P);
+            HIP_TRY(c, hipMemcpy(st.data(), c->ph.state, P * 8, hipMemcpyDeviceToHost));
+            int32_t* o = (int32_t*)dst;
+            for (size_t i = 0; i < P; i++) {
+                o[3 * i] = st[i].x;
+                o[3 * i + 1] = st[i].y & 0xffff;
+                o[3 * i + 2] = (st[i].y >> 16) & 1;
+            }
+            return PPM_OK;
+        }
+        case PPM_BUF_PHOTON_SLOT: return d2h(c->ph.slot, P * 4);
+        case PPM_BUF_SLOT_COUNT: return d2h(c->hash.count, H * 4);
+        case PPM_BUF_SLOT_REP: return d2h(c->hash.rep, H * 4);
+        case PPM_BUF_COUNTERS: {
+            DevGlobals g;
+            HIP_TRY(c, hipMemcpy(&g, c->glob, sizeof(g), hipMemcpyDeviceToHost));
+            ppm_counters k;
+            std::memset(&k, 0, sizeof(k));
+            k.frame_count = c->frame;
+            k.eye_samples = c->eye_samples;
+            k.photons_emitted = g.photons_emitted;
+            k.photon_segments = (uint64_t)c->P * c->frame;
+            k.num_photons_f = g.num_photons_f;
+            std::memcpy(dst, &k, sizeof(k));
+            return PPM_OK;
+        }
+        case PPM_BUF_GRID: {
+            DevGlobals g;
+            HIP_TRY(c, hipMemcpy(&g, c->glob, sizeof(g), hipMemcpyDeviceToHost));
+            ppm_grid k;
+            for (int i = 0; i < 3; i++) {
+                k.bbmin[i] = g.bbmin[i];
+                k.bbmax[i] = g.bbmax[i];
+            }
+            k.initial_radius = g.initial_radius;
+            k.grid_scale = g.grid_scale;
+            k.hash_scale = g.hash_scale;
+            std::memcpy(dst, &k, sizeof(k));
+            return PPM_OK;
+        }
+        default: return set_err(c, PPM_E_INVAL, "unknown buffer");
+    }
+}
+
+int ppm_profile_enable(ppm_ctx* c, int enable) {
+    if (!c) return PPM_E_INVAL;
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    for (auto& pe : c->ev_pending) {
+        c->ev_pool.push_back(pe.second.first);
+        c->ev_pool.push_back(pe.second.second);
+    }
+    c->ev_pending.clear();
+    for (int k = 0; k < PPM_KERNEL_KINDS; k++) {
+        c->prof_ms[k] = 0.0;
+        c->prof_n[k] = 0;
+    }
+    c->profiling = enable != 0;
+    return PPM_OK;
+}
+
+int ppm_profile_read(ppm_ctx* c, double* ms, uint64_t* launches) {
+    if (!c) return PPM_E_INVAL;
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    for (auto& pe : c->ev_pending) {
+        float t = 0.0f;
+        HIP_TRY(c, hipEventElapsedTime(&t, pe.second.first, pe.second.second));
+        c->prof_ms[pe.first] += t;
+        c->prof_n[pe.first] += 1;
+        c->ev_pool.push_back(pe.second.first);
+        c->ev_pool.push_back(pe.second.second);
+    }
+    c->ev_pending.clear();
+    for (int k = 0; k < PPM_KERNEL_KINDS; k++) {
+        if (ms) ms[k] = c->prof_ms[k];
+        if (launches) launches[k] = c->prof_n[k];
+    }
+    return PPM_OK;
+}
+
+void ppm_destroy(ppm_ctx* c) {
+    if (!c) return;
+    (void)hipSetDevice(c->device);
+    if (c->stream) (void)hipStreamSynchronize(c->stream);
+    free_all(c);
+    delete c;
+}
+
+const char* ppm_last_error(const ppm_ctx* c) { return c ? c->err.c_str() : g_create_error.c_str(); }
+
+}  // extern "C"

@@ -1,0 +1,241 @@
+/*
+ * ppm.h -- C ABI of the MI355X-native stochastic progressive photon mapping (SPPM) pass loop.
+ *
+ * The reference renders one SPPM pass per GLUT frame in m_display() (glslppm.cpp:214-536): eye
+ * trace every MaxNumberOfBounces frames, max/min reduction of hit positions, photon trace, sum of
+ * emitted flags, hash, scatter, correction, progressive update, draw.  Its GPU state lives in
+ * RGBA32F textures bound to GL programs; there is no library API.  This header replaces that GL
+ * texture ping-pong with three calls a render loop can make on one GPU:
+ *
+ *   ppm_create   -- replaces the texture/FBO set-up of main() (glslppm.cpp:692-795), the scene
+ *                   upload of CGPURT::CreateTextures (gpurt.cpp:47-552) and RandomizeTextures
+ *                   (glslppm.cpp:113-140).
+ *   ppm_pass     -- replaces m_display() (glslppm.cpp:214-536), same stage order, asynchronous.
+ *   ppm_readback -- replaces glReadPixels / reading the textures (glslppm.cpp:206, the draw of
+ *                   re.fs at glslppm.cpp:506-513).
+ *
+ * The host scene layer (OBJ/MTL load of mesh.cpp, camera.cpp, the threaded multi-BVH of bvh.cpp)
+ * is exposed as ppm_mesh_* and produces the flattened ppm_scene that ppm_create consumes.  A
+ * caller that already holds the reference's TMesh/TBVH fills ppm_scene itself.
+ *
+ * Conventions: every int-returning call returns PPM_OK (0) or a negative PPM_E_* code and never
+ * aborts; the message is available from ppm_last_error / ppm_mesh_last_error.  A context is used
+ * by one thread at a time; distinct contexts may be driven from distinct threads.  Caller-owned
+ * buffers are copied, never retained.  Pixel p = j*width + i with row j counted from the bottom,
+ * the GL convention of gl_FragCoord (glslppm.cpp:120-124).
+ */
+#ifndef PPM_H
+#define PPM_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define PPM_OK 0
+#define PPM_E_INVAL (-1)   /* bad argument, size or state */
+#define PPM_E_OOM (-2)     /* device or host allocation failed */
+#define PPM_E_HIP (-3)     /* a HIP runtime call failed */
+#define PPM_E_IO (-4)      /* file could not be read */
+#define PPM_E_NODEV (-5)   /* no usable GPU */
+
+/* BRDF codes of the material table (mesh.cpp:520-611, used by eyeraytrace.fs:448-539 and
+ * photontrace.fs:567-669). */
+#define PPM_BRDF_LIGHT (-1)
+#define PPM_BRDF_DIFFUSE 0
+#define PPM_BRDF_MIRROR 1
+#define PPM_BRDF_GLASS 2
+#define PPM_BRDF_PLASTIC 3
+#define PPM_BRDF_GLOSSY_MIRROR 4
+#define PPM_BRDF_GLOSSY_GLASS 5
+#define PPM_BRDF_GLOSSY_PLASTIC 6
+
+/* Look-at pinhole camera, TCamera::Set (camera.cpp:4-18) and the CameraParams uniform
+ * (glslppm.cpp:270-273). */
+typedef struct ppm_camera {
+    float origin[3];
+    float u[3], v[3], w[3];
+    float half_width;   /* width * 0.5f  */
+    float half_height;  /* height * 0.5f */
+    float distance;     /* height / (2 tan(fov/2)) */
+} ppm_camera;
+
+/* One material, the two texels of TextureMaterials (gpurt.cpp:177-198): (rgb, eta), (brdf, g). */
+typedef struct ppm_material {
+    float color[3];
+    float eta;
+    int32_t brdf;
+    float g;            /* 1 / ((2 / (1 - specularity))^2.71828 + 1), gpurt.cpp:194 */
+} ppm_material;
+
+/* Flattened scene: triangles, materials, light CDF and the threaded six-order BVH.
+ * Node k is the k-th node of the canonical (face 0) pre-order of TBVH::Build (bvh.cpp:280-353).
+ * node_links[(f * num_nodes + k) * 2 + {0,1}] are the hit and miss links of node k in the order
+ * of cube face f (0:+x 1:-x 2:+y 3:-y 4:+z 5:-z), as canonical node indices, -1 for "done";
+ * they are the texels of the six link tiles of textureBVHs (gpurt.cpp:408-473). */
+typedef struct ppm_scene {
+    int32_t num_triangles;
+    const float* positions;       /* [T][3 vertices][xyz] */
+    const float* normals;         /* [T][3 vertices][xyz] shading normals before packing */
+    const int32_t* material;      /* [T] material index */
+    int32_t num_materials;
+    const ppm_material* materials;/* [M] */
+    int32_t num_nodes;            /* 2T-1 */
+    const float* node_bounds;     /* [N][6] min xyz, max xyz */
+    const int32_t* node_triangle; /* [N] leaf triangle or -1 */
+    const int32_t* node_links;    /* [6][N][2] */
+    int32_t num_lights;
+    const int32_t* light_triangle;/* [NL] triangles whose material is a light */
+    const float* light_cdf;       /* [NL] normalised running area sum (mesh.cpp:456-488) */
+    float light_area;             /* LightSummedArea */
+    float bsphere[4];             /* SceneBSphere: bbox centre, half diagonal (glslppm.cpp:388-392) */
+    ppm_camera camera;
+} ppm_scene;
+
+/* Pass-loop parameters.  The reference couples Q = P = H = 512^2 (glslppm.cpp:36-40); here the
+ * three sizes are independent. */
+typedef struct ppm_params {
+    int32_t image_width, image_height;   /* Q eye pixels / hit points */
+    int32_t photon_width, photon_height; /* P photon slots (PhotonBufferSize^2 in the reference) */
+    int32_t hash_width, hash_height;     /* H hash slots; hash_width a power of two, H <= 2^24 */
+    int32_t max_path_length;             /* MaxNumberOfBounces: bounce cap and eye-pass cadence */
+    float alpha;                         /* 0.7 (glslppm.cpp:488) */
+    float initial_footprint;             /* 2.5 (glslppm.cpp:40) */
+    float fixed_radius;                  /* > 0: initial radius used instead of the footprint rule */
+    float focal_length;                  /* 13 (glslppm.cpp:31) */
+    float aperture;                      /* 0 (glslppm.cpp:32) */
+    uint32_t xorshift[4];                /* 123456789, 362436069, 521288629, 88675123 */
+    int32_t device;                      /* HIP device ordinal */
+    uint32_t flags;                      /* PPM_FLAG_* */
+} ppm_params;
+
+#define PPM_FLAG_NO_DRAW 1u     /* skip the per-pass RGBA8 draw (re.fs); readback still resolves */
+
+/* Buffers ppm_readback can copy out.  Element types and counts are given per entry; the byte
+ * count passed must equal the buffer size exactly. */
+typedef enum ppm_buf {
+    PPM_BUF_RADIANCE = 0,       /* float[Q][4]: tau/(R^2 pi NumPhotons) + E, w = 1 (re.fs:35-38) */
+    PPM_BUF_TONEMAPPED = 1,     /* float[Q][4]: 1-exp(-60000 L), sRGB (re.fs:41-47) */
+    PPM_BUF_RGBA8 = 2,          /* uint8[Q][4]: the window image of the last pass */
+    PPM_BUF_HIT_POSITION = 3,   /* float[Q][4]: QueryPositionTexture (pos, dir.x) */
+    PPM_BUF_HIT_REFLECTANCE = 4,/* float[Q][4]: QueryReflectanceTexture (col, dir.y) */
+    PPM_BUF_HIT_NORMAL = 5,     /* float[Q][4]: QueryNormalTexture (nrm, dir.z) */
+    PPM_BUF_HIT_TRIANGLE = 6,   /* int32[Q][2]: triangle and material of the last eye hit, -1 miss */
+    PPM_BUF_HIT_T = 7,          /* float[Q]: isect.w of the last eye trace */
+    PPM_BUF_EYE_RNG = 8,        /* float[Q][4]: RandomEyeRayTexture */
+    PPM_BUF_FLUX_RADIUS = 9,    /* float[Q][4]: QueryFluxRadiusTexture (tau, R) */
+    PPM_BUF_EMISSION_COUNT = 10,/* float[Q][4]: QueryEmissionPhotonCountTexture (E, N) */
+    PPM_BUF_PHOTON_POSITION = 11,  /* float[P][4]: (pos, nrm.x) */
+    PPM_BUF_PHOTON_FLUX = 12,      /* float[P][4]: (flux, nrm.y); flux.x < 0 marks no deposit */
+    PPM_BUF_PHOTON_DIRECTION = 13, /* float[P][4]: (dir, nrm.z) */
+    PPM_BUF_PHOTON_RNG = 14,       /* float[P][4]: RandomPhotonTexture */
+    PPM_BUF_PHOTON_STATE = 15,     /* int32[P][3]: triangle (-1 none), bounce count, emitted flag */
+    PPM_BUF_PHOTON_SLOT = 16,      /* int32[P]: hash slot of each photon (PhotonIndexTexture.zw), -1 */
+    PPM_BUF_SLOT_COUNT = 17,       /* uint32[H]: PhotonCorrectionTexture */
+    PPM_BUF_SLOT_REP = 18,         /* int32[H]: representative photon (min index), -1 when none */
+    PPM_BUF_COUNTERS = 19,         /* ppm_counters */
+    PPM_BUF_GRID = 20              /* ppm_grid */
+} ppm_buf;
+
+typedef struct ppm_counters {
+    uint64_t frame_count;      /* FrameCount: passes run */
+    uint64_t eye_samples;      /* NumEyeSamples */
+    uint64_t photons_emitted;  /* exact sum of emitted flags */
+    uint64_t photon_segments;  /* P * passes */
+    float num_photons_f;       /* NumPhotons as the reference accumulates it, in float */
+    uint32_t reserved;
+} ppm_counters;
+
+typedef struct ppm_grid {
+    float bbmin[3];            /* BBMin after expansion by the initial radius */
+    float bbmax[3];
+    float initial_radius;      /* InitialRadius */
+    float grid_scale;          /* GridScale = 0.5 / InitialRadius */
+    float hash_scale;          /* HashScale1 = 4194304 / GridScale */
+} ppm_grid;
+
+typedef struct ppm_ctx ppm_ctx;
+
+/* Default parameters: the reference's constants at a given square size. */
+void ppm_params_default(ppm_params* p, int32_t resolution);
+
+/* Replaces the set-up of main() (glslppm.cpp:692-829): copies the scene to HBM, allocates all
+ * pass state, seeds the per-pixel and per-photon generators from the xorshift state. */
+int ppm_create(const ppm_scene* scene, const ppm_params* params, ppm_ctx** out);
+
+/* Replaces m_display() (glslppm.cpp:214-536), run npasses times back to back on the context's
+ * stream.  Returns once the work is queued. */
+int ppm_pass(ppm_ctx* ctx, uint32_t npasses);
+
+/* Waits for queued passes and copies one buffer to host memory. */
+int ppm_readback(ppm_ctx* ctx, ppm_buf which, void* dst, size_t bytes);
+
+/* Size in bytes of a readback buffer for this context, 0 for an unknown id. */
+size_t ppm_buffer_bytes(const ppm_ctx* ctx, ppm_buf which);
+
+/* Waits for queued passes. */
+int ppm_sync(ppm_ctx* ctx);
+
+/* Device time in milliseconds of the last ppm_pass call, measured with HIP events on the context
+ * stream; -1 when not available. */
+float ppm_last_pass_ms(ppm_ctx* ctx);
+
+/* Per-kernel device timing with HIP events recorded around each launch on the context stream.
+ * ppm_profile_enable(ctx, 1) clears the accumulators and starts recording; ppm_profile_read waits
+ * for the stream and returns the accumulated milliseconds and launch counts per kernel kind.
+ * No counterpart in the reference (it times whole frames with glutGet, glslppm.cpp:526-535). */
+#define PPM_KERNEL_EYE 0      /* eyeraytrace.fs + max/min */
+#define PPM_KERNEL_CLEAR 1    /* glClear of the hash and correction targets */
+#define PPM_KERNEL_PHOTON 2   /* photontrace.fs + sum + hash + scatter + correction */
+#define PPM_KERNEL_GATHER 3   /* progressive.fs + re.fs */
+#define PPM_KERNEL_KINDS 4
+int ppm_profile_enable(ppm_ctx* ctx, int enable);
+int ppm_profile_read(ppm_ctx* ctx, double* ms, uint64_t* launches);
+
+void ppm_destroy(ppm_ctx* ctx);
+const char* ppm_last_error(const ppm_ctx* ctx);
+
+/* ---------------- host scene layer (mesh.cpp, camera.cpp, bvh.cpp) ---------------- */
+
+typedef struct ppm_mesh ppm_mesh;
+
+/* TMesh::LoadOBJ (mesh.cpp:491-693): triangles of an OBJ file with its MTL library, positions
+ * scaled then offset, materials classified by the MTL rules of mesh.cpp:520-611. */
+int ppm_mesh_load_obj(const char* path, const float position[3], float scale, ppm_mesh** out);
+
+/* A mesh from arrays.  normals may be NULL (face normals are used, mesh.cpp:642-652). */
+int ppm_mesh_from_arrays(int32_t num_triangles, const float* positions, const float* normals,
+                         const int32_t* material, int32_t num_materials,
+                         const ppm_material* materials, ppm_mesh** out);
+
+/* Seeded synthetic scenes: kind 0 = Cornell room with a mirror box and a diffuse box (36
+ * triangles); kind 1 = that room plus `count` jittered icospheres of subdivision `detail`
+ * (diffuse, mirror, glass); kind 2 = value-noise heightfield of detail x detail cells with
+ * `count` glass icospheres. */
+#define PPM_SCENE_CORNELL 0
+#define PPM_SCENE_SPHERES 1
+#define PPM_SCENE_TERRAIN 2
+int ppm_mesh_generate(int32_t kind, uint64_t seed, int32_t count, int32_t detail, ppm_mesh** out);
+
+/* TCamera::Set (camera.cpp:4-18). */
+int ppm_mesh_set_camera(ppm_mesh* mesh, const float origin[3], const float lookat[3],
+                        int32_t width, int32_t height, float fov_degrees);
+
+/* TBVH::Build (bvh.cpp:280-353) and the flattening of CGPURT::CreateTextures
+ * (gpurt.cpp:269-552). */
+int ppm_mesh_build(ppm_mesh* mesh);
+
+/* The flattened scene; valid after ppm_mesh_build until the mesh is destroyed. */
+const ppm_scene* ppm_mesh_scene(const ppm_mesh* mesh);
+
+int32_t ppm_mesh_num_triangles(const ppm_mesh* mesh);
+void ppm_mesh_destroy(ppm_mesh* mesh);
+const char* ppm_mesh_last_error(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* PPM_H */

@@ -1,0 +1,95 @@
+"""HIP pass loop against the CPU oracle, bit for bit, through the C ABI (include/ppm.h).
+
+Every state buffer of the pass loop is compared after a run of passes that spans several eye
+passes: hit points (position, reflectance, normal, triangle, t, generator state), the
+progressive statistics (tau, R, E, N), the photon state (position, flux, direction, generator
+state, triangle, bounce, emitted flag), the hash (slot per photon, count and representative per
+slot), the linear and tone-mapped radiance, the counters and the grid scalars.
+"""
+import numpy as np
+import pytest
+
+import glslppm_amd as g
+from parity import compare_all, format_report, run_pair
+
+pytestmark = pytest.mark.gpu
+
+
+def _check(mesh, params, passes, label):
+    print("case %s: Q=%dx%d P=%dx%d H=%dx%d L=%d passes=%d" % (
+        label, params.image_width, params.image_height, params.photon_width, params.photon_height,
+        params.hash_width, params.hash_height, params.max_path_length, passes), flush=True)
+    gpu, cpu = run_pair(mesh, params, passes)
+    rep = compare_all(gpu, cpu)
+    bad = format_report(rep)
+    assert not bad, "%s: mismatching elements: %s" % (label, bad)
+    return gpu, cpu
+
+
+def cornell(res):
+    return g.Mesh.generate(g.SCENE_CORNELL).set_camera((0, 0, 1.667), (0, 0, 0), res, res, 45.0).build()
+
+
+def test_cornell_first_pass():
+    _check(cornell(64), g.default_params(64, max_path_length=8), 1, "cornell-1")
+
+
+def test_cornell_three_eye_passes():
+    _check(cornell(64), g.default_params(64, max_path_length=8), 18, "cornell-18")
+
+
+def test_spheres_glass_mirror():
+    m = g.Mesh.generate(g.SCENE_SPHERES, seed=7, count=10, detail=2)
+    m.set_camera((0, 0, 1.667), (0, 0, 0), 64, 64, 45.0).build()
+    _check(m, g.default_params(64, max_path_length=10), 23, "spheres")
+
+
+def test_decoupled_sizes():
+    m = g.Mesh.generate(g.SCENE_SPHERES, seed=11, count=4, detail=1)
+    m.set_camera((0.1, 0.05, 1.6), (0, 0, 0), 48, 40, 55.0).build()
+    p = g.default_params(64, image_width=48, image_height=40, photon_width=64, photon_height=32,
+                         hash_width=128, hash_height=16, max_path_length=6)
+    _check(m, p, 13, "decoupled")
+
+
+def _glossy_variant(seed):
+    base = g.Mesh.generate(g.SCENE_SPHERES, seed=seed, count=8, detail=2).build()
+    a = base.arrays()
+    mats = []
+    kinds = [4, 5, 6, 3]
+    for i, (rgb, eta, brdf, gl) in enumerate(a["materials"]):
+        if i >= 5:  # the spheres: glossy mirror, glossy glass, glossy plastic, plastic
+            brdf = kinds[i % 4]
+            gl = 1.0 / ((2.0 / (1.0 - 0.6)) ** 2.71828 + 1.0)
+            eta = 1.5
+        mats.append((rgb, eta, brdf, gl))
+    m = g.Mesh.from_arrays(a["positions"], mats, a["material"], a["normals"])
+    return m.set_camera((0, 0, 1.667), (0, 0, 0), 64, 64, 45.0).build()
+
+
+def test_glossy_and_plastic_brdfs():
+    _check(_glossy_variant(3), g.default_params(64, max_path_length=10), 21, "glossy")
+
+
+def test_image_based_light_without_light_triangles():
+    base = cornell(48).arrays()
+    mats = [(rgb, eta, 0 if brdf == -1 else brdf, gl) for (rgb, eta, brdf, gl) in base["materials"]]
+    m = g.Mesh.from_arrays(base["positions"], mats, base["material"], base["normals"])
+    m.set_camera((0, 0, 1.667), (0, 0, 0), 48, 48, 45.0).build()
+    assert m.arrays()["light_triangle"].size == 0
+    _check(m, g.default_params(48, max_path_length=5), 7, "ibl")
+
+
+def test_cornell_c0_full_run():
+    """256^2 eye rays and photons, <= 8 bounces, fixed initial radius 1/50 of the room, 64 passes."""
+    p = g.default_params(256, max_path_length=8, fixed_radius=0.02)
+    gpu, cpu = _check(cornell(256), p, 64, "c0-full")
+    rad_g, rad_c = gpu.readback("radiance")[:, :3], cpu.readback("radiance")[:, :3]
+    rel = np.linalg.norm(rad_g - rad_c) / np.linalg.norm(rad_c)
+    assert rel <= 1e-3
+
+
+def test_reference_scene_file_layout_square_512():
+    """The reference's own configuration: Q = P = H = 512^2, 10 bounces, footprint radius rule."""
+    m = g.Mesh.generate(g.SCENE_CORNELL).set_camera((0, 0, 1.667), (0, 0, 0), 512, 512, 45.0).build()
+    _check(m, g.default_params(512), 12, "square-512")
