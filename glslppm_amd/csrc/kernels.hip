@@ -547,7 +547,13 @@ __global__ __launch_bounds__(kBlock) void k_photon(DevScene s, PhotonParams pp, 
                 hgn = i.gnrm;
                 hbrdf = i.brdf;
             }
-            if (hh.tri < 0 || (hbrdf != 0 && hbrdf != 6 && hbrdf != 3)) flux = -flux;
+            // flux = -flux on a miss or a non-diffuse hit, as a sign-bit flip selected without a
+            // branch (a divergent short-circuit here was compiled into a lost negation)
+            const bool no_deposit = (hh.tri < 0) | ((hbrdf != 0) & (hbrdf != 6) & (hbrdf != 3));
+            const unsigned int sign_bit = no_deposit ? 0x80000000u : 0u;
+            flux = mk3(__uint_as_float(__float_as_uint(flux.x) ^ sign_bit),
+                       __uint_as_float(__float_as_uint(flux.y) ^ sign_bit),
+                       __uint_as_float(__float_as_uint(flux.z) ^ sign_bit));
             // a miss has no geometric normal: 0/0 is NaN and the NaN-rejecting min gives sqrt(5)
             const float gf = min1(fabsf(dot3(dir, hnrm) / dot3(dir, hgn)), sqrtf(5.0f));
             flux = flux * gf;

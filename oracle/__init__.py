@@ -54,6 +54,8 @@ def library():
         lib.oracle_radiance.restype = None
         lib.oracle_radiance.argtypes = [fp, fp, ctypes.c_float, fp, fp]
         lib.oracle_num_threads.restype = ctypes.c_int
+        lib.oracle_update.restype = None
+        lib.oracle_update.argtypes = [fp, fp, fp, fp, ctypes.c_float, fp, ctypes.c_float]
         _lib = lib
     return _lib
 
@@ -126,6 +128,16 @@ def radiance(flux_radius, emission, total_photons):
     mp = np.zeros(4, np.float32)
     library().oracle_radiance(_fptr(fr), _fptr(em), float(np.float32(total_photons)), _fptr(lin), _fptr(mp))
     return lin, mp
+
+
+def update(tau, radius, count, phi, m, refl, alpha):
+    """The progressive update of one hit point; returns (tau[3], radius, count)."""
+    t = np.array(tau, np.float32)
+    r = np.array([radius], np.float32)
+    n = np.array([count], np.float32)
+    library().oracle_update(_fptr(t), _fptr(r), _fptr(n), _fptr(np.array(phi, np.float32)), float(np.float32(m)),
+                            _fptr(np.array(refl, np.float32)), float(np.float32(alpha)))
+    return t, r[0], n[0]
 
 
 def num_threads():

@@ -839,6 +839,22 @@ static void accumulate(Oracle* o, v3 qpos, v3 qnrm, float qr, v3 cell, v3* flux,
     }
 }
 
+/* progressive.fs:129-137: BRDF scaling of the gathered flux, then the N/R/tau update with
+ * g = min((N + alpha M) / (N + M), 1), NaN-rejecting so that N + M = 0 gives g = 1. */
+void oracle_update(float* tau, float* radius, float* count, const float* phi, float m, const float* refl,
+                   float alpha) {
+    v3 flux = vmul(V3(phi[0], phi[1], phi[2]), vscale(V3(refl[0], refl[1], refl[2]),
+                                                      0.31830995240629591621063460818591f));
+    float qn = *count, qr = *radius;
+    float g = gl_min((qn + alpha * m) / (qn + m), 1.0f);
+    qr = qr * sqrtf(g);
+    qn = qn + m * alpha;
+    v3 t = vscale(vadd(V3(tau[0], tau[1], tau[2]), flux), g);
+    tau[0] = t.x; tau[1] = t.y; tau[2] = t.z;
+    *radius = qr;
+    *count = qn;
+}
+
 static void progressive_fragment(Oracle* o, int px, float alpha) {
     v4 QP = o->qpos[px], QN = o->qnrm[px], QFR = o->qflux_r[px], QEN = o->qemi_n[px], QR = o->qrefl[px];
     v3 qflux = V3(QFR.x, QFR.y, QFR.z);
@@ -855,11 +871,9 @@ static void progressive_fragment(Oracle* o, int px, float alpha) {
                 for (int ix = 0; ix <= 1; ix++)
                     accumulate(o, qpos, V3(QN.x, QN.y, QN.z), qr,
                                vadd(ii, V3((float)ix, (float)iy, (float)iz)), &flux, &m);
-        flux = vmul(flux, vscale(V3(QR.x, QR.y, QR.z), 0.31830995240629591621063460818591f));
-        float g = gl_min((qn + alpha * m) / (qn + m), 1.0f);
-        qr = qr * sqrtf(g);
-        qn = qn + m * alpha;
-        qflux = vscale(vadd(qflux, flux), g);
+        float tau[3] = {qflux.x, qflux.y, qflux.z}, phi[3] = {flux.x, flux.y, flux.z}, refl[3] = {QR.x, QR.y, QR.z};
+        oracle_update(tau, &qr, &qn, phi, m, refl, alpha);
+        qflux = V3(tau[0], tau[1], tau[2]);
     }
     o->qflux_r[px] = V4(qflux.x, qflux.y, qflux.z, qr);
     o->qemi_n[px] = V4(QEN.x, QEN.y, QEN.z, qn);
