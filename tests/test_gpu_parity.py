@@ -77,7 +77,7 @@ def test_image_based_light_without_light_triangles():
     m = g.Mesh.from_arrays(base["positions"], mats, base["material"], base["normals"])
     m.set_camera((0, 0, 1.667), (0, 0, 0), 48, 48, 45.0).build()
     assert m.arrays()["light_triangle"].size == 0
-    _check(m, g.default_params(48, max_path_length=5), 7, "ibl")
+    _check(m, g.default_params(48, hash_width=64, hash_height=32, max_path_length=5), 7, "ibl")
 
 
 def test_cornell_c0_full_run():
