@@ -84,11 +84,24 @@ def main():
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--workload", default="spheres-1024", choices=sorted(WORKLOADS))
     ap.add_argument("--cpu-passes", type=int, default=10, help="passes of the CPU baseline sample (0: skip)")
+    ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
+                    help="first run a plain device copy of MIB MiB (torch), the known-size reference for "
+                         "rocprofv3 FETCH_SIZE/WRITE_SIZE counters")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (each box has one GPU); "
               "run with --gpus 1", file=sys.stderr)
         sys.exit(2)
+
+    if args.calibrate_copy > 0:
+        import torch
+        n = args.calibrate_copy * (1 << 20) // 4
+        src = torch.ones(n, dtype=torch.float32, device="cuda:0")
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
 
     import glslppm_amd as g
     w = WORKLOADS[args.workload]

@@ -2,7 +2,7 @@
 //
 // HBM layout (all arrays 16-byte aligned, structure-of-arrays so that a wave's 64 lanes read
 // 1 KiB contiguous per field):
-//   scene:  nodes[6][N][2] float4  per cube face, in that face's pre-order:
+//   scene:  nodes[6][N+1][2] float4 per cube face, in that face's pre-order (+1 padding node):
 //                                  (min.xyz, miss position), (max.xyz, leaf triangle or -1);
 //                                  the hit link is the next position, so it is not stored.
 //           tri_isect[T][3] float4 (v0, brdf), (v1 - v0, material), (v2 - v0, 0)
@@ -28,6 +28,8 @@ struct DevScene {
     int num_lights;
     float light_area;
     float bsphere_x, bsphere_y, bsphere_z, bsphere_r;
+    float cube_lo_x, cube_lo_y, cube_lo_z;  // bounding cube of the scene for the ray buckets
+    float cube_cells;                        // cells per unit length
 };
 
 struct DevCamera {
@@ -48,9 +50,10 @@ struct DevGlobals {
     float num_photons_f;
     unsigned long long photons_emitted;
     unsigned int emitted_this_pass;
-    unsigned int ticket_eye;
-    unsigned int ticket_photon;
-    unsigned int pad[3];
+    unsigned int pad1;
+    unsigned int pad2;
+    unsigned int queue_len;      // continuing photons of this pass (written by k_bucket_scan)
+    unsigned int pad0;
 };
 
 struct HitState {
@@ -75,6 +78,22 @@ struct PhotonState {
     int* slot;
     unsigned int* partial;  // [blocks] emitted counts
 };
+
+// Rays of the photons that continue this pass, compacted in queue order.
+struct RayQueue {
+    float4* org;   // [P] by photon slot: origin, flux.x
+    float4* dir;   // [P] direction, flux.y
+    int4* aux;     // [P] flux.z bits, bounce | emitted << 16, photon slot, 0
+    int2* key_rank;         // [P] coherence bucket (-1: photon ended) and rank inside it
+    int* perm;              // [rays] photon slots in bucket order
+    unsigned int* bucket;   // [kRayBuckets] counts, then (after the scan) first positions
+};
+
+// Coherence buckets of the photon rays: cube face x 8^3 cells of the scene's bounding cube.
+// Rays of one bucket start close together and walk the same face's tree, so a wave of them
+// reads far fewer distinct nodes than a wave of arbitrary photons.
+constexpr int kRayCellBits = 3;
+constexpr int kRayBuckets = 6 << (3 * kRayCellBits);
 
 struct HashState {
     unsigned int* count;
@@ -110,9 +129,20 @@ struct GatherParams {
     int draw;
 };
 
-__global__ void k_clear_slots(HashState h);
-__global__ void k_eye(DevScene s, DevCamera cam, EyeParams ep, HitState hs, DevGlobals* g);
-__global__ void k_photon(DevScene s, PhotonParams pp, PhotonState ps, HashState h, DevGlobals* g);
-__global__ void k_gather(GatherParams gp, HitState hs, PhotonState ps, HashState h, DevGlobals* g);
+// Pixels are processed in 16x16 tiles, 8x8 per wave, so that a wave's eye rays and hash cells
+// are neighbours; one 256-thread block per tile.
+struct Tiles {
+    int width, height;   // image
+    int tiles_x;         // ceil(width / 16)
+};
+
+__global__ void k_clear_slots(HashState h, RayQueue rq, DevGlobals* g);
+__global__ void k_bucket_scan(RayQueue rq, const unsigned int* partial, int n_partial, DevGlobals* g);
+__global__ void k_bucket_scatter(RayQueue rq, int count);
+__global__ void k_eye(DevScene s, DevCamera cam, EyeParams ep, Tiles tl, HitState hs, DevGlobals* g);
+__global__ void k_grid(EyeParams ep, const float* partial, int n_partial, DevGlobals* g);
+__global__ void k_photon_gen(DevScene s, PhotonParams pp, PhotonState ps, RayQueue rq);
+__global__ void k_photon_trace(DevScene s, PhotonParams pp, PhotonState ps, HashState h, RayQueue rq, DevGlobals* g);
+__global__ void k_gather(GatherParams gp, Tiles tl, HitState hs, PhotonState ps, HashState h, DevGlobals* g);
 __global__ void k_resolve(int count, HitState hs, const DevGlobals* g, float4* linear, float4* mapped);
 __global__ void k_init_state(int q, int p, HitState hs, PhotonState ps);

@@ -1,9 +1,13 @@
 // Kernels of the SPPM pass loop for gfx950.  One pass (m_display, glslppm.cpp:214-536) is:
-//   k_eye      every max_path passes: eyeraytrace.fs + the max.fs/min.fs reductions + the grid
-//              uniforms of glslppm.cpp:292-311, computed by the last block to finish
+//   k_eye      every max_path passes: eyeraytrace.fs + per-block max.fs/min.fs partials
+//   k_grid     the hit-point box and the grid uniforms of glslppm.cpp:292-311 (one block)
 //   k_clear_slots
-//   k_photon   photontrace.fs + sum.fs + hash.fs + scatter (atomic min of the photon index)
-//              + correction (atomic count), all in the photon's own thread
+//   k_photon_gen    photontrace.fs emission/bounce; continuing rays are bucketed by cube face and
+//                   origin cell (counting sort: LDS histogram per block)
+//   k_bucket_scan   bucket offsets + sum.fs and NumPhotons (one block)
+//   k_bucket_scatter
+//   k_photon_trace  photontrace.fs walk + deposit + hash.fs + scatter (atomic min of the photon
+//                   index) + correction (atomic count), one thread per ray in bucket order
 //   k_gather   progressive.fs (8-cell gather and the N/R/tau update) + re.fs draw to RGBA8
 // Arithmetic follows the GLSL statement by statement in strict fp32 (see fmath.h).
 
@@ -26,56 +30,74 @@ struct Hit {
     float bx, by, bz;  // Barycentric = uvt.ywx
 };
 
-// Closest hit over the threaded BVH of the ray's cube face (eyeraytrace.fs:67-254,
-// photontrace.fs:123-218).  Eye rays cull back faces on their first segment; photon rays skip
-// light triangles and start with isect.w = 1e30 instead of 1e20.
+// Box test of node i and, at a leaf, the triangle test (eyeraytrace.fs:92-147,
+// photontrace.fs:148-205); returns the next position: i + 1 on a box hit or after any leaf,
+// else the miss link.  Eye rays cull back faces on their first segment; photon rays skip light
+// triangles.
+template <bool PHOTON>
+PPM_DEV int visit(const DevScene& s, f3 org, f3 dir, f3 rcp, f3 t0, bool cull, float4 a, float4 b, int i, Hit& h) {
+    const float lx = rcp.x * a.x + t0.x, ly = rcp.y * a.y + t0.y, lz = rcp.z * a.z + t0.z;
+    const float hx = rcp.x * b.x + t0.x, hy = rcp.y * b.y + t0.y, hz = rcp.z * b.z + t0.z;
+    const float tmin = max1(max1(min1(lx, hx), min1(ly, hy)), min1(lz, hz));
+    const float tmax = min1(min1(max1(lx, hx), max1(ly, hy)), max1(lz, hz));
+    const bool box_hit = (tmin <= tmax) && (tmin <= h.t) && (tmax >= 0.0f);
+    const int tri = __float_as_int(b.w);
+    if (tri >= 0 && box_hit) {
+        const float4 q0 = s.tri_isect[3 * tri + 0];
+        const float4 q1 = s.tri_isect[3 * tri + 1];
+        const float4 q2 = s.tri_isect[3 * tri + 2];
+        const f3 e0 = xyz(q1), e1 = xyz(q2);
+        const f3 pv = cross3(dir, e1);
+        const float det = dot3(e0, pv);
+        if (!cull || det > 1e-10f) {
+            const f3 tv = org - xyz(q0);
+            const f3 qv = cross3(tv, e0);
+            const float u = dot3(tv, pv) / det;
+            const float v = dot3(dir, qv) / det;
+            const float t = dot3(e1, qv) / det;
+            const float w = (1.0f - u) - v;
+            bool ok = (u >= 0.0f) && (v >= 0.0f) && (t >= 0.0f) && (w >= 0.0f) && (t < h.t);
+            if (PHOTON) ok = ok && (__float_as_int(q0.w) != -1);
+            if (ok) {
+                h.tri = tri;
+                h.t = t;
+                h.bx = v;
+                h.by = w;
+                h.bz = u;
+            }
+        }
+    }
+    return box_hit ? i + 1 : __float_as_int(a.w);
+}
+
+// Closest hit over the threaded BVH of one cube face (eyeraytrace.fs:67-254,
+// photontrace.fs:123-218).  Nodes i and i + 1 are fetched together; when the walk continues at
+// i + 1 (55% of steps on the sphere scene) that node is visited without waiting for another
+// load.  The order of visits, and so every result including ties, is the reference's.  Each
+// face array ends with a padding node, so the second fetch stays in bounds.
+template <bool PHOTON>
+PPM_DEV void walk(const DevScene& s, int face, f3 org, f3 dir, f3 rcp, f3 t0, bool cull, Hit& h) {
+    const int n = s.num_nodes;
+    const float4* __restrict__ nd = s.nodes + (size_t)face * (size_t)(n + 1) * 2;
+    int i = 0;
+    while (i < n) {
+        const float4 a0 = nd[2 * i], b0 = nd[2 * i + 1];
+        const float4 a1 = nd[2 * i + 2], b1 = nd[2 * i + 3];
+        int next = visit<PHOTON>(s, org, dir, rcp, t0, cull, a0, b0, i, h);
+        if (next == i + 1 && next < n) next = visit<PHOTON>(s, org, dir, rcp, t0, cull, a1, b1, next, h);
+        i = next;
+    }
+}
+
 template <bool PHOTON>
 PPM_DEV Hit trace(const DevScene& s, f3 org, f3 dir, bool cull) {
-    const int n_nodes = s.num_nodes;
-    const float4* __restrict__ nd = s.nodes + (size_t)cube_face(dir) * (size_t)n_nodes * 2;
     const f3 rcp = mk3(1.0f / dir.x, 1.0f / dir.y, 1.0f / dir.z);
     const f3 t0 = mk3(-rcp.x * org.x, -rcp.y * org.y, -rcp.z * org.z);
     Hit h;
     h.tri = -1;
     h.t = PHOTON ? 1.0e30f : 1.0e20f;
     h.bx = h.by = h.bz = 0.0f;
-    int i = 0;
-    while (i < n_nodes) {
-        const float4 a = nd[2 * i];
-        const float4 b = nd[2 * i + 1];
-        const float lx = rcp.x * a.x + t0.x, ly = rcp.y * a.y + t0.y, lz = rcp.z * a.z + t0.z;
-        const float hx = rcp.x * b.x + t0.x, hy = rcp.y * b.y + t0.y, hz = rcp.z * b.z + t0.z;
-        const float tmin = max1(max1(min1(lx, hx), min1(ly, hy)), min1(lz, hz));
-        const float tmax = min1(min1(max1(lx, hx), max1(ly, hy)), max1(lz, hz));
-        const bool box_hit = (tmin <= tmax) && (tmin <= h.t) && (tmax >= 0.0f);
-        const int tri = __float_as_int(b.w);
-        if (tri >= 0 && box_hit) {
-            const float4 q0 = s.tri_isect[3 * tri + 0];
-            const float4 q1 = s.tri_isect[3 * tri + 1];
-            const float4 q2 = s.tri_isect[3 * tri + 2];
-            const f3 e0 = xyz(q1), e1 = xyz(q2);
-            const f3 pv = cross3(dir, e1);
-            const float det = dot3(e0, pv);
-            if (!cull || det > 1e-10f) {
-                const f3 tv = org - xyz(q0);
-                const f3 qv = cross3(tv, e0);
-                const float u = dot3(tv, pv) / det;
-                const float v = dot3(dir, qv) / det;
-                const float t = dot3(e1, qv) / det;
-                const float w = (1.0f - u) - v;
-                bool ok = (u >= 0.0f) && (v >= 0.0f) && (t >= 0.0f) && (w >= 0.0f) && (t < h.t);
-                if (PHOTON) ok = ok && (__float_as_int(q0.w) != -1);
-                if (ok) {
-                    h.tri = tri;
-                    h.t = t;
-                    h.bx = v;
-                    h.by = w;
-                    h.bz = u;
-                }
-            }
-        }
-        i = box_hit ? i + 1 : __float_as_int(a.w);
-    }
+    walk<PHOTON>(s, cube_face(dir), org, dir, rcp, t0, cull, h);
     return h;
 }
 
@@ -163,6 +185,15 @@ PPM_DEV unsigned int wave_sum(unsigned int v) {
 constexpr int kBlock = 256;
 constexpr int kWaves = kBlock / 64;
 
+// Pixel of this thread: block = 16x16 tile, wave = 8x8 quarter of it.
+PPM_DEV void tile_pixel(const Tiles& tl, int& i, int& j) {
+    const int tile = blockIdx.x;
+    const int tx = tile % tl.tiles_x, ty = tile / tl.tiles_x;
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    i = tx * 16 + (wave & 1) * 8 + (lane & 7);
+    j = ty * 16 + (wave >> 1) * 8 + (lane >> 3);
+}
+
 }  // namespace
 
 // ------------------------------------------------------------------------------------------
@@ -182,7 +213,10 @@ __global__ void k_init_state(int q, int p, HitState hs, PhotonState ps) {
     }
 }
 
-__global__ void k_clear_slots(HashState h) {
+__global__ void k_clear_slots(HashState h, RayQueue rq, DevGlobals* g) {
+    (void)g;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < kRayBuckets; i += gridDim.x * blockDim.x)
+        rq.bucket[i] = 0u;
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < h.size; i += gridDim.x * blockDim.x) {
         h.count[i] = 0u;
         h.rep[i] = 0xffffffffu;
@@ -190,14 +224,14 @@ __global__ void k_clear_slots(HashState h) {
 }
 
 // eyeraytrace.fs:396-554 for one pixel, then the min/max of hit positions.
-__global__ __launch_bounds__(kBlock) void k_eye(DevScene s, DevCamera cam, EyeParams ep, HitState hs,
+__global__ __launch_bounds__(kBlock) void k_eye(DevScene s, DevCamera cam, EyeParams ep, Tiles tl, HitState hs,
                                                 DevGlobals* g) {
-    const int q = ep.width * ep.height;
-    const int p = blockIdx.x * kBlock + threadIdx.x;
+    int px_i, px_j;
+    tile_pixel(tl, px_i, px_j);
+    const int p = px_j * ep.width + px_i;
     f3 pos = mk3(0.0f, 0.0f, 0.0f);
-    const bool active = p < q;
+    const bool active = px_i < ep.width && px_j < ep.height;
     if (active) {
-        const int px_i = p % ep.width, px_j = p / ep.width;
         float4 rndv = hs.rng[p];
         const f3 C = mk3(cam.ox, cam.oy, cam.oz);
         const f3 U = mk3(cam.ux, cam.uy, cam.uz), V = mk3(cam.vx, cam.vy, cam.vz), W = mk3(cam.wx, cam.wy, cam.wz);
@@ -301,9 +335,9 @@ __global__ __launch_bounds__(kBlock) void k_eye(DevScene s, DevCamera cam, EyePa
         hs.e_n[p] = en;
     }
 
-    // max.fs / min.fs over every pixel's position (misses contribute their last position)
+    // max.fs / min.fs over every pixel's position (misses contribute their last position): one
+    // partial per block, reduced by k_grid
     __shared__ float red[kWaves][6];
-    __shared__ bool last_block;
     float v[6] = {active ? pos.x : kInf, active ? pos.y : kInf, active ? pos.z : kInf,
                   active ? pos.x : -kInf, active ? pos.y : -kInf, active ? pos.z : -kInf};
     for (int k = 0; k < 3; k++) v[k] = wave_min(v[k]);
@@ -320,22 +354,22 @@ __global__ __launch_bounds__(kBlock) void k_eye(DevScene s, DevCamera cam, EyePa
             for (int k = 3; k < 6; k++) r[k] = max1(r[k], red[w][k]);
         }
         for (int k = 0; k < 6; k++) hs.partial[blockIdx.x * 6 + k] = r[k];
-        __threadfence();
-        const unsigned int t = atomicAdd(&g->ticket_eye, 1u);
-        last_block = (t == gridDim.x - 1);
     }
-    __syncthreads();
-    if (!last_block) return;
-    __threadfence();
+}
+
+// The hit-point bounding box from the eye blocks' partials, then the grid scalars of
+// glslppm.cpp:294-311 (one block of 1024 threads).
+__global__ __launch_bounds__(1024) void k_grid(EyeParams ep, const float* partial, int n_partial, DevGlobals* g) {
+    __shared__ float red[16][6];
     float r[6] = {kInf, kInf, kInf, -kInf, -kInf, -kInf};
-    for (int b = threadIdx.x; b < (int)gridDim.x; b += kBlock) {
-        const volatile float* pp = hs.partial + b * 6;
+    for (int b = threadIdx.x; b < n_partial; b += 1024) {
+        const float* pp = partial + b * 6;
         for (int k = 0; k < 3; k++) r[k] = min1(r[k], pp[k]);
         for (int k = 3; k < 6; k++) r[k] = max1(r[k], pp[k]);
     }
     for (int k = 0; k < 3; k++) r[k] = wave_min(r[k]);
     for (int k = 3; k < 6; k++) r[k] = wave_max(r[k]);
-    __syncthreads();
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     if (lane == 0)
         for (int k = 0; k < 6; k++) red[wave][k] = r[k];
     __syncthreads();
@@ -344,12 +378,11 @@ __global__ __launch_bounds__(kBlock) void k_eye(DevScene s, DevCamera cam, EyePa
         for (int k = 0; k < 3; k++) {
             bmin[k] = red[0][k];
             bmax[k] = red[0][k + 3];
-            for (int w = 1; w < kWaves; w++) {
+            for (int w = 1; w < 16; w++) {
                 bmin[k] = min1(bmin[k], red[w][k]);
                 bmax[k] = max1(bmax[k], red[w][k + 3]);
             }
         }
-        // glslppm.cpp:294-311
         float bbsize = 0.0f;
         for (int k = 0; k < 3; k++) bbsize += bmax[k] - bmin[k];
         float ir = (bbsize / 3.0f) / ep.footprint_width * ep.footprint;
@@ -362,8 +395,6 @@ __global__ __launch_bounds__(kBlock) void k_eye(DevScene s, DevCamera cam, EyePa
         g->initial_radius = ir;
         g->grid_scale = gs;
         g->hash_scale = 4194304.0f / gs;
-        g->ticket_eye = 0u;
-        __threadfence();
     }
 }
 
@@ -434,81 +465,69 @@ PPM_DEV void ibl_sample(const DevScene& s, float4& rndv, f3& flux, f3& dir, f3& 
 
 }  // namespace
 
-// photontrace.fs:464-720 for one photon slot, then hash.fs and the scatter/correction draws.
-__global__ __launch_bounds__(kBlock) void k_photon(DevScene s, PhotonParams pp, PhotonState ps, HashState h,
-                                                   DevGlobals* g) {
-    const int p = blockIdx.x * kBlock + threadIdx.x;
+// photontrace.fs:464-720, hash.fs and the scatter/correction draws, as two kernels.
+//
+// k_photon_gen runs one thread per photon slot: emission or bounce with Russian roulette
+// (photontrace.fs:466-686).  A photon that continues appends its ray to a queue (wave-aggregated
+// atomic); a terminated one writes its final state at once.  k_photon_trace is persistent: its
+// lanes take queue entries from a cursor, walk the BVH a few nodes at a time, and when a walk
+// ends deposit the photon (photontrace.fs:690-710, hash.fs, scatter, correction) and take the
+// next entry as soon as a quarter of the wave is idle -- a wave is no longer as slow as its
+// longest photon, and the walking kernel does not carry the sampling code's registers.  Each
+// photon's arithmetic is independent of which lane runs it and the slot table only sees
+// commutative atomics, so the result equals one thread per slot.
+
+namespace {
+
+// Emission or bounce of slot p.  Returns false when Russian roulette ends the photon, after
+// writing the terminated state (photontrace.fs:711-719).
+PPM_DEV bool photon_gen(const DevScene& s, const PhotonParams& pp, const PhotonState& ps, int p, f3& org_out,
+                        f3& dir_out, f3& flux_out, int& out_bits, unsigned int& emitted_out) {
+    float4 rndv = ps.rng[p];
+    const float4 ppos = ps.pos[p], pflux = ps.flux[p], pdir = ps.dir[p];
+    const f3 prev_nrm = mk3(ppos.w, pflux.w, pdir.w);
+    f3 flux = mk3(fabsf(pflux.x), fabsf(pflux.y), fabsf(pflux.z));
+    const int2 st = ps.state[p];
+    const int bounce = (st.y & 0xffff) + 1;
+    f3 org = mk3(0.0f, 0.0f, 0.0f), dir = mk3(0.0f, 0.0f, 0.0f);
+    bool cont;
     unsigned int emitted = 0;
-    if (p < pp.count) {
-        float4 rndv = ps.rng[p];
-        const float4 ppos = ps.pos[p], pflux = ps.flux[p], pdir = ps.dir[p];
-        const f3 prev_nrm = mk3(ppos.w, pflux.w, pdir.w);
-        f3 flux = mk3(fabsf(pflux.x), fabsf(pflux.y), fabsf(pflux.z));
-        const int2 st = ps.state[p];
-        const int bounce = (st.y & 0xffff) + 1;
-        f3 org = mk3(0.0f, 0.0f, 0.0f), dir = mk3(0.0f, 0.0f, 0.0f);
-        bool cont;
-        int out_bounce = bounce;
-        if (st.x < 0 || bounce >= pp.max_path) {
-            if (s.light_area != 0.0f) light_sample(s, rndv, flux, dir, org);
-            else ibl_sample(s, rndv, flux, dir, org);
-            emitted = 1;
-            out_bounce = 0;
-            cont = true;
-        } else {
-            const float4* tr = s.tri_shade + (size_t)6 * st.x;
-            const float4 q4 = tr[4], q5 = tr[5];
-            const f3 gnrm = mk3(q4.z, q4.w, q5.x);
-            const int mat = __float_as_int(q5.y);
-            const float4 m0 = s.mats[2 * mat], m1 = s.mats[2 * mat + 1];
-            f3 col = xyz(m0);
-            const int brdf = (int)m1.x;
-            const float eta = m0.w, gl = m1.y;
-            const f3 ipos = xyz(ppos);
-            f3 inrm = prev_nrm;
-            const f3 pd = xyz(pdir);
-            const float eps = 1e-6f;
-            if (brdf == 0) {
-                const float r0 = 6.283184f * gpu_rnd(rndv);
-                const float r1 = sqrtf(gpu_rnd(rndv));
-                const f3 v = mk3(ppm_sinf(r0) * r1, sqrtf(1.0f - r1 * r1), ppm_cosf(r0) * r1);
-                org = ipos + eps * gnrm;
-                dir = onb(v, inrm);
-                if (dot3(dir, gnrm) < 0.0f) dir = -dir;
-            } else if (brdf == 1 || brdf == 4) {
-                org = ipos + eps * gnrm;
-                if (brdf == 4) inrm = glossy_reflect(inrm, inrm, gl, rndv);
-                dir = reflect3(pd, inrm);
-                if (dot3(dir, gnrm) < 0.0f) dir = -dir;
-            } else if (brdf == 2 || brdf == 5) {
-                if (brdf == 5) inrm = glossy_reflect(inrm, inrm, gl, rndv);
-                const float ln = dot3(inrm, pd);
-                if (ln < 0.0f) {
-                    const float re = fresnel(-pd, inrm, 1.0f, eta);
-                    if (gpu_rnd(rndv) < re) {
-                        org = ipos + eps * gnrm;
-                        dir = reflect3(pd, inrm);
-                        col = mk3(1.0f, 1.0f, 1.0f);
-                        if (dot3(dir, gnrm) < 0.0f) dir = -dir;
-                    } else {
-                        org = ipos - eps * gnrm;
-                        dir = refract3(pd, inrm, 1.0f / eta);
-                        if (dot3(dir, gnrm) > 0.0f) dir = -dir;
-                    }
-                } else {
-                    const float re = fresnel(-pd, -inrm, eta, 1.0f);
-                    if (gpu_rnd(rndv) < re) {
-                        org = ipos - eps * gnrm;
-                        dir = reflect3(pd, -inrm);
-                        if (dot3(dir, -gnrm) < 0.0f) dir = -dir;
-                    } else {
-                        org = ipos + eps * gnrm;
-                        dir = refract3(pd, -inrm, eta);
-                        if (dot3(dir, -gnrm) > 0.0f) dir = -dir;
-                    }
-                }
-            } else if (brdf == 3 || brdf == 6) {
-                if (brdf == 6) inrm = glossy_reflect(inrm, inrm, gl, rndv);
+    int out_bounce = bounce;
+    if (st.x < 0 || bounce >= pp.max_path) {
+        if (s.light_area != 0.0f) light_sample(s, rndv, flux, dir, org);
+        else ibl_sample(s, rndv, flux, dir, org);
+        emitted = 1;
+        out_bounce = 0;
+        cont = true;
+    } else {
+        const float4* tr = s.tri_shade + (size_t)6 * st.x;
+        const float4 q4 = tr[4], q5 = tr[5];
+        const f3 gnrm = mk3(q4.z, q4.w, q5.x);
+        const int mat = __float_as_int(q5.y);
+        const float4 m0 = s.mats[2 * mat], m1 = s.mats[2 * mat + 1];
+        f3 col = xyz(m0);
+        const int brdf = (int)m1.x;
+        const float eta = m0.w, gl = m1.y;
+        const f3 ipos = xyz(ppos);
+        f3 inrm = prev_nrm;
+        const f3 pd = xyz(pdir);
+        const float eps = 1e-6f;
+        if (brdf == 0) {
+            const float r0 = 6.283184f * gpu_rnd(rndv);
+            const float r1 = sqrtf(gpu_rnd(rndv));
+            const f3 v = mk3(ppm_sinf(r0) * r1, sqrtf(1.0f - r1 * r1), ppm_cosf(r0) * r1);
+            org = ipos + eps * gnrm;
+            dir = onb(v, inrm);
+            if (dot3(dir, gnrm) < 0.0f) dir = -dir;
+        } else if (brdf == 1 || brdf == 4) {
+            org = ipos + eps * gnrm;
+            if (brdf == 4) inrm = glossy_reflect(inrm, inrm, gl, rndv);
+            dir = reflect3(pd, inrm);
+            if (dot3(dir, gnrm) < 0.0f) dir = -dir;
+        } else if (brdf == 2 || brdf == 5) {
+            if (brdf == 5) inrm = glossy_reflect(inrm, inrm, gl, rndv);
+            const float ln = dot3(inrm, pd);
+            if (ln < 0.0f) {
                 const float re = fresnel(-pd, inrm, 1.0f, eta);
                 if (gpu_rnd(rndv) < re) {
                     org = ipos + eps * gnrm;
@@ -516,77 +535,169 @@ __global__ __launch_bounds__(kBlock) void k_photon(DevScene s, PhotonParams pp, 
                     col = mk3(1.0f, 1.0f, 1.0f);
                     if (dot3(dir, gnrm) < 0.0f) dir = -dir;
                 } else {
-                    const float r0 = 6.283184f * gpu_rnd(rndv);
-                    const float r1 = sqrtf(gpu_rnd(rndv));
-                    const f3 v = mk3(ppm_sinf(r0) * r1, sqrtf(1.0f - r1 * r1), ppm_cosf(r0) * r1);
-                    org = ipos + eps * gnrm;
-                    dir = onb(v, inrm);
-                    if (dot3(dir, gnrm) < 0.0f) dir = -dir;
+                    org = ipos - eps * gnrm;
+                    dir = refract3(pd, inrm, 1.0f / eta);
+                    if (dot3(dir, gnrm) > 0.0f) dir = -dir;
                 }
-            }
-            // Russian roulette
-            const float pr = max1(max1(col.x, col.y), col.z);
-            if (pr < gpu_rnd(rndv)) {
-                cont = false;
             } else {
-                cont = true;
-                col = col / pr;
-                flux = flux * col;
-            }
-        }
-
-        int slot = -1;
-        if (cont) {
-            const Hit hh = trace<true>(s, org, dir, false);
-            f3 hpos = mk3(0.0f, 0.0f, 0.0f), hnrm = mk3(0.0f, 0.0f, 0.0f), hgn = mk3(0.0f, 0.0f, 0.0f);
-            int hbrdf = 0;
-            if (hh.tri >= 0) {
-                const Surface i = shade(s, hh);
-                hpos = i.pos;
-                hnrm = i.nrm;
-                hgn = i.gnrm;
-                hbrdf = i.brdf;
-            }
-            // flux = -flux on a miss or a non-diffuse hit, as a sign-bit flip selected without a
-            // branch (a divergent short-circuit here was compiled into a lost negation)
-            const bool no_deposit = (hh.tri < 0) | ((hbrdf != 0) & (hbrdf != 6) & (hbrdf != 3));
-            const unsigned int sign_bit = no_deposit ? 0x80000000u : 0u;
-            flux = mk3(__uint_as_float(__float_as_uint(flux.x) ^ sign_bit),
-                       __uint_as_float(__float_as_uint(flux.y) ^ sign_bit),
-                       __uint_as_float(__float_as_uint(flux.z) ^ sign_bit));
-            // a miss has no geometric normal: 0/0 is NaN and the NaN-rejecting min gives sqrt(5)
-            const float gf = min1(fabsf(dot3(dir, hnrm) / dot3(dir, hgn)), sqrtf(5.0f));
-            flux = flux * gf;
-            ps.pos[p] = make_float4(hpos.x, hpos.y, hpos.z, hnrm.x);
-            ps.flux[p] = make_float4(flux.x, flux.y, flux.z, hnrm.y);
-            ps.dir[p] = make_float4(dir.x, dir.y, dir.z, hnrm.z);
-            ps.state[p] = make_int2(hh.tri, out_bounce | (int)(emitted << 16));
-            // hash.fs: deposits with flux.x >= 0 (including -0) go to the grid
-            if (!(flux.x < 0.0f)) {
-                const f3 cell = mk3(floorf((hpos.x - g->bbmin[0]) * g->grid_scale),
-                                    floorf((hpos.y - g->bbmin[1]) * g->grid_scale),
-                                    floorf((hpos.z - g->bbmin[2]) * g->grid_scale));
-                const float sf = cell_hash(cell, g->grid_scale, g->hash_scale, h.num_f);
-                // a slot past the last row lands outside the hash viewport and is clipped
-                if (sf < h.num_f) {
-                    slot = (int)sf;
-                    atomicAdd(h.count + slot, 1u);
-                    if (p < pp.rep_limit) atomicMin(h.rep + slot, (unsigned int)p);
+                const float re = fresnel(-pd, -inrm, eta, 1.0f);
+                if (gpu_rnd(rndv) < re) {
+                    org = ipos - eps * gnrm;
+                    dir = reflect3(pd, -inrm);
+                    if (dot3(dir, -gnrm) < 0.0f) dir = -dir;
+                } else {
+                    org = ipos + eps * gnrm;
+                    dir = refract3(pd, -inrm, eta);
+                    if (dot3(dir, -gnrm) > 0.0f) dir = -dir;
                 }
             }
-        } else {
-            ps.pos[p] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-            ps.flux[p] = make_float4(-1.0f, -1.0f, -1.0f, 0.0f);
-            ps.dir[p] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-            ps.state[p] = make_int2(-1, (int)(emitted << 16));
+        } else if (brdf == 3 || brdf == 6) {
+            if (brdf == 6) inrm = glossy_reflect(inrm, inrm, gl, rndv);
+            const float re = fresnel(-pd, inrm, 1.0f, eta);
+            if (gpu_rnd(rndv) < re) {
+                org = ipos + eps * gnrm;
+                dir = reflect3(pd, inrm);
+                col = mk3(1.0f, 1.0f, 1.0f);
+                if (dot3(dir, gnrm) < 0.0f) dir = -dir;
+            } else {
+                const float r0 = 6.283184f * gpu_rnd(rndv);
+                const float r1 = sqrtf(gpu_rnd(rndv));
+                const f3 v = mk3(ppm_sinf(r0) * r1, sqrtf(1.0f - r1 * r1), ppm_cosf(r0) * r1);
+                org = ipos + eps * gnrm;
+                dir = onb(v, inrm);
+                if (dot3(dir, gnrm) < 0.0f) dir = -dir;
+            }
         }
-        ps.rng[p] = rndv;
-        ps.slot[p] = slot;
+        // Russian roulette
+        const float pr = max1(max1(col.x, col.y), col.z);
+        if (pr < gpu_rnd(rndv)) {
+            cont = false;
+        } else {
+            cont = true;
+            col = col / pr;
+            flux = flux * col;
+        }
     }
+    emitted_out = emitted;
+    ps.rng[p] = rndv;
+    if (!cont) {
+        ps.pos[p] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        ps.flux[p] = make_float4(-1.0f, -1.0f, -1.0f, 0.0f);
+        ps.dir[p] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        ps.state[p] = make_int2(-1, (int)(emitted << 16));
+        ps.slot[p] = -1;
+        return false;
+    }
+    org_out = org;
+    dir_out = dir;
+    flux_out = flux;
+    out_bits = out_bounce | (int)(emitted << 16);
+    return true;
+}
 
-    // sum.fs over the emitted flags, NumPhotons += int(sum) by the last block (glslppm.cpp:396-397)
+struct Walk {
+    f3 org, dir, rcp, t0;
+    int face;
+    Hit h;
+};
+
+// Deposit (photontrace.fs:690-710), hash.fs and the scatter/correction updates of slot p.
+PPM_DEV void photon_finish(const DevScene& s, const PhotonParams& pp, const PhotonState& ps, const HashState& h,
+                           const DevGlobals* g, int p, const Walk& W, f3 flux, int out_bits) {
+    const f3 dir = W.dir;
+    f3 hpos = mk3(0.0f, 0.0f, 0.0f), hnrm = mk3(0.0f, 0.0f, 0.0f), hgn = mk3(0.0f, 0.0f, 0.0f);
+    int hbrdf = 0;
+    if (W.h.tri >= 0) {
+        const Surface i = shade(s, W.h);
+        hpos = i.pos;
+        hnrm = i.nrm;
+        hgn = i.gnrm;
+        hbrdf = i.brdf;
+    }
+    // flux = -flux on a miss or a non-diffuse hit, as a sign-bit flip selected without a
+    // branch (a divergent short-circuit here was compiled into a lost negation)
+    const bool no_deposit = (W.h.tri < 0) | ((hbrdf != 0) & (hbrdf != 6) & (hbrdf != 3));
+    const unsigned int sign_bit = no_deposit ? 0x80000000u : 0u;
+    flux = mk3(__uint_as_float(__float_as_uint(flux.x) ^ sign_bit),
+               __uint_as_float(__float_as_uint(flux.y) ^ sign_bit),
+               __uint_as_float(__float_as_uint(flux.z) ^ sign_bit));
+    // a miss has no geometric normal: 0/0 is NaN and the NaN-rejecting min gives sqrt(5)
+    const float gf = min1(fabsf(dot3(dir, hnrm) / dot3(dir, hgn)), sqrtf(5.0f));
+    flux = flux * gf;
+    ps.pos[p] = make_float4(hpos.x, hpos.y, hpos.z, hnrm.x);
+    ps.flux[p] = make_float4(flux.x, flux.y, flux.z, hnrm.y);
+    ps.dir[p] = make_float4(dir.x, dir.y, dir.z, hnrm.z);
+    ps.state[p] = make_int2(W.h.tri, out_bits);
+    int slot = -1;
+    // hash.fs: deposits with flux.x >= 0 (including -0) go to the grid
+    if (!(flux.x < 0.0f)) {
+        const f3 cell = mk3(floorf((hpos.x - g->bbmin[0]) * g->grid_scale),
+                            floorf((hpos.y - g->bbmin[1]) * g->grid_scale),
+                            floorf((hpos.z - g->bbmin[2]) * g->grid_scale));
+        const float sf = cell_hash(cell, g->grid_scale, g->hash_scale, h.num_f);
+        // a slot past the last row lands outside the hash viewport and is clipped
+        if (sf < h.num_f) {
+            slot = (int)sf;
+            atomicAdd(h.count + slot, 1u);
+            if (p < pp.rep_limit) atomicMin(h.rep + slot, (unsigned int)p);
+        }
+    }
+    ps.slot[p] = slot;
+}
+
+constexpr int kPhotonStepChunk = 4;
+
+// Cube face and origin cell of a ray, interleaved as face | morton(cell).
+PPM_DEV int ray_bucket(const DevScene& s, f3 org, f3 dir) {
+    const int n = 1 << kRayCellBits;
+    int c[3];
+    const float o[3] = {(org.x - s.cube_lo_x) * s.cube_cells, (org.y - s.cube_lo_y) * s.cube_cells,
+                        (org.z - s.cube_lo_z) * s.cube_cells};
+    for (int k = 0; k < 3; k++) {
+        const int v = (int)floorf(o[k]);
+        c[k] = v < 0 ? 0 : (v > n - 1 ? n - 1 : v);
+    }
+    int code = 0;
+    for (int b = 0; b < kRayCellBits; b++)
+        code |= (((c[0] >> b) & 1) << (3 * b)) | (((c[1] >> b) & 1) << (3 * b + 1)) | (((c[2] >> b) & 1) << (3 * b + 2));
+    return (cube_face(dir) << (3 * kRayCellBits)) | code;
+}
+
+PPM_DEV unsigned long long lanes_below(int lane) { return lane == 0 ? 0ull : (~0ull >> (64 - lane)); }
+
+}  // namespace
+
+__global__ __launch_bounds__(kBlock) void k_photon_gen(DevScene s, PhotonParams pp, PhotonState ps, RayQueue rq) {
+    const int p = blockIdx.x * kBlock + threadIdx.x;
+    unsigned int emitted = 0;
+    bool live = false;
+    f3 org, dir, flux;
+    int out_bits = 0;
+    if (p < pp.count) live = photon_gen(s, pp, ps, p, org, dir, flux, out_bits, emitted);
+    int key = -1;
+    if (live) {
+        rq.org[p] = make_float4(org.x, org.y, org.z, flux.x);
+        rq.dir[p] = make_float4(dir.x, dir.y, dir.z, flux.y);
+        rq.aux[p] = make_int4(__float_as_int(flux.z), out_bits, p, 0);
+        key = ray_bucket(s, org, dir);
+    }
+    // counting sort of the continuing rays by coherence bucket: ranks inside the block from an
+    // LDS histogram, then one global atomic per non-empty bucket per block (spread over the
+    // buckets; no counter is shared by all blocks)
+    __shared__ unsigned int hist[kRayBuckets];
     __shared__ unsigned int red[kWaves];
-    __shared__ bool last_block;
+    for (int b = threadIdx.x; b < kRayBuckets; b += kBlock) hist[b] = 0u;
+    __syncthreads();
+    unsigned int local = 0;
+    if (live) local = atomicAdd(&hist[key], 1u);
+    __syncthreads();
+    for (int b = threadIdx.x; b < kRayBuckets; b += kBlock) {
+        const unsigned int c = hist[b];
+        if (c) hist[b] = atomicAdd(&rq.bucket[b], c);
+    }
+    __syncthreads();
+    if (p < pp.count) rq.key_rank[p] = make_int2(key, live ? (int)(hist[key] + local) : 0);
+    // sum.fs: one partial per block, added up by k_bucket_scan
     const unsigned int ws = wave_sum(emitted);
     if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = ws;
     __syncthreads();
@@ -594,28 +705,83 @@ __global__ __launch_bounds__(kBlock) void k_photon(DevScene s, PhotonParams pp, 
         unsigned int tot = 0;
         for (int w = 0; w < kWaves; w++) tot += red[w];
         ps.partial[blockIdx.x] = tot;
-        __threadfence();
-        const unsigned int t = atomicAdd(&g->ticket_photon, 1u);
-        last_block = (t == gridDim.x - 1);
     }
-    __syncthreads();
-    if (!last_block) return;
-    __threadfence();
-    unsigned int sum = 0;
-    for (int b = threadIdx.x; b < (int)gridDim.x; b += kBlock) sum += ((volatile unsigned int*)ps.partial)[b];
-    sum = wave_sum(sum);
-    __syncthreads();
-    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = sum;
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        unsigned int tot = 0;
-        for (int w = 0; w < kWaves; w++) tot += red[w];
-        g->num_photons_f = g->num_photons_f + (float)(int)tot;
-        g->photons_emitted += tot;
-        g->emitted_this_pass = tot;
-        g->ticket_photon = 0u;
-        __threadfence();
+}
+
+// Exclusive scan of the bucket counts into first positions, the number of continuing rays, and
+// NumPhotons += int(sum of emitted flags) (glslppm.cpp:396-397); one block of 1024 threads.
+__global__ __launch_bounds__(1024) void k_bucket_scan(RayQueue rq, const unsigned int* partial, int n_partial,
+                                                      DevGlobals* g) {
+    constexpr int per = (kRayBuckets + 1023) / 1024;
+    __shared__ unsigned int wsum[16];
+    __shared__ unsigned int esum[16];
+    const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
+    unsigned int v[per];
+    unsigned int local = 0;
+    for (int k = 0; k < per; k++) {
+        const int b = t * per + k;
+        v[k] = b < kRayBuckets ? rq.bucket[b] : 0u;
+        local += v[k];
     }
+    unsigned int inc = local;
+    for (int o = 1; o < 64; o <<= 1) {
+        const unsigned int y = __shfl_up(inc, o, 64);
+        if (lane >= o) inc += y;
+    }
+    unsigned int e = 0;
+    for (int b = t; b < n_partial; b += 1024) e += partial[b];
+    e = wave_sum(e);
+    if (lane == 63) wsum[wave] = inc;
+    if (lane == 0) esum[wave] = e;
+    __syncthreads();
+    unsigned int before = 0;
+    for (int w = 0; w < wave; w++) before += wsum[w];
+    unsigned int run = before + inc - local;
+    for (int k = 0; k < per; k++) {
+        const int b = t * per + k;
+        if (b < kRayBuckets) rq.bucket[b] = run;
+        run += v[k];
+    }
+    if (t == 0) {
+        unsigned int total = 0, emitted = 0;
+        for (int w = 0; w < 16; w++) {
+            total += wsum[w];
+            emitted += esum[w];
+        }
+        g->queue_len = total;
+        g->num_photons_f = g->num_photons_f + (float)(int)emitted;
+        g->photons_emitted += emitted;
+        g->emitted_this_pass = emitted;
+    }
+}
+
+// Places every continuing ray at its bucket position.
+__global__ void k_bucket_scatter(RayQueue rq, int count) {
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= count) return;
+    const int2 kr = rq.key_rank[p];
+    if (kr.x >= 0) rq.perm[rq.bucket[kr.x] + (unsigned int)kr.y] = p;
+}
+
+__global__ __launch_bounds__(kBlock) void k_photon_trace(DevScene s, PhotonParams pp, PhotonState ps, HashState h,
+                                                         RayQueue rq, DevGlobals* g) {
+    const unsigned int n_rays = g->queue_len;
+    const unsigned int t = blockIdx.x * kBlock + threadIdx.x;
+    if (t >= n_rays) return;
+    const int p = rq.perm[t];
+    const float4 o = rq.org[p], d = rq.dir[p];
+    const int4 x = rq.aux[p];
+    Walk W;
+    W.org = xyz(o);
+    W.dir = xyz(d);
+    W.rcp = mk3(1.0f / d.x, 1.0f / d.y, 1.0f / d.z);
+    W.t0 = mk3(-W.rcp.x * o.x, -W.rcp.y * o.y, -W.rcp.z * o.z);
+    W.face = cube_face(W.dir);
+    W.h.tri = -1;
+    W.h.t = 1.0e30f;
+    W.h.bx = W.h.by = W.h.bz = 0.0f;
+    walk<true>(s, W.face, W.org, W.dir, W.rcp, W.t0, false, W.h);
+    photon_finish(s, pp, ps, h, g, x.z, W, mk3(o.w, d.w, __int_as_float(x.x)), x.y);
 }
 
 namespace {
@@ -646,10 +812,12 @@ PPM_DEV unsigned char unorm8(float c) {
 }  // namespace
 
 // progressive.fs:98-142 for one hit point, then the re.fs draw of that pixel.
-__global__ __launch_bounds__(kBlock) void k_gather(GatherParams gp, HitState hs, PhotonState ps, HashState h,
-                                                   DevGlobals* g) {
-    const int p = blockIdx.x * kBlock + threadIdx.x;
-    if (p >= gp.count) return;
+__global__ __launch_bounds__(kBlock) void k_gather(GatherParams gp, Tiles tl, HitState hs, PhotonState ps,
+                                                   HashState h, DevGlobals* g) {
+    int px_i, px_j;
+    tile_pixel(tl, px_i, px_j);
+    if (px_i >= tl.width || px_j >= tl.height) return;
+    const int p = px_j * tl.width + px_i;
     float4 tr = gp.first_pass ? make_float4(0.0f, 0.0f, 0.0f, g->initial_radius) : hs.tau_r[p];
     float4 en = hs.e_n[p];
     if (hs.tri[p] >= 0) {

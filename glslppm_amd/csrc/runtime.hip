@@ -26,11 +26,13 @@ struct ppm_ctx {
     ppm_params prm;
     int Q = 0, P = 0, H = 0, N = 0, T = 0, M = 0, NL = 0;
     int eye_blocks = 0, photon_blocks = 0;
+    Tiles tiles;
     DevScene scene;
     DevCamera cam;
     HitState hit;
     PhotonState ph;
     HashState hash;
+    RayQueue rays;
     DevGlobals* glob = nullptr;
     float4* resolve_lin = nullptr;
     float4* resolve_map = nullptr;
@@ -141,10 +143,11 @@ int check_scene(const ppm_scene* s) {
     return PPM_OK;
 }
 
-// Per-face pre-order node records; the hit link of position i is i + 1.
+// Per-face pre-order node records; the hit link of position i is i + 1.  Each face array has
+// N + 1 records: the last is padding that the walk's look-ahead fetch may read but never visits.
 int pack_nodes(const ppm_scene* s, std::vector<float4>& out) {
     const int N = s->num_nodes;
-    out.assign((size_t)6 * N * 2, make_float4(0, 0, 0, 0));
+    out.assign((size_t)6 * (N + 1) * 2, make_float4(0, 0, 0, 0));
     std::vector<int> order(N), pos(N);
     for (int f = 0; f < 6; f++) {
         const int32_t* L = s->node_links + (size_t)f * N * 2;
@@ -163,9 +166,11 @@ int pack_nodes(const ppm_scene* s, std::vector<float4>& out) {
             const int mpos = miss < 0 ? N : pos[miss];
             if (miss >= 0 && mpos <= i) return set_err(nullptr, PPM_E_INVAL, "BVH miss link points backwards");
             const float* b = s->node_bounds + (size_t)nk * 6;
-            out[((size_t)f * N + i) * 2 + 0] = make_float4(b[0], b[1], b[2], as_float(mpos));
-            out[((size_t)f * N + i) * 2 + 1] = make_float4(b[3], b[4], b[5], as_float(s->node_triangle[nk]));
+            out[((size_t)f * (N + 1) + i) * 2 + 0] = make_float4(b[0], b[1], b[2], as_float(mpos));
+            out[((size_t)f * (N + 1) + i) * 2 + 1] = make_float4(b[3], b[4], b[5], as_float(s->node_triangle[nk]));
         }
+        out[((size_t)f * (N + 1) + N) * 2 + 0] = make_float4(0.0f, 0.0f, 0.0f, as_float(N + 1));
+        out[((size_t)f * (N + 1) + N) * 2 + 1] = make_float4(0.0f, 0.0f, 0.0f, as_float(-1));
     }
     return PPM_OK;
 }
@@ -322,8 +327,16 @@ int create_impl(ppm_ctx* c, const ppm_scene* s, const ppm_params* p) {
     c->cam = DevCamera{k.origin[0], k.origin[1], k.origin[2], k.u[0], k.u[1], k.u[2], k.v[0], k.v[1], k.v[2],
                        k.w[0],      k.w[1],      k.w[2],      k.half_width, k.half_height, k.distance};
 
-    c->eye_blocks = (c->Q + 255) / 256;
+    c->tiles.width = p->image_width;
+    c->tiles.height = p->image_height;
+    c->tiles.tiles_x = (p->image_width + 15) / 16;
+    c->eye_blocks = c->tiles.tiles_x * ((p->image_height + 15) / 16);
     c->photon_blocks = (c->P + 255) / 256;
+    // bounding cube of the scene for the photon ray buckets
+    c->scene.cube_lo_x = s->bsphere[0] - s->bsphere[3];
+    c->scene.cube_lo_y = s->bsphere[1] - s->bsphere[3];
+    c->scene.cube_lo_z = s->bsphere[2] - s->bsphere[3];
+    c->scene.cube_cells = s->bsphere[3] > 0.0f ? (float)(1 << kRayCellBits) / (2.0f * s->bsphere[3]) : 0.0f;
     const size_t Q = c->Q, P = c->P, H = c->H;
     HitState& hs = c->hit;
     PhotonState& ps = c->ph;
@@ -333,7 +346,10 @@ int create_impl(ppm_ctx* c, const ppm_scene* s, const ppm_params* p) {
         (rc = dev_alloc(c, &hs.partial, (size_t)c->eye_blocks * 6)) || (rc = dev_alloc(c, &ps.pos, P)) ||
         (rc = dev_alloc(c, &ps.flux, P)) || (rc = dev_alloc(c, &ps.dir, P)) || (rc = dev_alloc(c, &ps.rng, P)) ||
         (rc = dev_alloc(c, &ps.state, P)) || (rc = dev_alloc(c, &ps.slot, P)) ||
-        (rc = dev_alloc(c, &ps.partial, (size_t)c->photon_blocks)) || (rc = dev_alloc(c, &c->hash.count, H)) ||
+        (rc = dev_alloc(c, &ps.partial, (size_t)c->photon_blocks)) ||
+        (rc = dev_alloc(c, &c->rays.org, P)) || (rc = dev_alloc(c, &c->rays.dir, P)) ||
+        (rc = dev_alloc(c, &c->rays.aux, P)) || (rc = dev_alloc(c, &c->rays.key_rank, P)) ||
+        (rc = dev_alloc(c, &c->rays.perm, P)) || (rc = dev_alloc(c, &c->rays.bucket, (size_t)kRayBuckets)) || (rc = dev_alloc(c, &c->hash.count, H)) ||
         (rc = dev_alloc(c, &c->hash.rep, H)) || (rc = dev_alloc(c, &c->glob, 1)) ||
         (rc = dev_alloc(c, &c->resolve_lin, Q)) || (rc = dev_alloc(c, &c->resolve_map, Q)))
         return rc;
@@ -355,7 +371,7 @@ int create_impl(ppm_ctx* c, const ppm_scene* s, const ppm_params* p) {
     hipLaunchKernelGGL(k_init_state, dim3((n + 255) / 256), dim3(256), 0, c->stream, c->Q, c->P, hs, ps);
     HIP_TRY(c, hipGetLastError());
     hipLaunchKernelGGL(k_clear_slots, dim3(std::min<int>((c->H + 255) / 256, 65536)), dim3(256), 0, c->stream,
-                       c->hash);
+                       c->hash, c->rays, c->glob);
     HIP_TRY(c, hipGetLastError());
     HIP_TRY(c, hipStreamSynchronize(c->stream));
     return PPM_OK;
@@ -445,13 +461,15 @@ int ppm_pass(ppm_ctx* c, uint32_t npasses) {
             ep.footprint_width = (float)p.image_width;
             c->eye_samples++;
             hipEvent_t e0 = prof_begin(c);
-            hipLaunchKernelGGL(k_eye, dim3(c->eye_blocks), dim3(256), 0, c->stream, c->scene, c->cam, ep, c->hit,
-                               c->glob);
+            hipLaunchKernelGGL(k_eye, dim3(c->eye_blocks), dim3(256), 0, c->stream, c->scene, c->cam, ep, c->tiles,
+                               c->hit, c->glob);
+            hipLaunchKernelGGL(k_grid, dim3(1), dim3(1024), 0, c->stream, ep, (const float*)c->hit.partial,
+                               c->eye_blocks, c->glob);
             prof_end(c, PPM_KERNEL_EYE, e0);
         }
         hipEvent_t e1 = prof_begin(c);
         hipLaunchKernelGGL(k_clear_slots, dim3(std::min<int>((c->H + 255) / 256, 65536)), dim3(256), 0, c->stream,
-                           c->hash);
+                           c->hash, c->rays, c->glob);
         prof_end(c, PPM_KERNEL_CLEAR, e1);
         PhotonParams pp;
         pp.count = c->P;
@@ -465,8 +483,13 @@ int ppm_pass(ppm_ctx* c, uint32_t npasses) {
             pp.rep_limit = (int)(c->P - std::max(0ll, ex));
         }
         hipEvent_t e2 = prof_begin(c);
-        hipLaunchKernelGGL(k_photon, dim3(c->photon_blocks), dim3(256), 0, c->stream, c->scene, pp, c->ph, c->hash,
-                           c->glob);
+        hipLaunchKernelGGL(k_photon_gen, dim3(c->photon_blocks), dim3(256), 0, c->stream, c->scene, pp, c->ph,
+                           c->rays);
+        hipLaunchKernelGGL(k_bucket_scan, dim3(1), dim3(1024), 0, c->stream, c->rays, (const unsigned int*)c->ph.partial,
+                           c->photon_blocks, c->glob);
+        hipLaunchKernelGGL(k_bucket_scatter, dim3(c->photon_blocks), dim3(256), 0, c->stream, c->rays, c->P);
+        hipLaunchKernelGGL(k_photon_trace, dim3(c->photon_blocks), dim3(256), 0, c->stream, c->scene, pp, c->ph,
+                           c->hash, c->rays, c->glob);
         prof_end(c, PPM_KERNEL_PHOTON, e2);
         GatherParams gp;
         gp.count = c->Q;
@@ -474,8 +497,8 @@ int ppm_pass(ppm_ctx* c, uint32_t npasses) {
         gp.first_pass = c->frame == 0 ? 1 : 0;
         gp.draw = (p.flags & PPM_FLAG_NO_DRAW) ? 0 : 1;
         hipEvent_t e3 = prof_begin(c);
-        hipLaunchKernelGGL(k_gather, dim3(c->eye_blocks), dim3(256), 0, c->stream, gp, c->hit, c->ph, c->hash,
-                           c->glob);
+        hipLaunchKernelGGL(k_gather, dim3(c->eye_blocks), dim3(256), 0, c->stream, gp, c->tiles, c->hit, c->ph,
+                           c->hash, c->glob);
         prof_end(c, PPM_KERNEL_GATHER, e3);
         c->frame++;
     }
