@@ -46,6 +46,117 @@ WORKLOADS = {
 }
 
 
+BATCH_WORKLOADS = {
+    "batch-mix64": "64 seeded scenes (room + icospheres), triangles log-uniform 1k-500k, resolution uniform "
+                   "over 128^2/256^2/512^2/1024^2, P = Q, H = Q, 10 bounces, passes uniform 64-256",
+    "batch-cornell64": "64 Cornell rooms of 36 triangles with distinct generator seeds: Q=P=H=256^2, 8 bounces, "
+                       "radius 1/50 of the room, 64 passes each",
+}
+
+
+def batch_jobs(name, n_jobs=64):
+    """(mesh, params, passes) of every job of a batch workload; scenes built on a thread pool."""
+    import concurrent.futures as cf
+
+    import numpy as np
+
+    import glslppm_amd as g
+    specs = []
+    for k in range(n_jobs):
+        if name == "batch-cornell64":
+            specs.append(dict(kind=0, seed=0, count=0, detail=0, res=256, L=8, passes=64, fixed_radius=0.02,
+                              xs=k))
+        else:
+            rs = np.random.RandomState(0x67736C70 + k)
+            T = float(np.exp(rs.uniform(np.log(1000.0), np.log(500000.0))))
+            detail = 3 if T < 20000 else 4
+            count = max(1, int(round((T - 36) / (20 * 4 ** detail))))
+            res = int(rs.choice([128, 256, 512, 1024]))
+            specs.append(dict(kind=1, seed=0x67736C70 + k, count=count, detail=detail, res=res, L=10,
+                              passes=int(rs.randint(64, 257)), fixed_radius=0.0, xs=k))
+
+    def make(sp):
+        m = g.Mesh.generate(sp["kind"], seed=sp["seed"], count=sp["count"], detail=sp["detail"])
+        m.set_camera((0, 0, 1.667), (0, 0, 0), sp["res"], sp["res"], 45.0).build()
+        kw = dict(max_path_length=sp["L"])
+        if sp["fixed_radius"]:
+            kw["fixed_radius"] = sp["fixed_radius"]
+        p = g.default_params(sp["res"], **kw)
+        p.xorshift[0] = (123456789 + 7919 * sp["xs"]) & 0xFFFFFFFF
+        return m, p, sp["passes"]
+
+    with cf.ThreadPoolExecutor(max_workers=min(16, os.cpu_count() or 1)) as ex:
+        return list(ex.map(make, specs))
+
+
+def run_batch_workload(args):
+    """Aggregate photon segments/s of one batch against the plain one-scene loop, job after job,
+    over the same jobs and passes; scene upload and state creation are outside both timings."""
+    import glslppm_amd as g
+    t = time.time()
+    jobs = batch_jobs(args.workload)
+    build_s = time.time() - t
+    work = sum(p.photon_width * p.photon_height * n for (_, p, n) in jobs)
+    eye = sum(p.image_width * p.image_height * ((n + p.max_path_length - 1) // p.max_path_length) for (_, p, n) in jobs)
+    tris = [m.num_triangles for (m, _, _) in jobs]
+
+    # plain loop: one Renderer per job, its passes, synchronised
+    plain_s = 0.0
+    for (m, p, n) in jobs:
+        r = g.Renderer(m, p)
+        r.sync()
+        t0 = time.perf_counter()
+        r.run(n)
+        r.sync()
+        plain_s += time.perf_counter() - t0
+        r.close()
+
+    b = g.Batch(passes_per_graph=args.passes_per_graph)
+    ids = [b.add(m, p, n) for (m, p, n) in jobs]
+    most = max(n for (_, _, n) in jobs)
+    b.run(0)  # builds the tables and the graph
+    b.sync()
+    t0 = time.perf_counter()
+    b.run(most)
+    b.sync()
+    batch_s = time.perf_counter() - t0
+    for j, (_, _, n) in zip(ids, jobs):
+        assert b.progress(j) == (n, 0)
+    # spot check: the largest job equals itself run alone
+    big = max(range(len(jobs)), key=lambda i: jobs[i][1].photon_width * jobs[i][1].photon_height)
+    r = g.Renderer(jobs[big][0], jobs[big][1])
+    r.run(jobs[big][2])
+    same = bool((r.readback("flux_radius") == b.readback(ids[big], "flux_radius")).all()
+                and (r.readback("radiance") == b.readback(ids[big], "radiance")).all())
+    r.close()
+    hbm = b.device_bytes
+    b.close()
+    line = {
+        "metric": METRIC,
+        "value": work / batch_s,
+        "unit": "photons/s",
+        "n_gpus": 1,
+        "steps": most,
+        "warmup": 0,
+        "ms_per_step": batch_s * 1e3 / most,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (seeded scene generator, xorshift128 seeds of the reference)",
+        "config": {"workload": BATCH_WORKLOADS[args.workload], "jobs": len(jobs), "triangles_min": min(tris),
+                   "triangles_max": max(tris), "photon_segments": work, "eye_rays": eye,
+                   "passes_per_graph": args.passes_per_graph, "scene_build_s": round(build_s, 2),
+                   "batch_hbm_bytes": hbm},
+        "eye_rays_per_s": eye / batch_s,
+        "plain_loop": {"value": work / plain_s, "unit": "photons/s", "seconds": plain_s},
+        "batch_seconds": batch_s,
+        "speedup_vs_plain_loop": plain_s / batch_s,
+        "largest_job_equal_alone": same,
+    }
+    print(json.dumps(line), flush=True)
+
+
 def build_scene(w):
     import glslppm_amd as g
     m = g.Mesh.generate(w["kind"], seed=w["seed"], count=w["count"], detail=w["detail"])
@@ -82,7 +193,9 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=50)
     ap.add_argument("--warmup", type=int, default=10)
-    ap.add_argument("--workload", default="spheres-1024", choices=sorted(WORKLOADS))
+    ap.add_argument("--workload", default="spheres-1024", choices=sorted(WORKLOADS) + sorted(BATCH_WORKLOADS))
+    ap.add_argument("--passes-per-graph", type=int, default=1, help="batch workloads: passes per hipGraph")
+    ap.add_argument("--graph", action="store_true", help="plain workloads: replay passes as hipGraphs")
     ap.add_argument("--cpu-passes", type=int, default=10, help="passes of the CPU baseline sample (0: skip)")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="first run a plain device copy of MIB MiB (torch), the known-size reference for "
@@ -103,10 +216,16 @@ def main():
         torch.cuda.synchronize()
         del src, dst
 
+    if args.workload in BATCH_WORKLOADS:
+        run_batch_workload(args)
+        return
+
     import glslppm_amd as g
     w = WORKLOADS[args.workload]
     mesh, build_s = build_scene(w)
     params = params_for(w)
+    if args.graph:
+        params.flags |= g.FLAG_GRAPH
     r = g.Renderer(mesh, params)
     Q = params.image_width * params.image_height
     P = params.photon_width * params.photon_height

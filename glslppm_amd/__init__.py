@@ -26,6 +26,8 @@ SCENE_SPHERES = 1
 SCENE_TERRAIN = 2
 
 FLAG_NO_DRAW = 1
+FLAG_GRAPH = 2
+PASSES_UNLIMITED = 0xFFFFFFFF
 
 BRDF = {"light": -1, "diffuse": 0, "mirror": 1, "glass": 2, "plastic": 3, "glossy_mirror": 4,
         "glossy_glass": 5, "glossy_plastic": 6}
@@ -66,6 +68,10 @@ class Params(ctypes.Structure):
                 ("initial_footprint", ctypes.c_float), ("fixed_radius", ctypes.c_float),
                 ("focal_length", ctypes.c_float), ("aperture", ctypes.c_float),
                 ("xorshift", ctypes.c_uint32 * 4), ("device", ctypes.c_int32), ("flags", ctypes.c_uint32)]
+
+
+class BatchConfig(ctypes.Structure):
+    _fields_ = [("device", ctypes.c_int32), ("passes_per_graph", ctypes.c_uint32)]
 
 
 class Counters(ctypes.Structure):
@@ -140,6 +146,24 @@ def library():
         "ppm_mesh_num_triangles": (ctypes.c_int32, [vp]),
         "ppm_mesh_destroy": (None, [vp]),
         "ppm_mesh_last_error": (ctypes.c_char_p, []),
+        "ppm_batch_config_default": (None, [ctypes.POINTER(BatchConfig)]),
+        "ppm_batch_create": (ctypes.c_int, [ctypes.POINTER(BatchConfig), ctypes.POINTER(vp)]),
+        "ppm_batch_add": (ctypes.c_int, [vp, ctypes.POINTER(Scene), ctypes.POINTER(Params), ctypes.c_uint32,
+                                         ctypes.POINTER(ctypes.c_int32)]),
+        "ppm_batch_remove": (ctypes.c_int, [vp, ctypes.c_int32]),
+        "ppm_batch_set_passes": (ctypes.c_int, [vp, ctypes.c_int32, ctypes.c_uint32]),
+        "ppm_batch_pass": (ctypes.c_int, [vp, ctypes.c_uint32]),
+        "ppm_batch_sync": (ctypes.c_int, [vp]),
+        "ppm_batch_last_pass_ms": (ctypes.c_float, [vp]),
+        "ppm_batch_progress": (ctypes.c_int, [vp, ctypes.c_int32, ctypes.POINTER(ctypes.c_uint64),
+                                              ctypes.POINTER(ctypes.c_uint32)]),
+        "ppm_batch_readback": (ctypes.c_int, [vp, ctypes.c_int32, ctypes.c_int, vp, ctypes.c_size_t]),
+        "ppm_batch_buffer_bytes": (ctypes.c_size_t, [vp, ctypes.c_int32, ctypes.c_int]),
+        "ppm_batch_num_jobs": (ctypes.c_int32, [vp]),
+        "ppm_batch_device_bytes": (ctypes.c_uint64, [vp]),
+        "ppm_scene_footprint": (ctypes.c_uint64, [ctypes.POINTER(Scene), ctypes.POINTER(Params)]),
+        "ppm_batch_destroy": (None, [vp]),
+        "ppm_batch_last_error": (ctypes.c_char_p, [vp]),
     }
     for name, (res, args) in sig.items():
         f = getattr(lib, name)
@@ -331,6 +355,107 @@ class Renderer:
     def close(self):
         if self._h and self._h.value:
             library().ppm_destroy(self._h)
+            self._h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def scene_footprint(mesh, params):
+    """HBM bytes a job of this scene and these parameters allocates (ppm_scene_footprint)."""
+    return int(library().ppm_scene_footprint(mesh.scene, ctypes.byref(params)))
+
+
+class Batch:
+    """Many jobs advanced together, one launch per stage per pass (ppm_batch_*).
+
+    Each job is what a Renderer holds; after n of its passes its buffers equal those of a
+    Renderer of the same mesh and parameters after run(n), bit for bit."""
+
+    def __init__(self, device=0, passes_per_graph=1):
+        cfg = BatchConfig()
+        library().ppm_batch_config_default(ctypes.byref(cfg))
+        cfg.device = int(device)
+        cfg.passes_per_graph = int(passes_per_graph)
+        self._h = ctypes.c_void_p()
+        rc = library().ppm_batch_create(ctypes.byref(cfg), ctypes.byref(self._h))
+        if rc != 0:
+            raise PPMError("ppm_batch_create failed (%d): %s" % (rc, library().ppm_batch_last_error(None).decode()))
+        self._meshes = {}
+        self._sizes = {}
+
+    def _check(self, rc):
+        if rc != 0:
+            raise PPMError("ppm_batch error %d: %s" % (rc, library().ppm_batch_last_error(self._h).decode()))
+
+    def add(self, mesh, params, passes=PASSES_UNLIMITED):
+        """Adds a job with a budget of `passes` passes; returns its id."""
+        j = ctypes.c_int32()
+        self._check(library().ppm_batch_add(self._h, mesh.scene, ctypes.byref(params), int(passes), ctypes.byref(j)))
+        self._meshes[j.value] = mesh
+        self._sizes[j.value] = (params.image_width * params.image_height, params.photon_width * params.photon_height,
+                                params.hash_width * params.hash_height)
+        return j.value
+
+    def remove(self, job):
+        self._check(library().ppm_batch_remove(self._h, int(job)))
+        self._meshes.pop(job, None)
+        self._sizes.pop(job, None)
+
+    def set_passes(self, job, passes):
+        self._check(library().ppm_batch_set_passes(self._h, int(job), int(passes)))
+
+    def run(self, npasses=1):
+        self._check(library().ppm_batch_pass(self._h, int(npasses)))
+        return self
+
+    def sync(self):
+        self._check(library().ppm_batch_sync(self._h))
+
+    def last_pass_ms(self):
+        return float(library().ppm_batch_last_pass_ms(self._h))
+
+    def progress(self, job):
+        """(passes run, passes left) of a job; waits for the queued passes."""
+        f = ctypes.c_uint64()
+        left = ctypes.c_uint32()
+        self._check(library().ppm_batch_progress(self._h, int(job), ctypes.byref(f), ctypes.byref(left)))
+        return int(f.value), int(left.value)
+
+    @property
+    def num_jobs(self):
+        return int(library().ppm_batch_num_jobs(self._h))
+
+    @property
+    def device_bytes(self):
+        return int(library().ppm_batch_device_bytes(self._h))
+
+    def readback(self, job, name):
+        bid, dt, comp, which = BUFFERS[name]
+        q, p, h = self._sizes[job]
+        n = {"q": q, "p": p, "h": h}[which]
+        out = np.empty((n, comp) if comp > 1 else (n,), dtype=dt)
+        self._check(library().ppm_batch_readback(self._h, int(job), bid, out.ctypes.data, out.nbytes))
+        return out
+
+    def counters(self, job):
+        c = Counters()
+        self._check(library().ppm_batch_readback(self._h, int(job), BUF_COUNTERS, ctypes.addressof(c),
+                                                 ctypes.sizeof(c)))
+        return {k: getattr(c, k) for k, _ in Counters._fields_ if k != "reserved"}
+
+    def grid(self, job):
+        g = Grid()
+        self._check(library().ppm_batch_readback(self._h, int(job), BUF_GRID, ctypes.addressof(g), ctypes.sizeof(g)))
+        return {"bbmin": list(g.bbmin), "bbmax": list(g.bbmax), "initial_radius": g.initial_radius,
+                "grid_scale": g.grid_scale, "hash_scale": g.hash_scale}
+
+    def close(self):
+        if self._h and self._h.value:
+            library().ppm_batch_destroy(self._h)
             self._h = ctypes.c_void_p()
 
     def __del__(self):

@@ -40,7 +40,10 @@ struct DevCamera {
     float half_w, half_h, dist;
 };
 
-// Grid scalars and pass counters kept in HBM so that no pass needs a host round trip.
+// Grid scalars, pass counters and the per-pass scalars of one job, kept in HBM so that no pass
+// needs a host round trip and the pass sequence is the same launch sequence every pass (which a
+// hipGraph can replay, and which many jobs can share).  k_pass_begin writes the per-pass fields
+// from the job's frame count and its own XORShift stream (glslppm.cpp:214-287).
 struct DevGlobals {
     float bbmin[3];
     float bbmax[3];
@@ -49,11 +52,18 @@ struct DevGlobals {
     float hash_scale;
     float num_photons_f;
     unsigned long long photons_emitted;
+    unsigned long long frame;        // passes run (FrameCount)
+    unsigned long long eye_samples;  // eye passes run
     unsigned int emitted_this_pass;
-    unsigned int pad1;
-    unsigned int pad2;
-    unsigned int queue_len;      // continuing photons of this pass (written by k_bucket_scan)
-    unsigned int pad0;
+    unsigned int queue_len;          // continuing photons of this pass (written by k_bucket_scan)
+    unsigned int xs[4];              // XORShift state after the seed grids (AAOffset draws)
+    unsigned int passes_left;        // passes this job still runs; 0xffffffff: no limit
+    int active;                      // this pass advances the job
+    int eye_now;                     // this pass traces eye rays (FrameCount % MaxPathLength == 0)
+    int first_pass;                  // this pass is frame 0
+    int num_eye_samples;             // the NumEyeSamples uniform of this eye pass (count + 1)
+    float aa_x, aa_y;                // AAOffset of this eye pass
+    int pad0;
 };
 
 struct HitState {
@@ -106,10 +116,8 @@ struct HashState {
 
 struct EyeParams {
     int width, height;
-    float aa_x, aa_y;
     float focal_length, aperture;
     int max_path;
-    int num_eye_samples;   // the NumEyeSamples uniform (count + 1)
     float fixed_radius;
     float footprint;
     float footprint_width;
@@ -125,7 +133,6 @@ struct PhotonParams {
 struct GatherParams {
     int count;         // Q
     float alpha;
-    int first_pass;
     int draw;
 };
 
@@ -136,13 +143,52 @@ struct Tiles {
     int tiles_x;         // ceil(width / 16)
 };
 
-__global__ void k_clear_slots(HashState h, RayQueue rq, DevGlobals* g);
-__global__ void k_bucket_scan(RayQueue rq, const unsigned int* partial, int n_partial, DevGlobals* g);
-__global__ void k_bucket_scatter(RayQueue rq, int count);
-__global__ void k_eye(DevScene s, DevCamera cam, EyeParams ep, Tiles tl, HitState hs, DevGlobals* g);
-__global__ void k_grid(EyeParams ep, const float* partial, int n_partial, DevGlobals* g);
-__global__ void k_photon_gen(DevScene s, PhotonParams pp, PhotonState ps, RayQueue rq);
-__global__ void k_photon_trace(DevScene s, PhotonParams pp, PhotonState ps, HashState h, RayQueue rq, DevGlobals* g);
-__global__ void k_gather(GatherParams gp, Tiles tl, HitState hs, PhotonState ps, HashState h, DevGlobals* g);
+// Everything the kernels need about one job (one scene, camera and state set).  A plain pass loop
+// passes it by value; a batch keeps an array of them in HBM and every launch covers all jobs.
+struct Job {
+    DevScene s;
+    DevCamera cam;
+    EyeParams ep;
+    PhotonParams pp;
+    GatherParams gp;
+    Tiles tl;
+    HitState hs;
+    PhotonState ps;
+    HashState h;
+    RayQueue rq;
+    DevGlobals* g;
+    int tile_blocks;     // 16x16 pixel tiles (k_eye, k_gather)
+    int photon_blocks;   // 256 photon slots per block (k_photon_gen, k_bucket_scatter, k_photon_trace)
+    int clear_blocks;    // 256 hash slots or ray buckets per block (k_clear_slots)
+    int pad;
+};
+
+// Block map of a batched launch: entry b = (job, block of that job) for grid block b.
+struct BlockRef {
+    int job;
+    int block;
+};
+
+// Kernels of one job: grid = that job's blocks.  The _batch forms take the job array and, for
+// the kernels with several blocks per job, the block map of that kernel kind; k_pass_begin_batch
+// runs one thread per job, k_grid_batch and k_bucket_scan_batch one block per job.
+__global__ void k_pass_begin(Job J);
+__global__ void k_clear_slots(Job J);
+__global__ void k_eye(Job J);
+__global__ void k_grid(Job J);
+__global__ void k_photon_gen(Job J);
+__global__ void k_bucket_scan(Job J);
+__global__ void k_bucket_scatter(Job J);
+__global__ void k_photon_trace(Job J);
+__global__ void k_gather(Job J);
+__global__ void k_pass_begin_batch(const Job* jobs, int n_jobs);
+__global__ void k_clear_slots_batch(const Job* jobs, const BlockRef* map);
+__global__ void k_eye_batch(const Job* jobs, const BlockRef* map);
+__global__ void k_grid_batch(const Job* jobs);
+__global__ void k_photon_gen_batch(const Job* jobs, const BlockRef* map);
+__global__ void k_bucket_scan_batch(const Job* jobs);
+__global__ void k_bucket_scatter_batch(const Job* jobs, const BlockRef* map);
+__global__ void k_photon_trace_batch(const Job* jobs, const BlockRef* map);
+__global__ void k_gather_batch(const Job* jobs, const BlockRef* map);
 __global__ void k_resolve(int count, HitState hs, const DevGlobals* g, float4* linear, float4* mapped);
-__global__ void k_init_state(int q, int p, HitState hs, PhotonState ps);
+__global__ void k_init_state(Job J);

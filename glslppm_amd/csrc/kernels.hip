@@ -1,4 +1,5 @@
 // Kernels of the SPPM pass loop for gfx950.  One pass (m_display, glslppm.cpp:214-536) is:
+//   k_pass_begin    frame count, eye cadence and AAOffset of the pass (one thread per job)
 //   k_eye      every max_path passes: eyeraytrace.fs + per-block max.fs/min.fs partials
 //   k_grid     the hit-point box and the grid uniforms of glslppm.cpp:292-311 (one block)
 //   k_clear_slots
@@ -10,6 +11,12 @@
 //                   index) + correction (atomic count), one thread per ray in bucket order
 //   k_gather   progressive.fs (8-cell gather and the N/R/tau update) + re.fs draw to RGBA8
 // Arithmetic follows the GLSL statement by statement in strict fp32 (see fmath.h).
+//
+// Each kernel is written once as a body over (job, block of that job).  The plain form takes the
+// job by value and runs that job's blocks; the _batch form takes the job array in HBM and a block
+// map, so that one launch advances every job of a batch.  Every body starts by reading its job's
+// pass flags and returns when the job does not run this stage, so the launch sequence of a pass
+// is fixed and a hipGraph of it stays valid from pass to pass.
 
 #include "kernels.h"
 
@@ -186,8 +193,7 @@ constexpr int kBlock = 256;
 constexpr int kWaves = kBlock / 64;
 
 // Pixel of this thread: block = 16x16 tile, wave = 8x8 quarter of it.
-PPM_DEV void tile_pixel(const Tiles& tl, int& i, int& j) {
-    const int tile = blockIdx.x;
+PPM_DEV void tile_pixel(const Tiles& tl, int tile, int& i, int& j) {
     const int tx = tile % tl.tiles_x, ty = tile / tl.tiles_x;
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     i = tx * 16 + (wave & 1) * 8 + (lane & 7);
@@ -198,36 +204,91 @@ PPM_DEV void tile_pixel(const Tiles& tl, int& i, int& j) {
 
 // ------------------------------------------------------------------------------------------
 
-__global__ void k_init_state(int q, int p, HitState hs, PhotonState ps) {
+// Frame-0 state of a job: E, N, tau, R zero, every photon new, the hash and buckets clear.
+__global__ void k_init_state(Job J) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < q) {
-        hs.e_n[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-        hs.tau_r[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    if (i < J.gp.count) {
+        J.hs.e_n[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        J.hs.tau_r[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
     }
-    if (i < p) {
-        ps.pos[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-        ps.flux[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-        ps.dir[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-        ps.state[i] = make_int2(-1, 0);  // (-1, -1, 0, 1e30): every photon starts new
-        ps.slot[i] = -1;
+    if (i < J.pp.count) {
+        J.ps.pos[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        J.ps.flux[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        J.ps.dir[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        J.ps.state[i] = make_int2(-1, 0);  // (-1, -1, 0, 1e30): every photon starts new
+        J.ps.slot[i] = -1;
+    }
+    if (i < J.h.size) {
+        J.h.count[i] = 0u;
+        J.h.rep[i] = 0xffffffffu;
+    }
+    if (i < kRayBuckets) J.rq.bucket[i] = 0u;
+}
+
+namespace {
+
+// XORShift::m_frand (glslppm.cpp:77-91)
+PPM_DEV float xs_frand(unsigned int* s) {
+    const unsigned int t = s[0] ^ (s[0] << 11);
+    s[0] = s[1];
+    s[1] = s[2];
+    s[2] = s[3];
+    s[3] = (s[3] ^ (s[3] >> 19)) ^ (t ^ (t >> 8));
+    return (float)s[3] * (1.0f / 4294967295.0f);
+}
+
+// Per-pass uniforms of m_display (glslppm.cpp:214-287): eye pass when FrameCount % MaxPathLength
+// is 0, with two AAOffset draws from the job's XORShift stream and NumEyeSamples = count + 1;
+// FrameCount 0 is the first pass.  A job whose pass budget is spent stops advancing.
+PPM_DEV void pass_begin(const Job& J) {
+    DevGlobals* g = J.g;
+    const unsigned int left = g->passes_left;
+    if (left == 0u) {
+        g->active = 0;
+        return;
+    }
+    if (left != 0xffffffffu) g->passes_left = left - 1u;
+    const unsigned long long frame = g->frame;
+    const int eye = (frame % (unsigned long long)J.ep.max_path) == 0ull;
+    g->active = 1;
+    g->eye_now = eye;
+    g->first_pass = frame == 0ull;
+    if (eye) {
+        unsigned int xs[4] = {g->xs[0], g->xs[1], g->xs[2], g->xs[3]};
+        g->aa_x = (xs_frand(xs) - 0.5f) * 1.25f;
+        g->aa_y = (xs_frand(xs) - 0.5f) * 1.25f;
+        for (int k = 0; k < 4; k++) g->xs[k] = xs[k];
+        g->num_eye_samples = (int)(g->eye_samples + 1ull);
+        g->eye_samples = g->eye_samples + 1ull;
+    }
+    g->frame = frame + 1ull;
+}
+
+PPM_DEV void clear_slots(const Job& J, int blk) {
+    if (!J.g->active) return;
+    const int i = blk * kBlock + threadIdx.x;
+    if (i < kRayBuckets) J.rq.bucket[i] = 0u;
+    if (i < J.h.size) {
+        J.h.count[i] = 0u;
+        J.h.rep[i] = 0xffffffffu;
     }
 }
 
-__global__ void k_clear_slots(HashState h, RayQueue rq, DevGlobals* g) {
-    (void)g;
-    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < kRayBuckets; i += gridDim.x * blockDim.x)
-        rq.bucket[i] = 0u;
-    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < h.size; i += gridDim.x * blockDim.x) {
-        h.count[i] = 0u;
-        h.rep[i] = 0xffffffffu;
-    }
-}
+}  // namespace
+
+namespace {
 
 // eyeraytrace.fs:396-554 for one pixel, then the min/max of hit positions.
-__global__ __launch_bounds__(kBlock) void k_eye(DevScene s, DevCamera cam, EyeParams ep, Tiles tl, HitState hs,
-                                                DevGlobals* g) {
+PPM_DEV void eye(const Job& J, int blk) {
+    const DevGlobals* g = J.g;
+    if (!g->active || !g->eye_now) return;
+    const DevScene& s = J.s;
+    const DevCamera& cam = J.cam;
+    const EyeParams& ep = J.ep;
+    const HitState& hs = J.hs;
+    const float aa_x = g->aa_x, aa_y = g->aa_y;
     int px_i, px_j;
-    tile_pixel(tl, px_i, px_j);
+    tile_pixel(J.tl, blk, px_i, px_j);
     const int p = px_j * ep.width + px_i;
     f3 pos = mk3(0.0f, 0.0f, 0.0f);
     const bool active = px_i < ep.width && px_j < ep.height;
@@ -235,11 +296,11 @@ __global__ __launch_bounds__(kBlock) void k_eye(DevScene s, DevCamera cam, EyePa
         float4 rndv = hs.rng[p];
         const f3 C = mk3(cam.ox, cam.oy, cam.oz);
         const f3 U = mk3(cam.ux, cam.uy, cam.uz), V = mk3(cam.vx, cam.vy, cam.vz), W = mk3(cam.wx, cam.wy, cam.wz);
-        const float ppx = ((float)px_i + 0.5f + ep.aa_x) - cam.half_w;
-        const float ppy = ((float)px_j + 0.5f + ep.aa_y) - cam.half_h;
+        const float ppx = ((float)px_i + 0.5f + aa_x) - cam.half_w;
+        const float ppy = ((float)px_j + 0.5f + aa_y) - cam.half_h;
         const f3 rel = (ppx * U + ppy * V) + cam.dist * W;
         f3 dir = normalize3(rel);
-        const float filter = (1.0f - fabsf(ep.aa_x) / 1.5f) * (1.0f - fabsf(ep.aa_y) / 1.5f);
+        const float filter = (1.0f - fabsf(aa_x) / 1.5f) * (1.0f - fabsf(aa_y) / 1.5f);
         const f3 fp = C + dir * ep.focal_length;
         const float radius = sqrtf(gpu_rnd(rndv)) * ep.aperture;
         const float theta = 6.283184f * gpu_rnd(rndv);
@@ -322,7 +383,7 @@ __global__ __launch_bounds__(kBlock) void k_eye(DevScene s, DevCamera cam, EyePa
             }
         }
         float4 en = hs.e_n[p];
-        const float a = 1.0f / (float)ep.num_eye_samples;
+        const float a = 1.0f / (float)g->num_eye_samples;
         en.x = en.x * (1.0f - a) + emi.x * a;
         en.y = en.y * (1.0f - a) + emi.y * a;
         en.z = en.z * (1.0f - a) + emi.z * a;
@@ -353,13 +414,18 @@ __global__ __launch_bounds__(kBlock) void k_eye(DevScene s, DevCamera cam, EyePa
             for (int k = 0; k < 3; k++) r[k] = min1(r[k], red[w][k]);
             for (int k = 3; k < 6; k++) r[k] = max1(r[k], red[w][k]);
         }
-        for (int k = 0; k < 6; k++) hs.partial[blockIdx.x * 6 + k] = r[k];
+        for (int k = 0; k < 6; k++) hs.partial[blk * 6 + k] = r[k];
     }
 }
 
 // The hit-point bounding box from the eye blocks' partials, then the grid scalars of
 // glslppm.cpp:294-311 (one block of 1024 threads).
-__global__ __launch_bounds__(1024) void k_grid(EyeParams ep, const float* partial, int n_partial, DevGlobals* g) {
+PPM_DEV void grid(const Job& J) {
+    DevGlobals* g = J.g;
+    if (!g->active || !g->eye_now) return;
+    const EyeParams& ep = J.ep;
+    const float* partial = J.hs.partial;
+    const int n_partial = J.tile_blocks;
     __shared__ float red[16][6];
     float r[6] = {kInf, kInf, kInf, -kInf, -kInf, -kInf};
     for (int b = threadIdx.x; b < n_partial; b += 1024) {
@@ -397,6 +463,8 @@ __global__ __launch_bounds__(1024) void k_grid(EyeParams ep, const float* partia
         g->hash_scale = 4194304.0f / gs;
     }
 }
+
+}  // namespace
 
 namespace {
 
@@ -465,15 +533,14 @@ PPM_DEV void ibl_sample(const DevScene& s, float4& rndv, f3& flux, f3& dir, f3& 
 
 }  // namespace
 
-// photontrace.fs:464-720, hash.fs and the scatter/correction draws, as two kernels.
+// photontrace.fs:464-720, hash.fs and the scatter/correction draws, as four kernels.
 //
 // k_photon_gen runs one thread per photon slot: emission or bounce with Russian roulette
-// (photontrace.fs:466-686).  A photon that continues appends its ray to a queue (wave-aggregated
-// atomic); a terminated one writes its final state at once.  k_photon_trace is persistent: its
-// lanes take queue entries from a cursor, walk the BVH a few nodes at a time, and when a walk
-// ends deposit the photon (photontrace.fs:690-710, hash.fs, scatter, correction) and take the
-// next entry as soon as a quarter of the wave is idle -- a wave is no longer as slow as its
-// longest photon, and the walking kernel does not carry the sampling code's registers.  Each
+// (photontrace.fs:466-686).  A terminated photon writes its final state at once; a continuing
+// one writes its ray and is counted into a coherence bucket (cube face of the direction x cell of
+// the origin).  k_bucket_scan turns the counts into bucket offsets, k_bucket_scatter places the
+// rays in bucket order, and k_photon_trace runs one thread per continuing ray in that order: the
+// walk, then the deposit (photontrace.fs:690-710), hash.fs, scatter and correction.  Each
 // photon's arithmetic is independent of which lane runs it and the slot table only sees
 // commutative atomics, so the result equals one thread per slot.
 
@@ -645,8 +712,6 @@ PPM_DEV void photon_finish(const DevScene& s, const PhotonParams& pp, const Phot
     ps.slot[p] = slot;
 }
 
-constexpr int kPhotonStepChunk = 4;
-
 // Cube face and origin cell of a ray, interleaved as face | morton(cell).
 PPM_DEV int ray_bucket(const DevScene& s, f3 org, f3 dir) {
     const int n = 1 << kRayCellBits;
@@ -663,12 +728,13 @@ PPM_DEV int ray_bucket(const DevScene& s, f3 org, f3 dir) {
     return (cube_face(dir) << (3 * kRayCellBits)) | code;
 }
 
-PPM_DEV unsigned long long lanes_below(int lane) { return lane == 0 ? 0ull : (~0ull >> (64 - lane)); }
-
-}  // namespace
-
-__global__ __launch_bounds__(kBlock) void k_photon_gen(DevScene s, PhotonParams pp, PhotonState ps, RayQueue rq) {
-    const int p = blockIdx.x * kBlock + threadIdx.x;
+PPM_DEV void photon_gen_blk(const Job& J, int blk) {
+    if (!J.g->active) return;
+    const DevScene& s = J.s;
+    const PhotonParams& pp = J.pp;
+    const PhotonState& ps = J.ps;
+    const RayQueue& rq = J.rq;
+    const int p = blk * kBlock + threadIdx.x;
     unsigned int emitted = 0;
     bool live = false;
     f3 org, dir, flux;
@@ -704,14 +770,18 @@ __global__ __launch_bounds__(kBlock) void k_photon_gen(DevScene s, PhotonParams 
     if (threadIdx.x == 0) {
         unsigned int tot = 0;
         for (int w = 0; w < kWaves; w++) tot += red[w];
-        ps.partial[blockIdx.x] = tot;
+        ps.partial[blk] = tot;
     }
 }
 
 // Exclusive scan of the bucket counts into first positions, the number of continuing rays, and
 // NumPhotons += int(sum of emitted flags) (glslppm.cpp:396-397); one block of 1024 threads.
-__global__ __launch_bounds__(1024) void k_bucket_scan(RayQueue rq, const unsigned int* partial, int n_partial,
-                                                      DevGlobals* g) {
+PPM_DEV void bucket_scan(const Job& J) {
+    DevGlobals* g = J.g;
+    if (!g->active) return;
+    const RayQueue& rq = J.rq;
+    const unsigned int* partial = J.ps.partial;
+    const int n_partial = J.photon_blocks;
     constexpr int per = (kRayBuckets + 1023) / 1024;
     __shared__ unsigned int wsum[16];
     __shared__ unsigned int esum[16];
@@ -756,18 +826,23 @@ __global__ __launch_bounds__(1024) void k_bucket_scan(RayQueue rq, const unsigne
 }
 
 // Places every continuing ray at its bucket position.
-__global__ void k_bucket_scatter(RayQueue rq, int count) {
-    const int p = blockIdx.x * blockDim.x + threadIdx.x;
-    if (p >= count) return;
+PPM_DEV void bucket_scatter(const Job& J, int blk) {
+    if (!J.g->active) return;
+    const RayQueue& rq = J.rq;
+    const int p = blk * kBlock + threadIdx.x;
+    if (p >= J.pp.count) return;
     const int2 kr = rq.key_rank[p];
     if (kr.x >= 0) rq.perm[rq.bucket[kr.x] + (unsigned int)kr.y] = p;
 }
 
-__global__ __launch_bounds__(kBlock) void k_photon_trace(DevScene s, PhotonParams pp, PhotonState ps, HashState h,
-                                                         RayQueue rq, DevGlobals* g) {
+PPM_DEV void photon_trace(const Job& J, int blk) {
+    const DevGlobals* g = J.g;
+    if (!g->active) return;
     const unsigned int n_rays = g->queue_len;
-    const unsigned int t = blockIdx.x * kBlock + threadIdx.x;
+    const unsigned int t = blk * kBlock + threadIdx.x;
     if (t >= n_rays) return;
+    const DevScene& s = J.s;
+    const RayQueue& rq = J.rq;
     const int p = rq.perm[t];
     const float4 o = rq.org[p], d = rq.dir[p];
     const int4 x = rq.aux[p];
@@ -781,10 +856,8 @@ __global__ __launch_bounds__(kBlock) void k_photon_trace(DevScene s, PhotonParam
     W.h.t = 1.0e30f;
     W.h.bx = W.h.by = W.h.bz = 0.0f;
     walk<true>(s, W.face, W.org, W.dir, W.rcp, W.t0, false, W.h);
-    photon_finish(s, pp, ps, h, g, x.z, W, mk3(o.w, d.w, __int_as_float(x.x)), x.y);
+    photon_finish(s, J.pp, J.ps, J.h, g, x.z, W, mk3(o.w, d.w, __int_as_float(x.x)), x.y);
 }
-
-namespace {
 
 // re.fs: tone map of one channel
 PPM_DEV float srgb(float c) {
@@ -809,16 +882,20 @@ PPM_DEV unsigned char unorm8(float c) {
     return (unsigned char)(int)(c * 255.0f + 0.5f);
 }
 
-}  // namespace
-
 // progressive.fs:98-142 for one hit point, then the re.fs draw of that pixel.
-__global__ __launch_bounds__(kBlock) void k_gather(GatherParams gp, Tiles tl, HitState hs, PhotonState ps,
-                                                   HashState h, DevGlobals* g) {
+PPM_DEV void gather(const Job& J, int blk) {
+    const DevGlobals* g = J.g;
+    if (!g->active) return;
+    const GatherParams& gp = J.gp;
+    const Tiles& tl = J.tl;
+    const HitState& hs = J.hs;
+    const PhotonState& ps = J.ps;
+    const HashState& h = J.h;
     int px_i, px_j;
-    tile_pixel(tl, px_i, px_j);
+    tile_pixel(tl, blk, px_i, px_j);
     if (px_i >= tl.width || px_j >= tl.height) return;
     const int p = px_j * tl.width + px_i;
-    float4 tr = gp.first_pass ? make_float4(0.0f, 0.0f, 0.0f, g->initial_radius) : hs.tau_r[p];
+    float4 tr = g->first_pass ? make_float4(0.0f, 0.0f, 0.0f, g->initial_radius) : hs.tau_r[p];
     float4 en = hs.e_n[p];
     if (hs.tri[p] >= 0) {
         const f3 qpos = xyz(hs.pos[p]);
@@ -874,6 +951,42 @@ __global__ __launch_bounds__(kBlock) void k_gather(GatherParams gp, Tiles tl, Hi
         radiance(tr, en, g->num_photons_f, lin, mapped);
         hs.rgba8[p] = make_uchar4(unorm8(mapped.x), unorm8(mapped.y), unorm8(mapped.z), unorm8(mapped.w));
     }
+}
+
+}  // namespace
+
+// ------------------------------------------------------------------------------------------
+// Plain (one job) and batched entry points of every body.
+
+#define PPM_JOB_KERNEL(name, body, threads)                                                       \
+    __global__ __launch_bounds__(threads) void name(Job J) { body(J, (int)blockIdx.x); }          \
+    __global__ __launch_bounds__(threads) void name##_batch(const Job* __restrict__ jobs,          \
+                                                            const BlockRef* __restrict__ map) {    \
+        const BlockRef m = map[blockIdx.x];                                                         \
+        body(jobs[m.job], m.block);                                                                 \
+    }
+#define PPM_JOB_KERNEL1(name, body, threads)                                                      \
+    __global__ __launch_bounds__(threads) void name(Job J) { body(J); }                            \
+    __global__ __launch_bounds__(threads) void name##_batch(const Job* __restrict__ jobs) {        \
+        body(jobs[blockIdx.x]);                                                                     \
+    }
+
+PPM_JOB_KERNEL(k_clear_slots, clear_slots, kBlock)
+PPM_JOB_KERNEL(k_eye, eye, kBlock)
+PPM_JOB_KERNEL1(k_grid, grid, 1024)
+PPM_JOB_KERNEL(k_photon_gen, photon_gen_blk, kBlock)
+PPM_JOB_KERNEL1(k_bucket_scan, bucket_scan, 1024)
+PPM_JOB_KERNEL(k_bucket_scatter, bucket_scatter, kBlock)
+PPM_JOB_KERNEL(k_photon_trace, photon_trace, kBlock)
+PPM_JOB_KERNEL(k_gather, gather, kBlock)
+
+__global__ void k_pass_begin(Job J) {
+    if (threadIdx.x == 0) pass_begin(J);
+}
+
+__global__ void k_pass_begin_batch(const Job* __restrict__ jobs, int n_jobs) {
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j < n_jobs) pass_begin(jobs[j]);
 }
 
 __global__ void k_resolve(int count, HitState hs, const DevGlobals* g, float4* linear, float4* mapped) {

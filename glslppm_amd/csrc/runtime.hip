@@ -3,11 +3,12 @@
 // ppm_create replaces the texture/FBO set-up of main() (glslppm.cpp:664-829) and the scene
 // packing of CGPURT::CreateTextures (gpurt.cpp:47-552); ppm_pass replaces m_display()
 // (glslppm.cpp:214-536); ppm_readback replaces glReadPixels and reading the state textures.
-// Everything a pass needs (grid scalars, photon counters) stays in HBM, so ppm_pass only queues
-// kernels on the context's stream and never waits for the device.
+// Everything a pass needs (grid scalars, photon counters, frame count, eye cadence, AAOffset) stays
+// in HBM, so ppm_pass only queues kernels on the context's stream and never waits for the device.
 
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -15,38 +16,7 @@
 #include <string>
 #include <vector>
 
-#include "../../include/ppm.h"
-#include "kernels.h"
-
-struct ppm_ctx {
-    int device = 0;
-    hipStream_t stream = nullptr;
-    hipEvent_t ev0 = nullptr, ev1 = nullptr;
-    bool timed = false;
-    ppm_params prm;
-    int Q = 0, P = 0, H = 0, N = 0, T = 0, M = 0, NL = 0;
-    int eye_blocks = 0, photon_blocks = 0;
-    Tiles tiles;
-    DevScene scene;
-    DevCamera cam;
-    HitState hit;
-    PhotonState ph;
-    HashState hash;
-    RayQueue rays;
-    DevGlobals* glob = nullptr;
-    float4* resolve_lin = nullptr;
-    float4* resolve_map = nullptr;
-    std::vector<void*> allocs;
-    // per-kernel event timing (ppm_profile_enable)
-    bool profiling = false;
-    std::vector<hipEvent_t> ev_pool;
-    std::vector<std::pair<int, std::pair<hipEvent_t, hipEvent_t>>> ev_pending;
-    double prof_ms[PPM_KERNEL_KINDS] = {0, 0, 0, 0};
-    uint64_t prof_n[PPM_KERNEL_KINDS] = {0, 0, 0, 0};
-    uint64_t frame = 0, eye_samples = 0;
-    uint32_t xs[4];
-    std::string err;
-};
+#include "context.h"
 
 namespace {
 
@@ -71,20 +41,22 @@ int dev_alloc(ppm_ctx* c, T** out, size_t count) {
     void* p = nullptr;
     size_t bytes = count * sizeof(T);
     if (bytes == 0) bytes = 16;
-    hipError_t e = hipMalloc(&p, bytes);
+    hipError_t e = c->arena ? c->arena->alloc(&p, bytes) : hipMalloc(&p, bytes);
     if (e != hipSuccess)
         return set_err(c, e == hipErrorOutOfMemory ? PPM_E_OOM : PPM_E_HIP,
-                       std::string("hipMalloc(") + std::to_string(bytes) + "): " + hipGetErrorString(e));
+                       std::string("device allocation of ") + std::to_string(bytes) + " bytes: " + hipGetErrorString(e));
     c->allocs.push_back(p);
+    c->device_bytes += c->arena ? ppm_internal::Arena::round_size(bytes) : bytes;
     *out = (T*)p;
     return PPM_OK;
 }
 
+// Allocates and queues the copy on `st`; the caller keeps `v` alive until `st` is synchronised.
 template <typename T>
-int upload(ppm_ctx* c, T** out, const std::vector<T>& v) {
+int upload(ppm_ctx* c, T** out, const std::vector<T>& v, hipStream_t st) {
     int rc = dev_alloc(c, out, v.size());
     if (rc) return rc;
-    if (!v.empty()) HIP_TRY(c, hipMemcpy(*out, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice));
+    if (!v.empty()) HIP_TRY(c, hipMemcpyAsync(*out, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice, st));
     return PPM_OK;
 }
 
@@ -255,25 +227,196 @@ void prof_end(ppm_ctx* c, int kind, hipEvent_t start) {
     }
 }
 
-void free_all(ppm_ctx* c) {
-    for (auto& pe : c->ev_pending) {
-        (void)hipEventDestroy(pe.second.first);
-        (void)hipEventDestroy(pe.second.second);
-    }
-    c->ev_pending.clear();
-    for (hipEvent_t e : c->ev_pool) (void)hipEventDestroy(e);
-    c->ev_pool.clear();
-    for (void* p : c->allocs) (void)hipFree(p);
-    c->allocs.clear();
-    if (c->ev0) (void)hipEventDestroy(c->ev0);
-    if (c->ev1) (void)hipEventDestroy(c->ev1);
-    if (c->stream) (void)hipStreamDestroy(c->stream);
-    c->ev0 = c->ev1 = nullptr;
-    c->stream = nullptr;
+int check_params(const ppm_params* p) {
+    if (!p) return set_err(nullptr, PPM_E_INVAL, "null params");
+    if (p->image_width <= 0 || p->image_height <= 0 || p->photon_width <= 0 || p->photon_height <= 0 ||
+        p->hash_width <= 0 || p->hash_height <= 0)
+        return set_err(nullptr, PPM_E_INVAL, "sizes must be positive");
+    const long long Q = (long long)p->image_width * p->image_height;
+    const long long P = (long long)p->photon_width * p->photon_height;
+    const long long H = (long long)p->hash_width * p->hash_height;
+    if (Q > (1ll << 26) || P > (1ll << 26)) return set_err(nullptr, PPM_E_INVAL, "Q and P are limited to 2^26");
+    if (H > (1ll << 24)) return set_err(nullptr, PPM_E_INVAL, "H is limited to 2^24 (slot numbers are exact floats)");
+    if (!is_pow2(p->hash_width)) return set_err(nullptr, PPM_E_INVAL, "hash_width must be a power of two");
+    if (p->max_path_length <= 0 || p->max_path_length > 0xffff)
+        return set_err(nullptr, PPM_E_INVAL, "max_path_length out of range");
+    if (!(p->fixed_radius >= 0.0f) || !(p->initial_footprint > 0.0f) || !(p->alpha > 0.0f && p->alpha <= 1.0f))
+        return set_err(nullptr, PPM_E_INVAL, "bad radius or alpha");
+    if (p->flags & ~(PPM_FLAG_NO_DRAW | PPM_FLAG_GRAPH)) return set_err(nullptr, PPM_E_INVAL, "unknown flag");
+    return PPM_OK;
 }
 
-int create_impl(ppm_ctx* c, const ppm_scene* s, const ppm_params* p) {
+// scatter.vs: depth z = s/Pw + t; points with z >= 1 fail the depth test, which removes the
+// columns i >= Pw^2 / (2 Ph) - 1/2 of the last photon row from the representatives
+int rep_limit(const ppm_params& p) {
+    const long long P = (long long)p.photon_width * p.photon_height;
+    const double first = std::ceil((double)p.photon_width * p.photon_width / (2.0 * p.photon_height) - 0.5);
+    const long long ex = (long long)p.photon_width - (long long)std::max(0.0, first);
+    return (int)(P - std::max(0ll, ex));
+}
+
+// Launches one pass of job J on stream st: m_display's stage order (glslppm.cpp:214-536).  The
+// eye stage is queued only when `eye` is set; its kernels also check the device-side cadence.
+void queue_pass(ppm_ctx* c, hipStream_t st, bool eye) {
+    const Job& J = c->job;
+    hipLaunchKernelGGL(k_pass_begin, dim3(1), dim3(64), 0, st, J);
+    if (eye) {
+        hipEvent_t e0 = prof_begin(c);
+        hipLaunchKernelGGL(k_eye, dim3(J.tile_blocks), dim3(256), 0, st, J);
+        hipLaunchKernelGGL(k_grid, dim3(1), dim3(1024), 0, st, J);
+        prof_end(c, PPM_KERNEL_EYE, e0);
+    }
+    hipEvent_t e1 = prof_begin(c);
+    hipLaunchKernelGGL(k_clear_slots, dim3(J.clear_blocks), dim3(256), 0, st, J);
+    prof_end(c, PPM_KERNEL_CLEAR, e1);
+    hipEvent_t e2 = prof_begin(c);
+    hipLaunchKernelGGL(k_photon_gen, dim3(J.photon_blocks), dim3(256), 0, st, J);
+    hipLaunchKernelGGL(k_bucket_scan, dim3(1), dim3(1024), 0, st, J);
+    hipLaunchKernelGGL(k_bucket_scatter, dim3(J.photon_blocks), dim3(256), 0, st, J);
+    hipLaunchKernelGGL(k_photon_trace, dim3(J.photon_blocks), dim3(256), 0, st, J);
+    prof_end(c, PPM_KERNEL_PHOTON, e2);
+    hipEvent_t e3 = prof_begin(c);
+    hipLaunchKernelGGL(k_gather, dim3(J.tile_blocks), dim3(256), 0, st, J);
+    prof_end(c, PPM_KERNEL_GATHER, e3);
+}
+
+// One captured pass (photon-only or eye pass) for PPM_FLAG_GRAPH.
+int capture_pass(ppm_ctx* c, int eye) {
+    if (c->graph_pass[eye]) return PPM_OK;
+    hipGraph_t g = nullptr;
+    HIP_TRY(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+    const bool prof = c->profiling;
+    c->profiling = false;
+    queue_pass(c, c->stream, eye != 0);
+    c->profiling = prof;
+    HIP_TRY(c, hipStreamEndCapture(c->stream, &g));
+    const hipError_t e = hipGraphInstantiate(&c->graph_pass[eye], g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    HIP_TRY(c, e);
+    return PPM_OK;
+}
+
+}  // namespace
+
+namespace ppm_internal {
+
+size_t Arena::round_size(size_t bytes) {
+    if (bytes <= 65536) return 65536;
+    size_t top = 65536;
+    while (top < bytes) top <<= 1;
+    const size_t q = top >> 3;  // eighths of the power of two above
+    return (bytes + q - 1) / q * q;
+}
+
+Arena::~Arena() {
+    (void)hipSetDevice(device_);
+    for (auto& kv : free_)
+        for (auto& b : kv.second) {
+            if (b.ev) {
+                (void)hipEventSynchronize(b.ev);
+                (void)hipEventDestroy(b.ev);
+            }
+            (void)hipFree(b.p);
+        }
+}
+
+hipError_t Arena::alloc(void** out, size_t bytes) {
+    const size_t sz = round_size(bytes);
+    auto it = free_.find(sz);
+    if (it != free_.end()) {
+        auto& v = it->second;
+        for (size_t k = 0; k < v.size(); k++) {
+            if (v[k].ev && hipEventQuery(v[k].ev) != hipSuccess) continue;
+            if (v[k].ev) (void)hipEventDestroy(v[k].ev);
+            *out = v[k].p;
+            v.erase(v.begin() + (long)k);
+            in_use_ += sz;
+            return hipSuccess;
+        }
+    }
+    void* p = nullptr;
+    hipError_t e = hipMalloc(&p, sz);
+    if (e != hipSuccess) {
+        trim(0);
+        e = hipMalloc(&p, sz);
+        if (e != hipSuccess) return e;
+    }
+    size_of_[p] = sz;
+    held_ += sz;
+    in_use_ += sz;
+    *out = p;
+    return hipSuccess;
+}
+
+void Arena::release(void* p, hipStream_t last_use) {
+    auto it = size_of_.find(p);
+    if (it == size_of_.end()) return;
+    hipEvent_t ev = nullptr;
+    if (last_use && hipEventCreateWithFlags(&ev, hipEventDisableTiming) == hipSuccess) {
+        if (hipEventRecord(ev, last_use) != hipSuccess) {
+            (void)hipEventDestroy(ev);
+            ev = nullptr;
+        }
+    }
+    free_[it->second].push_back(Cached{p, ev});
+    in_use_ -= it->second;
+}
+
+void Arena::trim(size_t target) {
+    for (auto& kv : free_) {
+        auto& v = kv.second;
+        for (size_t k = 0; k < v.size() && held_ > target;) {
+            if (v[k].ev && hipEventQuery(v[k].ev) != hipSuccess) {
+                k++;
+                continue;
+            }
+            if (v[k].ev) (void)hipEventDestroy(v[k].ev);
+            (void)hipFree(v[k].p);
+            held_ -= kv.first;
+            size_of_.erase(v[k].p);
+            v.erase(v.begin() + (long)k);
+        }
+    }
+}
+
+int set_error(ppm_ctx* c, int code, const std::string& msg) { return set_err(c, code, msg); }
+const std::string& create_error() { return g_create_error; }
+
+int validate(const ppm_scene* s, const ppm_params* p) {
+    int rc;
+    if ((rc = check_params(p)) || (rc = check_scene(s))) return rc;
+    return PPM_OK;
+}
+
+int tile_blocks(const ppm_params* p) { return ((p->image_width + 15) / 16) * ((p->image_height + 15) / 16); }
+int photon_blocks(const ppm_params* p) { return (p->photon_width * p->photon_height + 255) / 256; }
+int clear_blocks(const ppm_params* p) {
+    return (std::max(p->hash_width * p->hash_height, kRayBuckets) + 255) / 256;
+}
+
+size_t footprint(const ppm_scene* s, const ppm_params* p) {
+    const size_t Q = (size_t)p->image_width * p->image_height, P = (size_t)p->photon_width * p->photon_height;
+    const size_t H = (size_t)p->hash_width * p->hash_height, N = s->num_nodes, T = s->num_triangles;
+    const size_t sizes[] = {
+        6 * (N + 1) * 32, T * 48, T * 96, (size_t)s->num_materials * 32, (size_t)s->num_lights * 4,
+        (size_t)s->num_lights * 4,
+        // hit points: pos, refl, nrm, rng, (tau, R), (E, N), tri, (mat, t), rgba8, eye partials
+        Q * 16, Q * 16, Q * 16, Q * 16, Q * 16, Q * 16, Q * 4, Q * 8, Q * 4, (size_t)tile_blocks(p) * 24,
+        // photons: pos, flux, dir, rng, state, slot, emitted partials
+        P * 16, P * 16, P * 16, P * 16, P * 8, P * 4, (size_t)photon_blocks(p) * 4,
+        // rays: org, dir, aux, key_rank, perm, buckets
+        P * 16, P * 16, P * 16, P * 8, P * 4, (size_t)kRayBuckets * 4,
+        // hash count, representative; globals; resolve targets
+        H * 4, H * 4, sizeof(DevGlobals), Q * 16, Q * 16};
+    size_t tot = 0;
+    for (size_t b : sizes) tot += Arena::round_size(b ? b : 16);
+    return tot;
+}
+
+int create_context(ppm_ctx* c, const ppm_scene* s, const ppm_params* p, hipStream_t stream, hipStream_t upload_st,
+                   Arena* arena, uint32_t passes_left) {
     c->prm = *p;
+    c->arena = arena;
     c->Q = p->image_width * p->image_height;
     c->P = p->photon_width * p->photon_height;
     c->H = p->hash_width * p->hash_height;
@@ -281,14 +424,22 @@ int create_impl(ppm_ctx* c, const ppm_scene* s, const ppm_params* p) {
     c->N = s->num_nodes;
     c->M = s->num_materials;
     c->NL = s->num_lights;
-    std::memcpy(c->xs, p->xorshift, sizeof(c->xs));
+    uint32_t xs[4];
+    std::memcpy(xs, p->xorshift, sizeof(xs));
 
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) return set_err(c, PPM_E_NODEV, "no HIP device");
     if (p->device < 0 || p->device >= ndev) return set_err(c, PPM_E_INVAL, "device ordinal out of range");
     c->device = p->device;
     HIP_TRY(c, hipSetDevice(c->device));
-    HIP_TRY(c, hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+    if (stream) {
+        c->stream = stream;
+        c->own_stream = false;
+    } else {
+        HIP_TRY(c, hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+        c->own_stream = true;
+    }
+    hipStream_t up = upload_st ? upload_st : c->stream;
     HIP_TRY(c, hipEventCreate(&c->ev0));
     HIP_TRY(c, hipEventCreate(&c->ev1));
 
@@ -303,100 +454,126 @@ int create_impl(ppm_ctx* c, const ppm_scene* s, const ppm_params* p) {
     }
     std::vector<int> ltri(s->light_triangle, s->light_triangle + c->NL);
     std::vector<float> lcdf(s->light_cdf, s->light_cdf + c->NL);
+    Job& J = c->job;
+    std::memset(&J, 0, sizeof(J));
     float4 *d_nodes, *d_isect, *d_shade, *d_mats;
     int* d_ltri;
     float* d_lcdf;
-    if ((rc = upload(c, &d_nodes, nodes)) || (rc = upload(c, &d_isect, tri_isect)) ||
-        (rc = upload(c, &d_shade, tri_shade)) || (rc = upload(c, &d_mats, mats)) ||
-        (rc = upload(c, &d_ltri, ltri)) || (rc = upload(c, &d_lcdf, lcdf)))
+    if ((rc = upload(c, &d_nodes, nodes, up)) || (rc = upload(c, &d_isect, tri_isect, up)) ||
+        (rc = upload(c, &d_shade, tri_shade, up)) || (rc = upload(c, &d_mats, mats, up)) ||
+        (rc = upload(c, &d_ltri, ltri, up)) || (rc = upload(c, &d_lcdf, lcdf, up)))
         return rc;
-    c->scene.nodes = d_nodes;
-    c->scene.tri_isect = d_isect;
-    c->scene.tri_shade = d_shade;
-    c->scene.mats = d_mats;
-    c->scene.light_tri = d_ltri;
-    c->scene.light_cdf = d_lcdf;
-    c->scene.num_nodes = c->N;
-    c->scene.num_lights = c->NL;
-    c->scene.light_area = s->light_area;
-    c->scene.bsphere_x = s->bsphere[0];
-    c->scene.bsphere_y = s->bsphere[1];
-    c->scene.bsphere_z = s->bsphere[2];
-    c->scene.bsphere_r = s->bsphere[3];
-    const ppm_camera& k = s->camera;
-    c->cam = DevCamera{k.origin[0], k.origin[1], k.origin[2], k.u[0], k.u[1], k.u[2], k.v[0], k.v[1], k.v[2],
-                       k.w[0],      k.w[1],      k.w[2],      k.half_width, k.half_height, k.distance};
-
-    c->tiles.width = p->image_width;
-    c->tiles.height = p->image_height;
-    c->tiles.tiles_x = (p->image_width + 15) / 16;
-    c->eye_blocks = c->tiles.tiles_x * ((p->image_height + 15) / 16);
-    c->photon_blocks = (c->P + 255) / 256;
+    DevScene& sc = J.s;
+    sc.nodes = d_nodes;
+    sc.tri_isect = d_isect;
+    sc.tri_shade = d_shade;
+    sc.mats = d_mats;
+    sc.light_tri = d_ltri;
+    sc.light_cdf = d_lcdf;
+    sc.num_nodes = c->N;
+    sc.num_lights = c->NL;
+    sc.light_area = s->light_area;
+    sc.bsphere_x = s->bsphere[0];
+    sc.bsphere_y = s->bsphere[1];
+    sc.bsphere_z = s->bsphere[2];
+    sc.bsphere_r = s->bsphere[3];
     // bounding cube of the scene for the photon ray buckets
-    c->scene.cube_lo_x = s->bsphere[0] - s->bsphere[3];
-    c->scene.cube_lo_y = s->bsphere[1] - s->bsphere[3];
-    c->scene.cube_lo_z = s->bsphere[2] - s->bsphere[3];
-    c->scene.cube_cells = s->bsphere[3] > 0.0f ? (float)(1 << kRayCellBits) / (2.0f * s->bsphere[3]) : 0.0f;
+    sc.cube_lo_x = s->bsphere[0] - s->bsphere[3];
+    sc.cube_lo_y = s->bsphere[1] - s->bsphere[3];
+    sc.cube_lo_z = s->bsphere[2] - s->bsphere[3];
+    sc.cube_cells = s->bsphere[3] > 0.0f ? (float)(1 << kRayCellBits) / (2.0f * s->bsphere[3]) : 0.0f;
+    const ppm_camera& k = s->camera;
+    J.cam = DevCamera{k.origin[0], k.origin[1], k.origin[2], k.u[0], k.u[1], k.u[2], k.v[0], k.v[1], k.v[2],
+                      k.w[0],      k.w[1],      k.w[2],      k.half_width, k.half_height, k.distance};
+    J.ep.width = p->image_width;
+    J.ep.height = p->image_height;
+    J.ep.focal_length = p->focal_length;
+    J.ep.aperture = p->aperture;
+    J.ep.max_path = p->max_path_length;
+    J.ep.fixed_radius = p->fixed_radius;
+    J.ep.footprint = p->initial_footprint;
+    J.ep.footprint_width = (float)p->image_width;
+    J.pp.count = c->P;
+    J.pp.width = p->photon_width;
+    J.pp.max_path = p->max_path_length;
+    J.pp.rep_limit = rep_limit(*p);
+    J.gp.count = c->Q;
+    J.gp.alpha = p->alpha;
+    J.gp.draw = (p->flags & PPM_FLAG_NO_DRAW) ? 0 : 1;
+    J.tl.width = p->image_width;
+    J.tl.height = p->image_height;
+    J.tl.tiles_x = (p->image_width + 15) / 16;
+    J.tile_blocks = tile_blocks(p);
+    J.photon_blocks = photon_blocks(p);
+    J.clear_blocks = clear_blocks(p);
+
     const size_t Q = c->Q, P = c->P, H = c->H;
-    HitState& hs = c->hit;
-    PhotonState& ps = c->ph;
+    HitState& hs = J.hs;
+    PhotonState& ps = J.ps;
+    RayQueue& rq = J.rq;
     if ((rc = dev_alloc(c, &hs.pos, Q)) || (rc = dev_alloc(c, &hs.refl, Q)) || (rc = dev_alloc(c, &hs.nrm, Q)) ||
         (rc = dev_alloc(c, &hs.rng, Q)) || (rc = dev_alloc(c, &hs.tau_r, Q)) || (rc = dev_alloc(c, &hs.e_n, Q)) ||
         (rc = dev_alloc(c, &hs.tri, Q)) || (rc = dev_alloc(c, &hs.mat_t, Q)) || (rc = dev_alloc(c, &hs.rgba8, Q)) ||
-        (rc = dev_alloc(c, &hs.partial, (size_t)c->eye_blocks * 6)) || (rc = dev_alloc(c, &ps.pos, P)) ||
+        (rc = dev_alloc(c, &hs.partial, (size_t)J.tile_blocks * 6)) || (rc = dev_alloc(c, &ps.pos, P)) ||
         (rc = dev_alloc(c, &ps.flux, P)) || (rc = dev_alloc(c, &ps.dir, P)) || (rc = dev_alloc(c, &ps.rng, P)) ||
         (rc = dev_alloc(c, &ps.state, P)) || (rc = dev_alloc(c, &ps.slot, P)) ||
-        (rc = dev_alloc(c, &ps.partial, (size_t)c->photon_blocks)) ||
-        (rc = dev_alloc(c, &c->rays.org, P)) || (rc = dev_alloc(c, &c->rays.dir, P)) ||
-        (rc = dev_alloc(c, &c->rays.aux, P)) || (rc = dev_alloc(c, &c->rays.key_rank, P)) ||
-        (rc = dev_alloc(c, &c->rays.perm, P)) || (rc = dev_alloc(c, &c->rays.bucket, (size_t)kRayBuckets)) || (rc = dev_alloc(c, &c->hash.count, H)) ||
-        (rc = dev_alloc(c, &c->hash.rep, H)) || (rc = dev_alloc(c, &c->glob, 1)) ||
+        (rc = dev_alloc(c, &ps.partial, (size_t)J.photon_blocks)) || (rc = dev_alloc(c, &rq.org, P)) ||
+        (rc = dev_alloc(c, &rq.dir, P)) || (rc = dev_alloc(c, &rq.aux, P)) || (rc = dev_alloc(c, &rq.key_rank, P)) ||
+        (rc = dev_alloc(c, &rq.perm, P)) || (rc = dev_alloc(c, &rq.bucket, (size_t)kRayBuckets)) ||
+        (rc = dev_alloc(c, &J.h.count, H)) || (rc = dev_alloc(c, &J.h.rep, H)) || (rc = dev_alloc(c, &J.g, 1)) ||
         (rc = dev_alloc(c, &c->resolve_lin, Q)) || (rc = dev_alloc(c, &c->resolve_map, Q)))
         return rc;
-    c->hash.size = c->H;
-    c->hash.width = p->hash_width;
-    c->hash.height = p->hash_height;
-    c->hash.num_f = (float)c->H;
+    J.h.size = c->H;
+    J.h.width = p->hash_width;
+    J.h.height = p->hash_height;
+    J.h.num_f = (float)c->H;
 
-    // per-pixel and per-photon generator states
-    std::vector<float4> seeds;
-    seed_grid(c->xs, seeds, Q);
-    HIP_TRY(c, hipMemcpy(hs.rng, seeds.data(), Q * sizeof(float4), hipMemcpyHostToDevice));
-    seed_grid(c->xs, seeds, P);
-    HIP_TRY(c, hipMemcpy(ps.rng, seeds.data(), P * sizeof(float4), hipMemcpyHostToDevice));
+    // per-pixel and per-photon generator states, then the stream's state for the AAOffset draws
+    std::vector<float4> eye_seeds, photon_seeds;
+    seed_grid(xs, eye_seeds, Q);
+    seed_grid(xs, photon_seeds, P);
+    HIP_TRY(c, hipMemcpyAsync(hs.rng, eye_seeds.data(), Q * sizeof(float4), hipMemcpyHostToDevice, up));
+    HIP_TRY(c, hipMemcpyAsync(ps.rng, photon_seeds.data(), P * sizeof(float4), hipMemcpyHostToDevice, up));
     DevGlobals g0;
     std::memset(&g0, 0, sizeof(g0));
-    HIP_TRY(c, hipMemcpy(c->glob, &g0, sizeof(g0), hipMemcpyHostToDevice));
-    const int n = (int)std::max(Q, P);
-    hipLaunchKernelGGL(k_init_state, dim3((n + 255) / 256), dim3(256), 0, c->stream, c->Q, c->P, hs, ps);
+    std::memcpy(g0.xs, xs, sizeof(xs));
+    g0.passes_left = passes_left;
+    HIP_TRY(c, hipMemcpyAsync(J.g, &g0, sizeof(g0), hipMemcpyHostToDevice, up));
+    const int n = (int)std::max(std::max(Q, P), std::max(H, (size_t)kRayBuckets));
+    hipLaunchKernelGGL(k_init_state, dim3((n + 255) / 256), dim3(256), 0, up, J);
     HIP_TRY(c, hipGetLastError());
-    hipLaunchKernelGGL(k_clear_slots, dim3(std::min<int>((c->H + 255) / 256, 65536)), dim3(256), 0, c->stream,
-                       c->hash, c->rays, c->glob);
-    HIP_TRY(c, hipGetLastError());
-    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    HIP_TRY(c, hipStreamSynchronize(up));
     return PPM_OK;
 }
 
-int check_params(const ppm_params* p) {
-    if (!p) return set_err(nullptr, PPM_E_INVAL, "null params");
-    if (p->image_width <= 0 || p->image_height <= 0 || p->photon_width <= 0 || p->photon_height <= 0 ||
-        p->hash_width <= 0 || p->hash_height <= 0)
-        return set_err(nullptr, PPM_E_INVAL, "sizes must be positive");
-    const long long Q = (long long)p->image_width * p->image_height;
-    const long long P = (long long)p->photon_width * p->photon_height;
-    const long long H = (long long)p->hash_width * p->hash_height;
-    if (Q > (1ll << 26) || P > (1ll << 26)) return set_err(nullptr, PPM_E_INVAL, "Q and P are limited to 2^26");
-    if (P > 0x7fffffffll) return set_err(nullptr, PPM_E_INVAL, "P too large");
-    if (H > (1ll << 24)) return set_err(nullptr, PPM_E_INVAL, "H is limited to 2^24 (slot numbers are exact floats)");
-    if (!is_pow2(p->hash_width)) return set_err(nullptr, PPM_E_INVAL, "hash_width must be a power of two");
-    if (p->max_path_length <= 0 || p->max_path_length > 0xffff)
-        return set_err(nullptr, PPM_E_INVAL, "max_path_length out of range");
-    if (!(p->fixed_radius >= 0.0f) || !(p->initial_footprint > 0.0f) || !(p->alpha > 0.0f && p->alpha <= 1.0f))
-        return set_err(nullptr, PPM_E_INVAL, "bad radius or alpha");
-    return PPM_OK;
+void release_context(ppm_ctx* c) {
+    (void)hipSetDevice(c->device);
+    for (auto& pe : c->ev_pending) {
+        (void)hipEventDestroy(pe.second.first);
+        (void)hipEventDestroy(pe.second.second);
+    }
+    c->ev_pending.clear();
+    for (hipEvent_t e : c->ev_pool) (void)hipEventDestroy(e);
+    c->ev_pool.clear();
+    for (int k = 0; k < 2; k++)
+        if (c->graph_pass[k]) {
+            (void)hipGraphExecDestroy(c->graph_pass[k]);
+            c->graph_pass[k] = nullptr;
+        }
+    for (void* p : c->allocs) {
+        if (c->arena) c->arena->release(p, c->stream);
+        else (void)hipFree(p);
+    }
+    c->allocs.clear();
+    c->device_bytes = 0;
+    if (c->ev0) (void)hipEventDestroy(c->ev0);
+    if (c->ev1) (void)hipEventDestroy(c->ev1);
+    if (c->stream && c->own_stream) (void)hipStreamDestroy(c->stream);
+    c->ev0 = c->ev1 = nullptr;
+    c->stream = nullptr;
 }
 
-}  // namespace
+}  // namespace ppm_internal
 
 extern "C" {
 
@@ -424,13 +601,13 @@ int ppm_create(const ppm_scene* scene, const ppm_params* params, ppm_ctx** out) 
     if (!out) return set_err(nullptr, PPM_E_INVAL, "null output");
     *out = nullptr;
     int rc;
-    if ((rc = check_params(params)) || (rc = check_scene(scene))) return rc;
+    if ((rc = ppm_internal::validate(scene, params))) return rc;
     ppm_ctx* c = new (std::nothrow) ppm_ctx();
     if (!c) return set_err(nullptr, PPM_E_OOM, "out of host memory");
-    rc = create_impl(c, scene, params);
+    rc = ppm_internal::create_context(c, scene, params, nullptr, nullptr, nullptr, 0xffffffffu);
     if (rc != PPM_OK) {
         g_create_error = c->err.empty() ? g_create_error : c->err;
-        free_all(c);
+        ppm_internal::release_context(c);
         delete c;
         return rc;
     }
@@ -440,66 +617,20 @@ int ppm_create(const ppm_scene* scene, const ppm_params* params, ppm_ctx** out) 
 
 int ppm_pass(ppm_ctx* c, uint32_t npasses) {
     if (!c) return PPM_E_INVAL;
+    if (!c->own_stream) return set_err(c, PPM_E_INVAL, "this context belongs to a batch; use ppm_batch_pass");
     HIP_TRY(c, hipSetDevice(c->device));
-    const ppm_params& p = c->prm;
-    const unsigned L = (unsigned)p.max_path_length;
     if (npasses == 0) return PPM_OK;
+    const uint64_t L = (uint64_t)c->prm.max_path_length;
+    const bool graph = (c->prm.flags & PPM_FLAG_GRAPH) && !c->profiling;
+    if (graph) {
+        int rc;
+        if ((rc = capture_pass(c, 0)) || (rc = capture_pass(c, 1))) return rc;
+    }
     HIP_TRY(c, hipEventRecord(c->ev0, c->stream));
     for (uint32_t it = 0; it < npasses; it++) {
-        if (c->frame % L == 0) {
-            EyeParams ep;
-            ep.width = p.image_width;
-            ep.height = p.image_height;
-            ep.aa_x = (xs_frand(c->xs) - 0.5f) * 1.25f;
-            ep.aa_y = (xs_frand(c->xs) - 0.5f) * 1.25f;
-            ep.focal_length = p.focal_length;
-            ep.aperture = p.aperture;
-            ep.max_path = p.max_path_length;
-            ep.num_eye_samples = (int)(c->eye_samples + 1);
-            ep.fixed_radius = p.fixed_radius;
-            ep.footprint = p.initial_footprint;
-            ep.footprint_width = (float)p.image_width;
-            c->eye_samples++;
-            hipEvent_t e0 = prof_begin(c);
-            hipLaunchKernelGGL(k_eye, dim3(c->eye_blocks), dim3(256), 0, c->stream, c->scene, c->cam, ep, c->tiles,
-                               c->hit, c->glob);
-            hipLaunchKernelGGL(k_grid, dim3(1), dim3(1024), 0, c->stream, ep, (const float*)c->hit.partial,
-                               c->eye_blocks, c->glob);
-            prof_end(c, PPM_KERNEL_EYE, e0);
-        }
-        hipEvent_t e1 = prof_begin(c);
-        hipLaunchKernelGGL(k_clear_slots, dim3(std::min<int>((c->H + 255) / 256, 65536)), dim3(256), 0, c->stream,
-                           c->hash, c->rays, c->glob);
-        prof_end(c, PPM_KERNEL_CLEAR, e1);
-        PhotonParams pp;
-        pp.count = c->P;
-        pp.width = p.photon_width;
-        pp.max_path = p.max_path_length;
-        // scatter.vs: depth z = s/Pw + t; points with z >= 1 fail the depth test, which removes
-        // the columns i >= Pw^2 / (2 Ph) - 1/2 of the last photon row
-        {
-            const double first = std::ceil((double)p.photon_width * p.photon_width / (2.0 * p.photon_height) - 0.5);
-            const long long ex = (long long)p.photon_width - (long long)std::max(0.0, first);
-            pp.rep_limit = (int)(c->P - std::max(0ll, ex));
-        }
-        hipEvent_t e2 = prof_begin(c);
-        hipLaunchKernelGGL(k_photon_gen, dim3(c->photon_blocks), dim3(256), 0, c->stream, c->scene, pp, c->ph,
-                           c->rays);
-        hipLaunchKernelGGL(k_bucket_scan, dim3(1), dim3(1024), 0, c->stream, c->rays, (const unsigned int*)c->ph.partial,
-                           c->photon_blocks, c->glob);
-        hipLaunchKernelGGL(k_bucket_scatter, dim3(c->photon_blocks), dim3(256), 0, c->stream, c->rays, c->P);
-        hipLaunchKernelGGL(k_photon_trace, dim3(c->photon_blocks), dim3(256), 0, c->stream, c->scene, pp, c->ph,
-                           c->hash, c->rays, c->glob);
-        prof_end(c, PPM_KERNEL_PHOTON, e2);
-        GatherParams gp;
-        gp.count = c->Q;
-        gp.alpha = p.alpha;
-        gp.first_pass = c->frame == 0 ? 1 : 0;
-        gp.draw = (p.flags & PPM_FLAG_NO_DRAW) ? 0 : 1;
-        hipEvent_t e3 = prof_begin(c);
-        hipLaunchKernelGGL(k_gather, dim3(c->eye_blocks), dim3(256), 0, c->stream, gp, c->tiles, c->hit, c->ph,
-                           c->hash, c->glob);
-        prof_end(c, PPM_KERNEL_GATHER, e3);
+        const bool eye = c->frame % L == 0;
+        if (graph) HIP_TRY(c, hipGraphLaunch(c->graph_pass[eye ? 1 : 0], c->stream));
+        else queue_pass(c, c->stream, eye);
         c->frame++;
     }
     HIP_TRY(c, hipGetLastError());
@@ -559,35 +690,38 @@ int ppm_readback(ppm_ctx* c, ppm_buf which, void* dst, size_t bytes) {
     HIP_TRY(c, hipSetDevice(c->device));
     HIP_TRY(c, hipStreamSynchronize(c->stream));
     const size_t Q = c->Q, P = c->P, H = c->H;
+    const Job& J = c->job;
     auto d2h = [&](const void* src, size_t n) -> int {
         HIP_TRY(c, hipMemcpy(dst, src, n, hipMemcpyDeviceToHost));
         return PPM_OK;
     };
+    DevGlobals g;
+    HIP_TRY(c, hipMemcpy(&g, J.g, sizeof(g), hipMemcpyDeviceToHost));
     switch (which) {
         case PPM_BUF_RADIANCE:
         case PPM_BUF_TONEMAPPED:
         case PPM_BUF_RGBA8: {
-            if (c->frame == 0) return set_err(c, PPM_E_INVAL, "no pass has run");
-            hipLaunchKernelGGL(k_resolve, dim3((c->Q + 255) / 256), dim3(256), 0, c->stream, c->Q, c->hit, c->glob,
+            if (g.frame == 0) return set_err(c, PPM_E_INVAL, "no pass has run");
+            hipLaunchKernelGGL(k_resolve, dim3((c->Q + 255) / 256), dim3(256), 0, c->stream, c->Q, J.hs, J.g,
                                c->resolve_lin, c->resolve_map);
             HIP_TRY(c, hipGetLastError());
             HIP_TRY(c, hipStreamSynchronize(c->stream));
             if (which == PPM_BUF_RADIANCE) return d2h(c->resolve_lin, Q * 16);
             if (which == PPM_BUF_TONEMAPPED) return d2h(c->resolve_map, Q * 16);
-            return d2h(c->hit.rgba8, Q * 4);
+            return d2h(J.hs.rgba8, Q * 4);
         }
-        case PPM_BUF_HIT_POSITION: return d2h(c->hit.pos, Q * 16);
-        case PPM_BUF_HIT_REFLECTANCE: return d2h(c->hit.refl, Q * 16);
-        case PPM_BUF_HIT_NORMAL: return d2h(c->hit.nrm, Q * 16);
-        case PPM_BUF_EYE_RNG: return d2h(c->hit.rng, Q * 16);
-        case PPM_BUF_FLUX_RADIUS: return d2h(c->hit.tau_r, Q * 16);
-        case PPM_BUF_EMISSION_COUNT: return d2h(c->hit.e_n, Q * 16);
+        case PPM_BUF_HIT_POSITION: return d2h(J.hs.pos, Q * 16);
+        case PPM_BUF_HIT_REFLECTANCE: return d2h(J.hs.refl, Q * 16);
+        case PPM_BUF_HIT_NORMAL: return d2h(J.hs.nrm, Q * 16);
+        case PPM_BUF_EYE_RNG: return d2h(J.hs.rng, Q * 16);
+        case PPM_BUF_FLUX_RADIUS: return d2h(J.hs.tau_r, Q * 16);
+        case PPM_BUF_EMISSION_COUNT: return d2h(J.hs.e_n, Q * 16);
         case PPM_BUF_HIT_TRIANGLE:
         case PPM_BUF_HIT_T: {
             std::vector<int> tri(Q);
             std::vector<float2> mt(Q);
-            HIP_TRY(c, hipMemcpy(tri.data(), c->hit.tri, Q * 4, hipMemcpyDeviceToHost));
-            HIP_TRY(c, hipMemcpy(mt.data(), c->hit.mat_t, Q * 8, hipMemcpyDeviceToHost));
+            HIP_TRY(c, hipMemcpy(tri.data(), J.hs.tri, Q * 4, hipMemcpyDeviceToHost));
+            HIP_TRY(c, hipMemcpy(mt.data(), J.hs.mat_t, Q * 8, hipMemcpyDeviceToHost));
             if (which == PPM_BUF_HIT_T) {
                 float* o = (float*)dst;
                 for (size_t i = 0; i < Q; i++) o[i] = mt[i].y;
@@ -600,13 +734,13 @@ int ppm_readback(ppm_ctx* c, ppm_buf which, void* dst, size_t bytes) {
             }
             return PPM_OK;
         }
-        case PPM_BUF_PHOTON_POSITION: return d2h(c->ph.pos, P * 16);
-        case PPM_BUF_PHOTON_FLUX: return d2h(c->ph.flux, P * 16);
-        case PPM_BUF_PHOTON_DIRECTION: return d2h(c->ph.dir, P * 16);
-        case PPM_BUF_PHOTON_RNG: return d2h(c->ph.rng, P * 16);
+        case PPM_BUF_PHOTON_POSITION: return d2h(J.ps.pos, P * 16);
+        case PPM_BUF_PHOTON_FLUX: return d2h(J.ps.flux, P * 16);
+        case PPM_BUF_PHOTON_DIRECTION: return d2h(J.ps.dir, P * 16);
+        case PPM_BUF_PHOTON_RNG: return d2h(J.ps.rng, P * 16);
         case PPM_BUF_PHOTON_STATE: {
             std::vector<int2> st(P);
-            HIP_TRY(c, hipMemcpy(st.data(), c->ph.state, P * 8, hipMemcpyDeviceToHost));
+            HIP_TRY(c, hipMemcpy(st.data(), J.ps.state, P * 8, hipMemcpyDeviceToHost));
             int32_t* o = (int32_t*)dst;
             for (size_t i = 0; i < P; i++) {
                 o[3 * i] = st[i].x;
@@ -615,25 +749,21 @@ int ppm_readback(ppm_ctx* c, ppm_buf which, void* dst, size_t bytes) {
             }
             return PPM_OK;
         }
-        case PPM_BUF_PHOTON_SLOT: return d2h(c->ph.slot, P * 4);
-        case PPM_BUF_SLOT_COUNT: return d2h(c->hash.count, H * 4);
-        case PPM_BUF_SLOT_REP: return d2h(c->hash.rep, H * 4);
+        case PPM_BUF_PHOTON_SLOT: return d2h(J.ps.slot, P * 4);
+        case PPM_BUF_SLOT_COUNT: return d2h(J.h.count, H * 4);
+        case PPM_BUF_SLOT_REP: return d2h(J.h.rep, H * 4);
         case PPM_BUF_COUNTERS: {
-            DevGlobals g;
-            HIP_TRY(c, hipMemcpy(&g, c->glob, sizeof(g), hipMemcpyDeviceToHost));
             ppm_counters k;
             std::memset(&k, 0, sizeof(k));
-            k.frame_count = c->frame;
-            k.eye_samples = c->eye_samples;
+            k.frame_count = g.frame;
+            k.eye_samples = g.eye_samples;
             k.photons_emitted = g.photons_emitted;
-            k.photon_segments = (uint64_t)c->P * c->frame;
+            k.photon_segments = (uint64_t)c->P * g.frame;
             k.num_photons_f = g.num_photons_f;
             std::memcpy(dst, &k, sizeof(k));
             return PPM_OK;
         }
         case PPM_BUF_GRID: {
-            DevGlobals g;
-            HIP_TRY(c, hipMemcpy(&g, c->glob, sizeof(g), hipMemcpyDeviceToHost));
             ppm_grid k;
             for (int i = 0; i < 3; i++) {
                 k.bbmin[i] = g.bbmin[i];
@@ -688,7 +818,7 @@ void ppm_destroy(ppm_ctx* c) {
     if (!c) return;
     (void)hipSetDevice(c->device);
     if (c->stream) (void)hipStreamSynchronize(c->stream);
-    free_all(c);
+    ppm_internal::release_context(c);
     delete c;
 }
 

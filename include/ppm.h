@@ -112,6 +112,8 @@ typedef struct ppm_params {
 } ppm_params;
 
 #define PPM_FLAG_NO_DRAW 1u     /* skip the per-pass RGBA8 draw (re.fs); readback still resolves */
+#define PPM_FLAG_GRAPH 2u       /* ppm_pass replays each pass as a captured hipGraph (one for eye
+                                   passes, one for photon-only passes) instead of 9 launches */
 
 /* Buffers ppm_readback can copy out.  Element types and counts are given per entry; the byte
  * count passed must equal the buffer size exactly. */
@@ -196,6 +198,68 @@ int ppm_profile_read(ppm_ctx* ctx, double* ms, uint64_t* launches);
 
 void ppm_destroy(ppm_ctx* ctx);
 const char* ppm_last_error(const ppm_ctx* ctx);
+
+/* ---------------- batched pass loop ----------------
+ *
+ * No counterpart in the reference, which renders one scene per process (glslppm.cpp:664-832).  A
+ * batch holds many jobs -- each one scene, camera, parameter set and SPPM state, exactly what
+ * ppm_create holds -- resident in HBM, and ppm_batch_pass advances all of them with one launch per
+ * stage of m_display (glslppm.cpp:214-536).  Each job keeps its own frame count, eye cadence,
+ * XORShift stream and pass budget on the device; a job whose budget is spent is skipped by the
+ * kernels.  Every job's buffers after n of its passes are bit-identical to those of the same
+ * scene and parameters after ppm_pass(ctx, n).  The launch sequence is captured as a hipGraph of
+ * passes_per_graph passes and replayed; it is captured again after ppm_batch_add/remove.  Device
+ * memory comes from a caching pool owned by the batch, so adding and removing jobs does not wait
+ * for the passes in flight.  A batch is used by one thread at a time. */
+
+typedef struct ppm_batch ppm_batch;
+
+typedef struct ppm_batch_config {
+    int32_t device;            /* HIP device ordinal; every job's params.device must match */
+    uint32_t passes_per_graph; /* passes per captured hipGraph (1-64); 0: plain launches */
+} ppm_batch_config;
+
+#define PPM_PASSES_UNLIMITED 0xffffffffu
+
+void ppm_batch_config_default(ppm_batch_config* cfg);
+int ppm_batch_create(const ppm_batch_config* cfg, ppm_batch** out);
+
+/* Adds a job (copies the scene, allocates its state, seeds it like ppm_create) with a budget of
+ * `passes` passes (PPM_PASSES_UNLIMITED: no limit).  Returns its id in *job. */
+int ppm_batch_add(ppm_batch* b, const ppm_scene* scene, const ppm_params* params, uint32_t passes, int32_t* job);
+
+/* Removes a job; its memory is reused once the passes queued so far have finished. */
+int ppm_batch_remove(ppm_batch* b, int32_t job);
+
+/* Sets a job's remaining pass budget, after the passes queued so far.  Synchronises. */
+int ppm_batch_set_passes(ppm_batch* b, int32_t job, uint32_t passes);
+
+/* Queues npasses batched passes; returns once they are queued. */
+int ppm_batch_pass(ppm_batch* b, uint32_t npasses);
+
+/* Waits for the queued passes. */
+int ppm_batch_sync(ppm_batch* b);
+
+/* Device time in milliseconds of the last ppm_batch_pass call (HIP events); -1 if unavailable. */
+float ppm_batch_last_pass_ms(ppm_batch* b);
+
+/* Passes a job has run and its remaining budget.  Synchronises. */
+int ppm_batch_progress(ppm_batch* b, int32_t job, uint64_t* frames, uint32_t* passes_left);
+
+/* ppm_readback and ppm_buffer_bytes of one job. */
+int ppm_batch_readback(ppm_batch* b, int32_t job, ppm_buf which, void* dst, size_t bytes);
+size_t ppm_batch_buffer_bytes(ppm_batch* b, int32_t job, ppm_buf which);
+
+int32_t ppm_batch_num_jobs(const ppm_batch* b);
+
+/* HBM held by the batch's pool (jobs in use plus cached blocks). */
+uint64_t ppm_batch_device_bytes(const ppm_batch* b);
+
+/* HBM a job of this scene and these parameters allocates in a batch. */
+uint64_t ppm_scene_footprint(const ppm_scene* scene, const ppm_params* params);
+
+void ppm_batch_destroy(ppm_batch* b);
+const char* ppm_batch_last_error(const ppm_batch* b);
 
 /* ---------------- host scene layer (mesh.cpp, camera.cpp, bvh.cpp) ---------------- */
 

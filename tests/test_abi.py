@@ -111,3 +111,55 @@ def test_default_params_are_the_reference_constants():
     assert p.max_path_length == 10 and p.alpha == np.float32(0.7) and p.initial_footprint == 2.5
     assert p.focal_length == 13.0 and p.aperture == 0.0
     assert list(p.xorshift) == [123456789, 362436069, 521288629, 88675123]
+
+
+def test_unknown_flag_is_rejected():
+    p = g.default_params(32)
+    p.flags = 4
+    assert _create(_scene(), p) == -1
+
+
+def test_null_batch_calls_fail_cleanly():
+    lib = g.library()
+    assert lib.ppm_batch_pass(None, 1) == -1
+    assert lib.ppm_batch_sync(None) == -1
+    assert lib.ppm_batch_remove(None, 0) == -1
+    assert lib.ppm_batch_set_passes(None, 0, 1) == -1
+    assert lib.ppm_batch_buffer_bytes(None, 0, 0) == 0
+    assert lib.ppm_batch_num_jobs(None) == 0
+    assert lib.ppm_batch_device_bytes(None) == 0
+    assert lib.ppm_batch_last_pass_ms(None) == -1.0
+    buf = np.zeros(16, np.uint8)
+    assert lib.ppm_batch_readback(None, 0, 0, buf.ctypes.data, 16) == -1
+    assert lib.ppm_batch_create(None, None) == -1
+    lib.ppm_batch_destroy(None)
+
+
+def test_batch_create_without_gpu_reports_no_device():
+    count = ctypes.c_int(0)
+    try:
+        hip = ctypes.CDLL("libamdhip64.so")
+        rc = hip.hipGetDeviceCount(ctypes.byref(count))
+    except OSError:
+        rc = 1
+    if rc == 0 and count.value > 0:
+        pytest.skip("a GPU is present")
+    cfg = g.BatchConfig()
+    g.library().ppm_batch_config_default(ctypes.byref(cfg))
+    assert cfg.passes_per_graph == 1
+    h = ctypes.c_void_p()
+    assert g.library().ppm_batch_create(ctypes.byref(cfg), ctypes.byref(h)) == -5
+    assert g.library().ppm_batch_last_error(None)
+
+
+def test_scene_footprint_counts_every_state_array():
+    """Per-job HBM: 144 B per pixel, 136 B per photon slot, 8 B per hash slot, plus the scene and
+    small tables, each array rounded up to the pool's block size (at most 1/8 above)."""
+    m = _scene()
+    small = g.scene_footprint(m, g.default_params(32))
+    p = g.default_params(1024)
+    big = g.scene_footprint(m, p)
+    Q = P = H = 1024 * 1024
+    exact = 144 * Q + 136 * P + 8 * H
+    assert exact <= big <= exact * 1.125 + 64 * 65536
+    assert small < big

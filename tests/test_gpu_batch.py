@@ -1,0 +1,125 @@
+"""Batched pass loop (ppm_batch_*) and graph replay against the plain loop, bit for bit.
+
+A job advanced inside a batch -- next to other scenes, resolutions, hash sizes, bounce caps and
+pass budgets, added late or removed early -- must end with exactly the buffers of the same scene
+and parameters run alone through ppm_create / ppm_pass for the same number of its passes.
+"""
+import pytest
+
+import glslppm_amd as g
+from parity import STATE_BUFFERS, compare_all, format_report
+
+pytestmark = pytest.mark.gpu
+
+
+class JobView:
+    """A batch job with the readback interface of a Renderer."""
+
+    def __init__(self, batch, job):
+        self.b, self.j = batch, job
+
+    def readback(self, name):
+        return self.b.readback(self.j, name)
+
+    def counters(self):
+        return self.b.counters(self.j)
+
+    def grid(self):
+        return self.b.grid(self.j)
+
+
+def cornell(w, h=None):
+    return g.Mesh.generate(g.SCENE_CORNELL).set_camera((0, 0, 1.667), (0, 0, 0), w, h or w, 45.0).build()
+
+
+def spheres(seed, count, detail, w, h):
+    m = g.Mesh.generate(g.SCENE_SPHERES, seed=seed, count=count, detail=detail)
+    return m.set_camera((0.1, 0.05, 1.6), (0, 0, 0), w, h, 55.0).build()
+
+
+def ibl(res):
+    base = cornell(res).arrays()
+    mats = [(rgb, eta, 0 if brdf == -1 else brdf, gl) for (rgb, eta, brdf, gl) in base["materials"]]
+    m = g.Mesh.from_arrays(base["positions"], mats, base["material"], base["normals"])
+    return m.set_camera((0, 0, 1.667), (0, 0, 0), res, res, 45.0).build()
+
+
+def job_specs():
+    """(label, mesh, params, budget) for a mix of shapes and cadences."""
+    return [
+        ("cornell-64", cornell(64), g.default_params(64, max_path_length=8), 20),
+        ("spheres-48x40", spheres(7, 6, 2, 48, 40),
+         g.default_params(64, image_width=48, image_height=40, photon_width=64, photon_height=32, hash_width=128,
+                          hash_height=16, max_path_length=6), 13),
+        ("cornell-32-unlimited", cornell(32), g.default_params(32, max_path_length=5, fixed_radius=0.02),
+         g.PASSES_UNLIMITED),
+        ("ibl-48", ibl(48), g.default_params(48, hash_width=64, hash_height=32, max_path_length=5), 7),
+    ]
+
+
+def check_equal_alone(view, mesh, params, passes, label):
+    print("compare %s after %d passes" % (label, passes), flush=True)
+    alone = g.Renderer(mesh, params)
+    alone.run(passes)
+    rep = compare_all(view, alone, STATE_BUFFERS)
+    bad = format_report(rep)
+    assert not bad, "%s: batch job differs from the job alone: %s" % (label, bad)
+    assert view.counters()["frame_count"] == passes
+
+
+@pytest.mark.parametrize("passes_per_graph", [1, 4, 0])
+def test_batch_jobs_equal_alone(passes_per_graph):
+    print("passes_per_graph=%d" % passes_per_graph, flush=True)
+    b = g.Batch(passes_per_graph=passes_per_graph)
+    specs = job_specs()
+    ids = [b.add(m, p, n) for (_, m, p, n) in specs]
+    b.run(5)
+    late_mesh = cornell(40, 24)
+    late_params = g.default_params(40, image_width=40, image_height=24, hash_width=64, hash_height=16,
+                                   max_path_length=4)
+    late = b.add(late_mesh, late_params, 30)
+    b.run(10)
+    # the IBL job (budget 7) finished during the first 15 passes; check it and remove it
+    assert b.progress(ids[3]) == (7, 0)
+    check_equal_alone(JobView(b, ids[3]), specs[3][1], specs[3][2], 7, specs[3][0])
+    b.remove(ids[3])
+    b.run(10)
+    b.sync()
+    assert b.num_jobs == 4
+    expect = {ids[0]: 20, ids[1]: 13, ids[2]: 25, late: 20}
+    for (label, m, p, _), j in zip(specs[:3], ids[:3]):
+        check_equal_alone(JobView(b, j), m, p, expect[j], label)
+    check_equal_alone(JobView(b, late), late_mesh, late_params, 20, "late-40x24")
+    b.close()
+
+
+def test_batch_budget_extension_and_reuse_of_freed_memory():
+    b = g.Batch(passes_per_graph=2)
+    m, p = cornell(32), g.default_params(32, max_path_length=6)
+    j0 = b.add(m, p, 3)
+    b.run(5)
+    assert b.progress(j0) == (3, 0)
+    b.set_passes(j0, 4)
+    b.run(6)
+    check_equal_alone(JobView(b, j0), m, p, 7, "extended")
+    held = b.device_bytes
+    b.remove(j0)
+    b.sync()
+    j1 = b.add(m, p, 9)  # same sizes: served from the batch's cached blocks
+    assert b.device_bytes == held
+    b.run(9)
+    check_equal_alone(JobView(b, j1), m, p, 9, "reused")
+    b.close()
+
+
+def test_plain_loop_graph_replay_equals_launches():
+    m = spheres(3, 5, 2, 64, 64)
+    p = g.default_params(64, max_path_length=7)
+    pg = g.default_params(64, max_path_length=7, flags=g.FLAG_GRAPH)
+    a = g.Renderer(m, p)
+    b = g.Renderer(m, pg)
+    a.run(9)
+    b.run(4)
+    b.run(5)
+    rep = compare_all(b, a, STATE_BUFFERS)
+    assert not format_report(rep), format_report(rep)
