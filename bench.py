@@ -157,6 +157,118 @@ def run_batch_workload(args):
     print(json.dumps(line), flush=True)
 
 
+SERVE_DESC = ("Poisson arrivals at RATE jobs/s for SECONDS s; each job one of 64 seeded scenes (room + icospheres, "
+              "triangles log-uniform 1k-500k, resolution 128^2-1024^2, P = H = Q, 10 bounces) with a random camera "
+              "(position uniform in the room, FOV uniform 40-70 degrees) and a target pass count log-uniform 16-512")
+
+
+def run_serve_workload(args):
+    """Continuous batching under a Poisson request stream: latency percentiles and throughput,
+    beside the plain one-scene loop timed on a sample of the same jobs."""
+    import threading
+
+    import numpy as np
+
+    import glslppm_amd as g
+    t = time.time()
+    pool = batch_jobs("batch-mix64")
+    build_s = time.time() - t
+    srv = g.Server(hbm_budget=args.hbm_budget << 30, passes_per_step=args.passes_per_step)
+    sids = [srv.add_scene(m) for (m, _, _) in pool]
+    rs = np.random.RandomState(0x5E2E)
+    n_jobs = int(rs.poisson(args.rate * args.serve_seconds))
+    arrivals = np.sort(rs.uniform(0.0, args.serve_seconds, n_jobs))
+    specs = []
+    for k in range(n_jobs):
+        si = int(rs.randint(len(pool)))
+        res = pool[si][1].image_width
+        while True:
+            o = rs.uniform(-0.45, 0.45, 3)
+            if np.linalg.norm(o) > 0.15:
+                break
+        fov = float(rs.uniform(40.0, 70.0))
+        target = int(round(float(np.exp(rs.uniform(np.log(16.0), np.log(512.0))))))
+        p = g.default_params(res, max_path_length=10)
+        p.xorshift[0] = (123456789 + 104729 * k) & 0xFFFFFFFF
+        specs.append((si, (tuple(o), (0.0, 0.0, 0.0), res, res, fov), p, target))
+    cams = [g.make_camera(*sp[1]) for sp in specs]
+    ids = [None] * n_jobs
+    t0 = srv.now()
+
+    def feeder():
+        for k in range(n_jobs):
+            delay = t0 + arrivals[k] - srv.now()
+            if delay > 0:
+                time.sleep(delay)
+            si, _, p, target = specs[k]
+            ids[k] = srv.submit(sids[si], p, target, camera=cams[k])
+
+    th = threading.Thread(target=feeder)
+    th.start()
+    th.join()
+    for j in ids:
+        srv.wait(j)
+    infos = [srv.info(j) for j in ids]
+    st = srv.stats()
+    first = np.array([i["t_first_pass"] - i["t_submit"] for i in infos])
+    target_t = np.array([i["t_done"] - i["t_submit"] for i in infos])
+    work = sum(specs[k][2].photon_width * specs[k][2].photon_height * specs[k][3] for k in range(n_jobs))
+    span = max(i["t_done"] for i in infos) - min(i["t_submit"] for i in infos)
+    # alone versus in the mix, bit for bit, on three jobs
+    same = True
+    for k in (0, n_jobs // 2, n_jobs - 1):
+        si, cam, p, target = specs[k]
+        m = pool[si][0]
+        m.set_camera(*cam)
+        r = g.Renderer(m, p)
+        r.run(target)
+        same &= bool((r.readback("radiance") == srv.result(ids[k], "radiance")).all())
+        same &= bool((r.readback("flux_radius") == srv.result(ids[k], "flux_radius")).all())
+        r.close()
+    srv.close()
+    # the plain one-scene loop on a sample of the same jobs, one after another
+    sample = list(range(0, n_jobs, max(1, n_jobs // args.plain_sample)))
+    plain_s, plain_work = 0.0, 0
+    for k in sample:
+        si, cam, p, target = specs[k]
+        m = pool[si][0]
+        m.set_camera(*cam)
+        r = g.Renderer(m, p)
+        r.sync()
+        t1 = time.perf_counter()
+        r.run(target)
+        r.sync()
+        plain_s += time.perf_counter() - t1
+        plain_work += p.photon_width * p.photon_height * target
+        r.close()
+    pct = lambda a, q: float(np.percentile(a, q))
+    line = {
+        "metric": METRIC,
+        "value": work / span,
+        "unit": "photons/s",
+        "n_gpus": 1,
+        "steps": int(st["steps"]),
+        "warmup": 0,
+        "ms_per_step": span * 1e3 / max(1, st["steps"]),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (seeded scenes, cameras, arrivals and targets)",
+        "config": {"workload": SERVE_DESC.replace("RATE", str(args.rate)).replace("SECONDS", str(args.serve_seconds)),
+                   "jobs": n_jobs, "hbm_budget_gib": args.hbm_budget, "passes_per_step": args.passes_per_step,
+                   "scene_build_s": round(build_s, 2)},
+        "time_to_first_pass_s": {"p50": pct(first, 50), "p99": pct(first, 99), "max": float(first.max())},
+        "time_to_target_s": {"p50": pct(target_t, 50), "p99": pct(target_t, 99), "max": float(target_t.max())},
+        "span_s": span,
+        "gpu_busy_fraction": st["busy_seconds"] / span,
+        "plain_loop": {"value": plain_work / plain_s, "unit": "photons/s", "jobs": len(sample)},
+        "speedup_vs_plain_loop": (work / span) / (plain_work / plain_s),
+        "jobs_equal_alone": same,
+    }
+    print(json.dumps(line), flush=True)
+
+
 def build_scene(w):
     import glslppm_amd as g
     m = g.Mesh.generate(w["kind"], seed=w["seed"], count=w["count"], detail=w["detail"])
@@ -184,8 +296,8 @@ def cpu_baseline(mesh, params, passes):
     dt = time.time() - t
     P = params.photon_width * params.photon_height
     return {"value": P * passes / dt, "unit": "photons/s", "cores": oracle.num_threads(), "kind": "port",
-            "sample": "first %d passes (1 eye pass + %d photon passes) of the same workload, %.1f s" % (
-                passes, passes, dt)}
+            "sample": "first %d passes (%d eye passes, %d photon passes) of the same workload, %.1f s" % (
+                passes, (passes + params.max_path_length - 1) // params.max_path_length, passes, dt)}
 
 
 def main():
@@ -193,10 +305,18 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=50)
     ap.add_argument("--warmup", type=int, default=10)
-    ap.add_argument("--workload", default="spheres-1024", choices=sorted(WORKLOADS) + sorted(BATCH_WORKLOADS))
+    ap.add_argument("--workload", default="spheres-1024",
+                    choices=sorted(WORKLOADS) + sorted(BATCH_WORKLOADS) + ["serve"])
+    ap.add_argument("--rate", type=float, default=20.0, help="serve: job arrivals per second")
+    ap.add_argument("--serve-seconds", type=float, default=20.0, help="serve: length of the arrival window")
+    ap.add_argument("--hbm-budget", type=int, default=200, help="serve: device memory budget in GiB")
+    ap.add_argument("--passes-per-step", type=int, default=4, help="serve: batched passes between admissions")
+    ap.add_argument("--plain-sample", type=int, default=40, help="serve: jobs timed in the plain loop")
     ap.add_argument("--passes-per-graph", type=int, default=1, help="batch workloads: passes per hipGraph")
     ap.add_argument("--graph", action="store_true", help="plain workloads: replay passes as hipGraphs")
-    ap.add_argument("--cpu-passes", type=int, default=10, help="passes of the CPU baseline sample (0: skip)")
+    ap.add_argument("--autotune", action="store_true",
+                    help="plain workloads: pick launch shapes with ppm_autotune during the warm-up")
+    ap.add_argument("--cpu-passes", type=int, default=50, help="passes of the CPU baseline sample (0: skip)")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="first run a plain device copy of MIB MiB (torch), the known-size reference for "
                          "rocprofv3 FETCH_SIZE/WRITE_SIZE counters")
@@ -219,6 +339,9 @@ def main():
     if args.workload in BATCH_WORKLOADS:
         run_batch_workload(args)
         return
+    if args.workload == "serve":
+        run_serve_workload(args)
+        return
 
     import glslppm_amd as g
     w = WORKLOADS[args.workload]
@@ -232,6 +355,9 @@ def main():
     H = params.hash_width * params.hash_height
     L = params.max_path_length
 
+    tuning = None
+    if args.autotune:
+        tuning = r.autotune()
     r.run(args.warmup)
     r.sync()
     c0 = r.counters()
@@ -288,6 +414,7 @@ def main():
         "emitted_photons_per_s": emitted / wall,
         "m_paths_per_s": (emitted + Q * eye_passes) / wall / (1024.0 * 1024.0),
         "kernel_ms_per_launch": {k: (v[0] / v[1] if v[1] else None) for k, v in prof.items()},
+        "tuning": tuning if tuning is not None else r.tuning(),
         "roofline": {"bound": "hbm", "kernel": "k_" + dom, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
                      "bytes_per_launch": bytes_per_launch, "avg_launch_ms": avg_ms},

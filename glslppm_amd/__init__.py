@@ -74,6 +74,31 @@ class BatchConfig(ctypes.Structure):
     _fields_ = [("device", ctypes.c_int32), ("passes_per_graph", ctypes.c_uint32)]
 
 
+class Tuning(ctypes.Structure):
+    _fields_ = [("trace_block", ctypes.c_int32), ("tile_wave_width", ctypes.c_int32),
+                ("passes_per_graph", ctypes.c_uint32), ("ms_per_pass", ctypes.c_float)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+class ServerConfig(ctypes.Structure):
+    _fields_ = [("device", ctypes.c_int32), ("hbm_budget", ctypes.c_uint64), ("max_jobs", ctypes.c_int32),
+                ("passes_per_step", ctypes.c_uint32), ("passes_per_graph", ctypes.c_uint32)]
+
+
+class JobInfo(ctypes.Structure):
+    _fields_ = [("state", ctypes.c_int32), ("target_passes", ctypes.c_uint32), ("passes_done", ctypes.c_uint64),
+                ("footprint", ctypes.c_uint64), ("t_submit", ctypes.c_double), ("t_admit", ctypes.c_double),
+                ("t_first_pass", ctypes.c_double), ("t_done", ctypes.c_double)]
+
+
+class ServerStats(ctypes.Structure):
+    _fields_ = [("queued", ctypes.c_int32), ("resident", ctypes.c_int32), ("done", ctypes.c_int32),
+                ("failed", ctypes.c_int32), ("device_bytes", ctypes.c_uint64), ("steps", ctypes.c_uint64),
+                ("photon_segments", ctypes.c_uint64), ("busy_seconds", ctypes.c_double)]
+
+
 class Counters(ctypes.Structure):
     _fields_ = [("frame_count", ctypes.c_uint64), ("eye_samples", ctypes.c_uint64),
                 ("photons_emitted", ctypes.c_uint64), ("photon_segments", ctypes.c_uint64),
@@ -146,7 +171,27 @@ def library():
         "ppm_mesh_num_triangles": (ctypes.c_int32, [vp]),
         "ppm_mesh_destroy": (None, [vp]),
         "ppm_mesh_last_error": (ctypes.c_char_p, []),
+        "ppm_get_tuning": (ctypes.c_int, [vp, ctypes.POINTER(Tuning)]),
+        "ppm_set_tuning": (ctypes.c_int, [vp, ctypes.POINTER(Tuning)]),
+        "ppm_autotune": (ctypes.c_int, [vp, ctypes.c_uint32, ctypes.POINTER(Tuning)]),
+        "ppm_batch_get_tuning": (ctypes.c_int, [vp, ctypes.POINTER(Tuning)]),
+        "ppm_batch_set_tuning": (ctypes.c_int, [vp, ctypes.POINTER(Tuning)]),
+        "ppm_batch_autotune": (ctypes.c_int, [vp, ctypes.c_uint32, ctypes.POINTER(Tuning)]),
         "ppm_batch_config_default": (None, [ctypes.POINTER(BatchConfig)]),
+        "ppm_server_config_default": (None, [ctypes.POINTER(ServerConfig)]),
+        "ppm_server_create": (ctypes.c_int, [ctypes.POINTER(ServerConfig), ctypes.POINTER(vp)]),
+        "ppm_server_add_scene": (ctypes.c_int, [vp, ctypes.POINTER(Scene), ctypes.POINTER(ctypes.c_int64)]),
+        "ppm_server_remove_scene": (ctypes.c_int, [vp, ctypes.c_int64]),
+        "ppm_server_submit": (ctypes.c_int, [vp, ctypes.c_int64, ctypes.POINTER(Camera), ctypes.POINTER(Params),
+                                             ctypes.c_uint32, ctypes.POINTER(ctypes.c_int64)]),
+        "ppm_server_wait": (ctypes.c_int, [vp, ctypes.c_int64, ctypes.c_double]),
+        "ppm_server_info": (ctypes.c_int, [vp, ctypes.c_int64, ctypes.POINTER(JobInfo)]),
+        "ppm_server_result": (ctypes.c_int, [vp, ctypes.c_int64, ctypes.c_int, vp, ctypes.c_size_t]),
+        "ppm_server_release": (ctypes.c_int, [vp, ctypes.c_int64]),
+        "ppm_server_stats_get": (ctypes.c_int, [vp, ctypes.POINTER(ServerStats)]),
+        "ppm_server_now": (ctypes.c_double, [vp]),
+        "ppm_server_destroy": (None, [vp]),
+        "ppm_server_last_error": (ctypes.c_char_p, [vp]),
         "ppm_batch_create": (ctypes.c_int, [ctypes.POINTER(BatchConfig), ctypes.POINTER(vp)]),
         "ppm_batch_add": (ctypes.c_int, [vp, ctypes.POINTER(Scene), ctypes.POINTER(Params), ctypes.c_uint32,
                                          ctypes.POINTER(ctypes.c_int32)]),
@@ -323,6 +368,28 @@ class Renderer:
     def last_pass_ms(self):
         return float(library().ppm_last_pass_ms(self._h))
 
+    def tuning(self):
+        t = Tuning()
+        self._check(library().ppm_get_tuning(self._h, ctypes.byref(t)))
+        return t.as_dict()
+
+    def set_tuning(self, trace_block=None, tile_wave_width=None, passes_per_graph=None):
+        t = Tuning()
+        self._check(library().ppm_get_tuning(self._h, ctypes.byref(t)))
+        if trace_block is not None:
+            t.trace_block = int(trace_block)
+        if tile_wave_width is not None:
+            t.tile_wave_width = int(tile_wave_width)
+        if passes_per_graph is not None:
+            t.passes_per_graph = int(passes_per_graph)
+        self._check(library().ppm_set_tuning(self._h, ctypes.byref(t)))
+
+    def autotune(self, passes_per_trial=0):
+        """Times real passes under each launch shape and keeps the fastest; returns it."""
+        t = Tuning()
+        self._check(library().ppm_autotune(self._h, int(passes_per_trial), ctypes.byref(t)))
+        return t.as_dict()
+
     def profile(self, enable=True):
         """Starts (and clears) or stops per-kernel event timing."""
         self._check(library().ppm_profile_enable(self._h, 1 if enable else 0))
@@ -425,6 +492,28 @@ class Batch:
         self._check(library().ppm_batch_progress(self._h, int(job), ctypes.byref(f), ctypes.byref(left)))
         return int(f.value), int(left.value)
 
+    def tuning(self):
+        t = Tuning()
+        self._check(library().ppm_batch_get_tuning(self._h, ctypes.byref(t)))
+        return t.as_dict()
+
+    def set_tuning(self, trace_block=None, tile_wave_width=None, passes_per_graph=None):
+        t = Tuning()
+        self._check(library().ppm_batch_get_tuning(self._h, ctypes.byref(t)))
+        if trace_block is not None:
+            t.trace_block = int(trace_block)
+        if tile_wave_width is not None:
+            t.tile_wave_width = int(tile_wave_width)
+        if passes_per_graph is not None:
+            t.passes_per_graph = int(passes_per_graph)
+        self._check(library().ppm_batch_set_tuning(self._h, ctypes.byref(t)))
+
+    def autotune(self, passes_per_trial=0):
+        """Times real passes under each launch shape and keeps the fastest; returns it."""
+        t = Tuning()
+        self._check(library().ppm_batch_autotune(self._h, int(passes_per_trial), ctypes.byref(t)))
+        return t.as_dict()
+
     @property
     def num_jobs(self):
         return int(library().ppm_batch_num_jobs(self._h))
@@ -463,3 +552,106 @@ class Batch:
             self.close()
         except Exception:
             pass
+
+
+JOB_STATES = ("queued", "running", "done", "failed")
+SERVER_BUFFERS = ("radiance", "rgba8", "flux_radius", "emission_count")
+
+
+class Server:
+    """Continuous batching of render jobs (ppm_server_*): submit from any thread, wait, read results.
+
+    A job's results equal those of a Renderer of the same scene, camera and parameters after
+    run(target_passes), bit for bit."""
+
+    def __init__(self, device=0, hbm_budget=200 << 30, max_jobs=256, passes_per_step=4, passes_per_graph=1):
+        cfg = ServerConfig()
+        library().ppm_server_config_default(ctypes.byref(cfg))
+        cfg.device, cfg.hbm_budget, cfg.max_jobs = int(device), int(hbm_budget), int(max_jobs)
+        cfg.passes_per_step, cfg.passes_per_graph = int(passes_per_step), int(passes_per_graph)
+        self._h = ctypes.c_void_p()
+        rc = library().ppm_server_create(ctypes.byref(cfg), ctypes.byref(self._h))
+        if rc != 0:
+            raise PPMError("ppm_server_create failed (%d): %s" % (rc, library().ppm_server_last_error(None).decode()))
+        self._q = {}
+
+    def _check(self, rc):
+        if rc != 0:
+            raise PPMError("ppm_server error %d: %s" % (rc, library().ppm_server_last_error(self._h).decode()))
+
+    def add_scene(self, mesh):
+        """Registers the mesh's built scene (packed now, uploaded with its first job); returns its id."""
+        sid = ctypes.c_int64()
+        self._check(library().ppm_server_add_scene(self._h, mesh.scene, ctypes.byref(sid)))
+        return sid.value
+
+    def remove_scene(self, scene_id):
+        self._check(library().ppm_server_remove_scene(self._h, int(scene_id)))
+
+    def submit(self, scene_id, params, target_passes, camera=None):
+        """Queues a job; camera is a Camera or None (the scene's own).  Returns the job id."""
+        j = ctypes.c_int64()
+        cam = None if camera is None else ctypes.byref(camera)
+        self._check(library().ppm_server_submit(self._h, int(scene_id), cam, ctypes.byref(params),
+                                                int(target_passes), ctypes.byref(j)))
+        self._q[j.value] = params.image_width * params.image_height
+        return j.value
+
+    def wait(self, job, timeout=-1.0):
+        """True when done, False on time-out; raises PPMError when the job failed."""
+        rc = library().ppm_server_wait(self._h, int(job), float(timeout))
+        if rc == 1:
+            return False
+        self._check(rc)
+        return True
+
+    def info(self, job):
+        i = JobInfo()
+        self._check(library().ppm_server_info(self._h, int(job), ctypes.byref(i)))
+        d = {k: getattr(i, k) for k, _ in JobInfo._fields_}
+        d["state"] = JOB_STATES[d["state"]]
+        return d
+
+    def result(self, job, name):
+        bid, dt, comp, _ = BUFFERS[name]
+        out = np.empty((self._q[job], comp), dtype=dt)
+        self._check(library().ppm_server_result(self._h, int(job), bid, out.ctypes.data, out.nbytes))
+        return out
+
+    def counters(self, job):
+        c = Counters()
+        self._check(library().ppm_server_result(self._h, int(job), BUF_COUNTERS, ctypes.addressof(c),
+                                                ctypes.sizeof(c)))
+        return {k: getattr(c, k) for k, _ in Counters._fields_ if k != "reserved"}
+
+    def release(self, job):
+        self._check(library().ppm_server_release(self._h, int(job)))
+        self._q.pop(job, None)
+
+    def stats(self):
+        st = ServerStats()
+        self._check(library().ppm_server_stats_get(self._h, ctypes.byref(st)))
+        return {k: getattr(st, k) for k, _ in ServerStats._fields_}
+
+    def now(self):
+        return float(library().ppm_server_now(self._h))
+
+    def close(self):
+        if self._h and self._h.value:
+            library().ppm_server_destroy(self._h)
+            self._h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def make_camera(origin, lookat, width, height, fov):
+    """TCamera::Set (camera.cpp:4-18) as a Camera structure, through the host scene layer."""
+    m = Mesh.generate(SCENE_CORNELL)
+    m.set_camera(origin, lookat, width, height, fov).build()
+    c = Camera()
+    ctypes.pointer(c)[0] = m.scene.contents.camera
+    return c

@@ -45,7 +45,7 @@ def build(verbose=False):
     os.makedirs(BUILD, exist_ok=True)
     hipcc = _hipcc()
     objs = []
-    for src in ("kernels.hip", "runtime.hip", "batch.hip"):
+    for src in ("kernels.hip", "runtime.hip", "batch.hip", "server.hip"):
         obj = os.path.join(BUILD, src.replace(".hip", ".o"))
         out = _run([hipcc] + HIP_FLAGS + ["-c", os.path.join(CSRC, src), "-o", obj])
         if verbose and out.strip():

@@ -23,30 +23,6 @@
 
 #include "context.h"
 
-struct ppm_batch {
-    int device = 0;
-    hipStream_t stream = nullptr;   // passes
-    hipStream_t upload = nullptr;   // scene uploads and job initialisation
-    ppm_internal::Arena* arena = nullptr;
-    uint32_t passes_per_graph = 1;  // 0: plain launches
-    std::vector<ppm_ctx*> slots;    // job id -> context (null once removed)
-    // device tables of the current job set
-    bool dirty = true;
-    int n_jobs = 0;
-    Job* d_jobs = nullptr;
-    BlockRef* d_map[3] = {nullptr, nullptr, nullptr};  // tiles, photon slots, hash clears
-    int n_blocks[3] = {0, 0, 0};
-    hipGraphExec_t graph = nullptr;
-    struct Retired {
-        hipGraphExec_t g;
-        hipEvent_t ev;
-    };
-    std::vector<Retired> retired;
-    hipEvent_t ev0 = nullptr, ev1 = nullptr;
-    bool timed = false;
-    std::string err;
-};
-
 namespace {
 
 thread_local std::string g_batch_error;
@@ -65,7 +41,7 @@ int berr(ppm_batch* b, int code, const std::string& msg) {
                         std::string(#call) + ": " + hipGetErrorString(e_));                  \
     } while (0)
 
-enum { kMapTile = 0, kMapPhoton = 1, kMapClear = 2 };
+enum { kMapTile = 0, kMapPhoton = 1, kMapClear = 2, kMapTrace = 3, kMaps = 4 };
 
 // Destroys retired graphs whose last launch has finished (all of them when `wait`).
 void reap(ppm_batch* b, bool wait) {
@@ -110,8 +86,8 @@ void queue_batch_pass(ppm_batch* b, hipStream_t st) {
     hipLaunchKernelGGL(k_bucket_scan_batch, dim3(n), dim3(1024), 0, st, J);
     hipLaunchKernelGGL(k_bucket_scatter_batch, dim3(b->n_blocks[kMapPhoton]), dim3(256), 0, st, J,
                        (const BlockRef*)b->d_map[kMapPhoton]);
-    hipLaunchKernelGGL(k_photon_trace_batch, dim3(b->n_blocks[kMapPhoton]), dim3(256), 0, st, J,
-                       (const BlockRef*)b->d_map[kMapPhoton]);
+    hipLaunchKernelGGL(k_photon_trace_batch, dim3(b->n_blocks[kMapTrace]), dim3(b->trace_block), 0, st, J,
+                       (const BlockRef*)b->d_map[kMapTrace]);
     hipLaunchKernelGGL(k_gather_batch, dim3(b->n_blocks[kMapTile]), dim3(256), 0, st, J, (const BlockRef*)b->d_map[kMapTile]);
 }
 
@@ -119,32 +95,35 @@ void queue_batch_pass(ppm_batch* b, hipStream_t st) {
 int rebuild(ppm_batch* b) {
     if (!b->dirty) return PPM_OK;
     std::vector<Job> jobs;
-    std::vector<BlockRef> maps[3];
-    for (ppm_ctx* c : b->slots) {
+    std::vector<BlockRef> maps[kMaps];
+    b->table_ids.clear();
+    for (size_t id = 0; id < b->slots.size(); id++) {
+        ppm_ctx* c = b->slots[id];
         if (!c) continue;
         const int j = (int)jobs.size();
         jobs.push_back(c->job);
-        const int per[3] = {c->job.tile_blocks, c->job.photon_blocks, c->job.clear_blocks};
-        for (int k = 0; k < 3; k++)
+        b->table_ids.push_back((int32_t)id);
+        const int per[kMaps] = {c->job.tile_blocks, c->job.photon_blocks, c->job.clear_blocks, c->job.trace_blocks};
+        for (int k = 0; k < kMaps; k++)
             for (int i = 0; i < per[k]; i++) maps[k].push_back(BlockRef{j, i});
     }
     // the previous tables may still be read by queued passes: hand them back to the arena, which
     // reuses them only after the stream has passed this point
     if (b->d_jobs) b->arena->release(b->d_jobs, b->stream);
-    for (int k = 0; k < 3; k++)
+    for (int k = 0; k < kMaps; k++)
         if (b->d_map[k]) b->arena->release(b->d_map[k], b->stream);
     b->d_jobs = nullptr;
-    for (int k = 0; k < 3; k++) b->d_map[k] = nullptr;
+    for (int k = 0; k < kMaps; k++) b->d_map[k] = nullptr;
     retire_graph(b);
     b->n_jobs = (int)jobs.size();
-    for (int k = 0; k < 3; k++) b->n_blocks[k] = (int)maps[k].size();
+    for (int k = 0; k < kMaps; k++) b->n_blocks[k] = (int)maps[k].size();
     if (b->n_jobs == 0) {
         b->dirty = false;
         return PPM_OK;
     }
     BHIP(b, b->arena->alloc((void**)&b->d_jobs, jobs.size() * sizeof(Job)));
     BHIP(b, hipMemcpyAsync(b->d_jobs, jobs.data(), jobs.size() * sizeof(Job), hipMemcpyHostToDevice, b->upload));
-    for (int k = 0; k < 3; k++) {
+    for (int k = 0; k < kMaps; k++) {
         BHIP(b, b->arena->alloc((void**)&b->d_map[k], maps[k].size() * sizeof(BlockRef)));
         BHIP(b, hipMemcpyAsync(b->d_map[k], maps[k].data(), maps[k].size() * sizeof(BlockRef), hipMemcpyHostToDevice,
                                b->upload));
@@ -169,6 +148,78 @@ ppm_ctx* job_ctx(ppm_batch* b, int32_t job) {
 }
 
 }  // namespace
+
+namespace ppm_internal {
+
+int batch_add_resident(ppm_batch* b, std::shared_ptr<SceneDev> scene, const ppm_camera& cam, const ppm_params* p,
+                       uint32_t passes, int32_t* job) {
+    *job = -1;
+    ppm_ctx* c = new (std::nothrow) ppm_ctx();
+    if (!c) return berr(b, PPM_E_OOM, "out of host memory");
+    const int rc = create_context(c, scene, cam, p, b->stream, b->upload, b->arena, passes);
+    if (rc != PPM_OK) {
+        const std::string m = c->err.empty() ? create_error() : c->err;
+        release_context(c);
+        delete c;
+        return berr(b, rc, m);
+    }
+    set_launch_shape(c, b->trace_block, b->wave_shift);
+    b->slots.push_back(c);
+    *job = (int32_t)(b->slots.size() - 1);
+    b->dirty = true;
+    return PPM_OK;
+}
+
+int batch_prepare(ppm_batch* b) {
+    BHIP(b, hipSetDevice(b->device));
+    return rebuild(b);
+}
+
+int batch_snapshot(ppm_batch* b, unsigned long long* dev_tmp, unsigned long long* host_out, hipEvent_t done,
+                   std::vector<int32_t>* ids) {
+    *ids = b->table_ids;
+    const int n = b->n_jobs;
+    if (n > 0) {
+        hipLaunchKernelGGL(k_batch_status, dim3((n + 63) / 64), dim3(64), 0, b->stream, (const Job*)b->d_jobs, n, dev_tmp);
+        BHIP(b, hipGetLastError());
+        BHIP(b, hipMemcpyAsync(host_out, dev_tmp, (size_t)n * 16, hipMemcpyDeviceToHost, b->stream));
+    }
+    BHIP(b, hipEventRecord(done, b->stream));
+    return PPM_OK;
+}
+
+int batch_fetch(ppm_batch* b, int32_t job, float4* radiance, float4* flux_radius, float4* emission_count,
+                uchar4* rgba8, ppm_counters* counters) {
+    ppm_ctx* c = job_ctx(b, job);
+    if (!c) return berr(b, PPM_E_INVAL, "no such job");
+    BHIP(b, hipSetDevice(b->device));
+    const Job& J = c->job;
+    const size_t Q = (size_t)c->Q;
+    hipStream_t up = b->upload;
+    if (radiance || rgba8) {
+        hipLaunchKernelGGL(k_resolve, dim3((c->Q + 255) / 256), dim3(256), 0, up, c->Q, J.hs, (const DevGlobals*)J.g,
+                           c->resolve_lin, c->resolve_map);
+        BHIP(b, hipGetLastError());
+    }
+    if (radiance) BHIP(b, hipMemcpyAsync(radiance, c->resolve_lin, Q * 16, hipMemcpyDeviceToHost, up));
+    if (flux_radius) BHIP(b, hipMemcpyAsync(flux_radius, J.hs.tau_r, Q * 16, hipMemcpyDeviceToHost, up));
+    if (emission_count) BHIP(b, hipMemcpyAsync(emission_count, J.hs.e_n, Q * 16, hipMemcpyDeviceToHost, up));
+    if (rgba8) BHIP(b, hipMemcpyAsync(rgba8, J.hs.rgba8, Q * 4, hipMemcpyDeviceToHost, up));
+    DevGlobals g;
+    BHIP(b, hipMemcpyAsync(&g, J.g, sizeof(g), hipMemcpyDeviceToHost, up));
+    BHIP(b, hipStreamSynchronize(up));
+    if (counters) {
+        std::memset(counters, 0, sizeof(*counters));
+        counters->frame_count = g.frame;
+        counters->eye_samples = g.eye_samples;
+        counters->photons_emitted = g.photons_emitted;
+        counters->photon_segments = (uint64_t)c->P * g.frame;
+        counters->num_photons_f = g.num_photons_f;
+    }
+    return PPM_OK;
+}
+
+}  // namespace ppm_internal
 
 extern "C" {
 
@@ -218,19 +269,17 @@ int ppm_batch_add(ppm_batch* b, const ppm_scene* scene, const ppm_params* params
     int rc;
     if ((rc = ppm_internal::validate(scene, params))) return berr(b, rc, ppm_internal::create_error());
     if (params->device != b->device) return berr(b, PPM_E_INVAL, "params.device differs from the batch's device");
-    ppm_ctx* c = new (std::nothrow) ppm_ctx();
-    if (!c) return berr(b, PPM_E_OOM, "out of host memory");
-    rc = ppm_internal::create_context(c, scene, params, b->stream, b->upload, b->arena, passes);
-    if (rc != PPM_OK) {
-        const std::string m = c->err.empty() ? ppm_internal::create_error() : c->err;
-        ppm_internal::release_context(c);
-        delete c;
-        return berr(b, rc, m);
+    BHIP(b, hipSetDevice(b->device));
+    ppm_internal::PackedScene ps;
+    if ((rc = ppm_internal::pack_scene(scene, &ps))) return berr(b, rc, ppm_internal::create_error());
+    std::shared_ptr<ppm_internal::SceneDev> sd;
+    std::string err;
+    if ((rc = ppm_internal::upload_scene(ps, b->arena, b->upload, b->stream, &sd, &err))) {
+        (void)hipStreamSynchronize(b->upload);
+        return berr(b, rc, err);
     }
-    b->slots.push_back(c);
-    *job = (int32_t)(b->slots.size() - 1);
-    b->dirty = true;
-    return PPM_OK;
+    BHIP(b, hipStreamSynchronize(b->upload));  // the packed host copy is released on return
+    return ppm_internal::batch_add_resident(b, sd, scene->camera, params, passes, job);
 }
 
 int ppm_batch_remove(ppm_batch* b, int32_t job) {
@@ -328,7 +377,54 @@ uint64_t ppm_batch_device_bytes(const ppm_batch* b) { return b && b->arena ? b->
 
 uint64_t ppm_scene_footprint(const ppm_scene* scene, const ppm_params* params) {
     if (!scene || !params) return 0;
-    return ppm_internal::footprint(scene, params);
+    return ppm_internal::state_footprint(params) +
+           ppm_internal::scene_footprint(scene->num_triangles, scene->num_nodes, scene->num_materials, scene->num_lights);
+}
+
+int ppm_batch_get_tuning(ppm_batch* b, ppm_tuning* t) {
+    if (!b || !t) return PPM_E_INVAL;
+    t->trace_block = b->trace_block;
+    t->tile_wave_width = 1 << b->wave_shift;
+    t->passes_per_graph = b->passes_per_graph;
+    t->ms_per_pass = 0.0f;
+    return PPM_OK;
+}
+
+int ppm_batch_set_tuning(ppm_batch* b, const ppm_tuning* t) {
+    if (!b) return PPM_E_INVAL;
+    int sh = 0;
+    if (ppm_internal::check_tuning(t, &sh)) return berr(b, PPM_E_INVAL, "tuning out of range");
+    b->trace_block = t->trace_block;
+    b->wave_shift = sh;
+    b->passes_per_graph = t->passes_per_graph;
+    for (ppm_ctx* c : b->slots)
+        if (c) ppm_internal::set_launch_shape(c, b->trace_block, b->wave_shift);
+    b->dirty = true;
+    return PPM_OK;
+}
+
+int ppm_batch_autotune(ppm_batch* b, uint32_t passes_per_trial, ppm_tuning* best) {
+    if (!b) return PPM_E_INVAL;
+    if (ppm_batch_num_jobs(b) == 0) return berr(b, PPM_E_INVAL, "the batch has no job to tune on");
+    if (passes_per_trial == 0) {
+        int L = 1;
+        for (ppm_ctx* c : b->slots)
+            if (c) L = std::max(L, c->prm.max_path_length);
+        passes_per_trial = 2u * (uint32_t)L;
+    }
+    ppm_tuning start;
+    ppm_batch_get_tuning(b, &start);
+    auto trial = [&](const ppm_tuning& t, float* ms) -> int {
+        int rc;
+        if ((rc = ppm_batch_set_tuning(b, &t))) return rc;
+        if ((rc = ppm_batch_pass(b, 1))) return rc;  // table upload and graph capture stay out of the timing
+        if ((rc = ppm_batch_pass(b, passes_per_trial))) return rc;
+        const float v = ppm_batch_last_pass_ms(b);
+        if (v < 0.0f) return berr(b, PPM_E_HIP, "trial timing failed");
+        *ms = v / (float)passes_per_trial;
+        return PPM_OK;
+    };
+    return ppm_internal::autotune(start, true, trial, best);
 }
 
 void ppm_batch_destroy(ppm_batch* b) {
@@ -346,7 +442,7 @@ void ppm_batch_destroy(ppm_batch* b) {
     reap(b, true);
     if (b->arena) {
         if (b->d_jobs) b->arena->release(b->d_jobs, nullptr);
-        for (int k = 0; k < 3; k++)
+        for (int k = 0; k < kMaps; k++)
             if (b->d_map[k]) b->arena->release(b->d_map[k], nullptr);
         delete b->arena;
     }

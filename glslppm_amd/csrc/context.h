@@ -7,6 +7,7 @@
 
 #include <cstdint>
 #include <map>
+#include <memory>
 #include <string>
 #include <vector>
 
@@ -45,6 +46,30 @@ private:
     std::map<void*, size_t> size_of_;
 };
 
+// A scene in the form the kernels read (CGPURT::CreateTextures, gpurt.cpp:47-552), on the host.
+struct PackedScene {
+    std::vector<float4> nodes, isect, shade, mats;
+    std::vector<int> light_tri;
+    std::vector<float> light_cdf;
+    int T = 0, N = 0, M = 0, NL = 0;
+    float light_area = 0.0f;
+    float bsphere[4] = {0, 0, 0, 0};
+    ppm_camera camera;
+    size_t device_bytes() const;
+};
+
+// A packed scene in HBM, shared by every job that renders it (jobs differ in camera and
+// parameters).  Its blocks go back to the arena (or are freed) when the last job lets go of it.
+struct SceneDev {
+    DevScene s;
+    int T = 0, N = 0, M = 0, NL = 0;
+    Arena* arena = nullptr;
+    hipStream_t last_use = nullptr;  // the stream whose passes read the scene
+    std::vector<void*> allocs;
+    size_t bytes = 0;
+    ~SceneDev();
+};
+
 }  // namespace ppm_internal
 
 struct ppm_ctx {
@@ -57,6 +82,7 @@ struct ppm_ctx {
     ppm_params prm;
     int Q = 0, P = 0, H = 0, N = 0, T = 0, M = 0, NL = 0;
     Job job;
+    std::shared_ptr<ppm_internal::SceneDev> scene;
     float4* resolve_lin = nullptr;
     float4* resolve_map = nullptr;
     std::vector<void*> allocs;
@@ -68,29 +94,127 @@ struct ppm_ctx {
     double prof_ms[PPM_KERNEL_KINDS] = {0, 0, 0, 0};
     uint64_t prof_n[PPM_KERNEL_KINDS] = {0, 0, 0, 0};
     uint64_t frame = 0;             // passes queued by ppm_pass (the plain loop's eye cadence)
+    int trace_block = 256;          // launch shape (ppm_set_tuning)
+    bool use_graph = false;         // replay passes as hipGraphs (PPM_FLAG_GRAPH)
     // hipGraphs of one photon-only pass and one eye pass (PPM_FLAG_GRAPH)
     hipGraphExec_t graph_pass[2] = {nullptr, nullptr};
     std::string err;
 };
 
+// The batch (ppm_batch_*): resident jobs advanced by one launch per stage per pass.
+struct ppm_batch {
+    int device = 0;
+    hipStream_t stream = nullptr;   // passes
+    hipStream_t upload = nullptr;   // scene uploads and job initialisation
+    ppm_internal::Arena* arena = nullptr;
+    uint32_t passes_per_graph = 1;  // 0: plain launches
+    int trace_block = 256;          // launch shape shared by every job (ppm_batch_set_tuning)
+    int wave_shift = 3;
+    std::vector<ppm_ctx*> slots;    // job id -> context (null once removed)
+    // device tables of the current job set
+    bool dirty = true;
+    int n_jobs = 0;
+    Job* d_jobs = nullptr;
+    BlockRef* d_map[4] = {nullptr, nullptr, nullptr, nullptr};  // tiles, photon slots, hash clears, rays
+    int n_blocks[4] = {0, 0, 0, 0};
+    hipGraphExec_t graph = nullptr;
+    struct Retired {
+        hipGraphExec_t g;
+        hipEvent_t ev;
+    };
+    std::vector<Retired> retired;
+    std::vector<int32_t> table_ids;  // job id of each entry of d_jobs
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    bool timed = false;
+    std::string err;
+};
+
+
 namespace ppm_internal {
 
+// Adds a job that renders an already resident scene (the server's shared scenes).
+int batch_add_resident(ppm_batch* b, std::shared_ptr<SceneDev> scene, const ppm_camera& cam, const ppm_params* p,
+                       uint32_t passes, int32_t* job);
+// Rebuilds the batch tables if jobs were added or removed.
+int batch_prepare(ppm_batch* b);
+// Queues, after the passes queued so far, a copy of every table job's (frame, passes_left) into
+// host_out[2 i], host_out[2 i + 1] (pinned, >= 2 x jobs entries; dev_tmp the same in HBM) and
+// records `done` after it.  *ids receives the job id of each entry.
+int batch_snapshot(ppm_batch* b, unsigned long long* dev_tmp, unsigned long long* host_out, hipEvent_t done,
+                   std::vector<int32_t>* ids);
+// Copies a finished job's results on the upload stream while passes of other jobs run: linear
+// and tone-mapped radiance are resolved there first.  Every pointer may be null.
+int batch_fetch(ppm_batch* b, int32_t job, float4* radiance, float4* flux_radius, float4* emission_count,
+                uchar4* rgba8, ppm_counters* counters);
+
 int validate(const ppm_scene* s, const ppm_params* p);
-// Creates the context's device state.  stream: where passes will run (null: a stream of its
-// own).  Uploads and initialisation run on `upload` (null: the pass stream) and are complete on
-// return.  passes_left: the device-side pass budget (0xffffffff: unlimited).
-int create_context(ppm_ctx* c, const ppm_scene* s, const ppm_params* p, hipStream_t stream, hipStream_t upload,
-                   Arena* arena, uint32_t passes_left);
+int check_scene_public(const ppm_scene* s);
+int validate_params(const ppm_params* p);
+// Host packing of a scene (checks the link table).  Error text in create_error().
+int pack_scene(const ppm_scene* s, PackedScene* out);
+// Uploads a packed scene; complete when `up` has been synchronised (the caller keeps `ps` alive
+// until then).  last_use: the stream whose passes will read it.
+int upload_scene(const PackedScene& ps, Arena* arena, hipStream_t up, hipStream_t last_use,
+                 std::shared_ptr<SceneDev>* out, std::string* err);
+// Creates a context's pass state for a resident scene and a camera.  stream: where passes will
+// run (null: a stream of its own).  Uploads and initialisation run on `upload` (null: the pass
+// stream) and are complete on return.  passes_left: the device-side pass budget (0xffffffff:
+// unlimited).
+int create_context(ppm_ctx* c, std::shared_ptr<SceneDev> scene, const ppm_camera& cam, const ppm_params* p,
+                   hipStream_t stream, hipStream_t upload, Arena* arena, uint32_t passes_left);
 // Releases the device state.  Arena blocks become reusable once the work queued on the
 // context's stream so far has finished; other blocks are freed at once.
 void release_context(ppm_ctx* c);
-// HBM bytes a context of this scene and these parameters allocates.
-size_t footprint(const ppm_scene* s, const ppm_params* p);
+// HBM bytes of a context's pass state, and of a scene, as the arena allocates them.
+size_t state_footprint(const ppm_params* p);
+size_t scene_footprint(int T, int N, int M, int NL);
 int set_error(ppm_ctx* c, int code, const std::string& msg);
 const std::string& create_error();
 // Blocks of each kind for one job.
-int tile_blocks(const ppm_params* p);
+int tile_blocks(int width, int height, int wave_shift);
+int max_tile_blocks(const ppm_params* p);  // over every tile shape
 int photon_blocks(const ppm_params* p);
 int clear_blocks(const ppm_params* p);
+// Checks a tuning and converts it to (trace block, wave shift); PPM_E_INVAL if out of range.
+int check_tuning(const ppm_tuning* t, int* wave_shift);
+// Sets the launch shape of a context's job (invalidates its captured graphs).
+void set_launch_shape(ppm_ctx* c, int trace_block, int wave_shift);
+// Coordinate search over the launch shapes: for each knob in turn, `trial(t)` applies t, runs
+// passes and returns their milliseconds; the fastest value of each knob is kept.  The winner is
+// applied last and written to *best.
+template <class Trial>
+int autotune(ppm_tuning start, bool batch, Trial trial, ppm_tuning* best) {
+    ppm_tuning cur = start;
+    float ms = 0.0f;
+    int rc = trial(cur, &ms);
+    if (rc) return rc;
+    cur.ms_per_pass = ms;
+    const int blocks[] = {64, 128, 256, 512};
+    const int widths[] = {4, 8, 16, 32, 64};
+    const uint32_t graphs_batch[] = {0, 1, 4};
+    const uint32_t graphs_plain[] = {0, 1};
+    for (int knob = 0; knob < 3; knob++) {
+        const int n = knob == 0 ? 4 : knob == 1 ? 5 : (batch ? 3 : 2);
+        const ppm_tuning base = cur;  // the value this knob starts from is already timed
+        for (int k = 0; k < n; k++) {
+            ppm_tuning t = cur;
+            if (knob == 0) t.trace_block = blocks[k];
+            if (knob == 1) t.tile_wave_width = widths[k];
+            if (knob == 2) t.passes_per_graph = batch ? graphs_batch[k] : graphs_plain[k];
+            if ((knob == 0 && t.trace_block == base.trace_block) ||
+                (knob == 1 && t.tile_wave_width == base.tile_wave_width) ||
+                (knob == 2 && t.passes_per_graph == base.passes_per_graph))
+                continue;
+            if ((rc = trial(t, &ms))) return rc;
+            if (ms < cur.ms_per_pass) {
+                cur = t;
+                cur.ms_per_pass = ms;
+            }
+        }
+    }
+    rc = trial(cur, &ms);
+    if (best) *best = cur;
+    return rc;
+}
 
 }  // namespace ppm_internal

@@ -136,12 +136,17 @@ struct GatherParams {
     int draw;
 };
 
-// Pixels are processed in 16x16 tiles, 8x8 per wave, so that a wave's eye rays and hash cells
-// are neighbours; one 256-thread block per tile.
+// Pixels are processed in tiles of 2x2 waves, each wave a (2^wave_shift) x (64 / 2^wave_shift)
+// rectangle (8x8 by default), so that a wave's eye rays and hash cells are neighbours; one
+// 256-thread block per tile.  The shape changes which pixels share a wave, never a pixel's result.
 struct Tiles {
     int width, height;   // image
-    int tiles_x;         // ceil(width / 16)
+    int tiles_x;         // ceil(width / tile width)
+    int wave_shift;      // log2 of a wave's width in pixels (0..6)
 };
+
+// Launch shapes that leave every result unchanged, chosen by the autotuner.
+constexpr int kTraceBlockMax = 512;
 
 // Everything the kernels need about one job (one scene, camera and state set).  A plain pass loop
 // passes it by value; a batch keeps an array of them in HBM and every launch covers all jobs.
@@ -157,10 +162,10 @@ struct Job {
     HashState h;
     RayQueue rq;
     DevGlobals* g;
-    int tile_blocks;     // 16x16 pixel tiles (k_eye, k_gather)
-    int photon_blocks;   // 256 photon slots per block (k_photon_gen, k_bucket_scatter, k_photon_trace)
+    int tile_blocks;     // pixel tiles (k_eye, k_gather)
+    int photon_blocks;   // 256 photon slots per block (k_photon_gen, k_bucket_scatter)
     int clear_blocks;    // 256 hash slots or ray buckets per block (k_clear_slots)
-    int pad;
+    int trace_blocks;    // rays per block = the launch's block size (k_photon_trace)
 };
 
 // Block map of a batched launch: entry b = (job, block of that job) for grid block b.
@@ -171,7 +176,9 @@ struct BlockRef {
 
 // Kernels of one job: grid = that job's blocks.  The _batch forms take the job array and, for
 // the kernels with several blocks per job, the block map of that kernel kind; k_pass_begin_batch
-// runs one thread per job, k_grid_batch and k_bucket_scan_batch one block per job.
+// runs one thread per job, k_grid_batch and k_bucket_scan_batch one block per job.  k_photon_trace
+// runs with any block size up to kTraceBlockMax (a multiple of 64); the others with 256 (1024
+// for k_grid and k_bucket_scan).
 __global__ void k_pass_begin(Job J);
 __global__ void k_clear_slots(Job J);
 __global__ void k_eye(Job J);
@@ -190,5 +197,6 @@ __global__ void k_bucket_scan_batch(const Job* jobs);
 __global__ void k_bucket_scatter_batch(const Job* jobs, const BlockRef* map);
 __global__ void k_photon_trace_batch(const Job* jobs, const BlockRef* map);
 __global__ void k_gather_batch(const Job* jobs, const BlockRef* map);
+__global__ void k_batch_status(const Job* jobs, int n_jobs, unsigned long long* out);
 __global__ void k_resolve(int count, HitState hs, const DevGlobals* g, float4* linear, float4* mapped);
 __global__ void k_init_state(Job J);

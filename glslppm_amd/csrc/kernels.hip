@@ -192,12 +192,13 @@ PPM_DEV unsigned int wave_sum(unsigned int v) {
 constexpr int kBlock = 256;
 constexpr int kWaves = kBlock / 64;
 
-// Pixel of this thread: block = 16x16 tile, wave = 8x8 quarter of it.
+// Pixel of this thread: block = tile of 2x2 waves, wave = ww x wh pixels (ww = 2^wave_shift).
 PPM_DEV void tile_pixel(const Tiles& tl, int tile, int& i, int& j) {
+    const int sh = tl.wave_shift, ww = 1 << sh, wh = 64 >> sh;
     const int tx = tile % tl.tiles_x, ty = tile / tl.tiles_x;
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    i = tx * 16 + (wave & 1) * 8 + (lane & 7);
-    j = ty * 16 + (wave >> 1) * 8 + (lane >> 3);
+    i = tx * 2 * ww + (wave & 1) * ww + (lane & (ww - 1));
+    j = ty * 2 * wh + (wave >> 1) * wh + (lane >> sh);
 }
 
 }  // namespace
@@ -839,7 +840,7 @@ PPM_DEV void photon_trace(const Job& J, int blk) {
     const DevGlobals* g = J.g;
     if (!g->active) return;
     const unsigned int n_rays = g->queue_len;
-    const unsigned int t = blk * kBlock + threadIdx.x;
+    const unsigned int t = blk * blockDim.x + threadIdx.x;
     if (t >= n_rays) return;
     const DevScene& s = J.s;
     const RayQueue& rq = J.rq;
@@ -977,7 +978,7 @@ PPM_JOB_KERNEL1(k_grid, grid, 1024)
 PPM_JOB_KERNEL(k_photon_gen, photon_gen_blk, kBlock)
 PPM_JOB_KERNEL1(k_bucket_scan, bucket_scan, 1024)
 PPM_JOB_KERNEL(k_bucket_scatter, bucket_scatter, kBlock)
-PPM_JOB_KERNEL(k_photon_trace, photon_trace, kBlock)
+PPM_JOB_KERNEL(k_photon_trace, photon_trace, kTraceBlockMax)
 PPM_JOB_KERNEL(k_gather, gather, kBlock)
 
 __global__ void k_pass_begin(Job J) {
@@ -987,6 +988,15 @@ __global__ void k_pass_begin(Job J) {
 __global__ void k_pass_begin_batch(const Job* __restrict__ jobs, int n_jobs) {
     const int j = blockIdx.x * blockDim.x + threadIdx.x;
     if (j < n_jobs) pass_begin(jobs[j]);
+}
+
+// (frame, passes_left) of every job of a batch, for the host's progress snapshot.
+__global__ void k_batch_status(const Job* __restrict__ jobs, int n_jobs, unsigned long long* out) {
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= n_jobs) return;
+    const DevGlobals* g = jobs[j].g;
+    out[2 * j] = g->frame;
+    out[2 * j + 1] = g->passes_left;
 }
 
 __global__ void k_resolve(int count, HitState hs, const DevGlobals* g, float4* linear, float4* mapped) {

@@ -51,15 +51,6 @@ int dev_alloc(ppm_ctx* c, T** out, size_t count) {
     return PPM_OK;
 }
 
-// Allocates and queues the copy on `st`; the caller keeps `v` alive until `st` is synchronised.
-template <typename T>
-int upload(ppm_ctx* c, T** out, const std::vector<T>& v, hipStream_t st) {
-    int rc = dev_alloc(c, out, v.size());
-    if (rc) return rc;
-    if (!v.empty()) HIP_TRY(c, hipMemcpyAsync(*out, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice, st));
-    return PPM_OK;
-}
-
 inline float as_float(int v) {
     float f;
     std::memcpy(&f, &v, 4);
@@ -273,7 +264,7 @@ void queue_pass(ppm_ctx* c, hipStream_t st, bool eye) {
     hipLaunchKernelGGL(k_photon_gen, dim3(J.photon_blocks), dim3(256), 0, st, J);
     hipLaunchKernelGGL(k_bucket_scan, dim3(1), dim3(1024), 0, st, J);
     hipLaunchKernelGGL(k_bucket_scatter, dim3(J.photon_blocks), dim3(256), 0, st, J);
-    hipLaunchKernelGGL(k_photon_trace, dim3(J.photon_blocks), dim3(256), 0, st, J);
+    hipLaunchKernelGGL(k_photon_trace, dim3(J.trace_blocks), dim3(c->trace_block), 0, st, J);
     prof_end(c, PPM_KERNEL_PHOTON, e2);
     hipEvent_t e3 = prof_begin(c);
     hipLaunchKernelGGL(k_gather, dim3(J.tile_blocks), dim3(256), 0, st, J);
@@ -388,20 +379,58 @@ int validate(const ppm_scene* s, const ppm_params* p) {
     return PPM_OK;
 }
 
-int tile_blocks(const ppm_params* p) { return ((p->image_width + 15) / 16) * ((p->image_height + 15) / 16); }
+int validate_params(const ppm_params* p) { return check_params(p); }
+int check_scene_public(const ppm_scene* s) { return check_scene(s); }
+
+int tile_blocks(int width, int height, int wave_shift) {
+    const int tw = 2 << wave_shift, th = 2 * (64 >> wave_shift);
+    return ((width + tw - 1) / tw) * ((height + th - 1) / th);
+}
+
+int max_tile_blocks(const ppm_params* p) {
+    int m = 0;
+    for (int sh = 0; sh <= 6; sh++) m = std::max(m, tile_blocks(p->image_width, p->image_height, sh));
+    return m;
+}
+
+int check_tuning(const ppm_tuning* t, int* wave_shift) {
+    if (!t) return PPM_E_INVAL;
+    if (t->trace_block < 64 || t->trace_block > kTraceBlockMax || t->trace_block % 64) return PPM_E_INVAL;
+    int sh = -1;
+    for (int k = 0; k <= 6; k++)
+        if (t->tile_wave_width == (1 << k)) sh = k;
+    if (sh < 0) return PPM_E_INVAL;
+    if (t->passes_per_graph > 64) return PPM_E_INVAL;
+    if (wave_shift) *wave_shift = sh;
+    return PPM_OK;
+}
+
+void set_launch_shape(ppm_ctx* c, int trace_block, int wave_shift) {
+    Job& J = c->job;
+    J.tl.wave_shift = wave_shift;
+    J.tl.tiles_x = (J.tl.width + (2 << wave_shift) - 1) / (2 << wave_shift);
+    J.tile_blocks = tile_blocks(J.tl.width, J.tl.height, wave_shift);
+    c->trace_block = trace_block;
+    J.trace_blocks = (c->P + trace_block - 1) / trace_block;
+    for (int k = 0; k < 2; k++)
+        if (c->graph_pass[k]) {
+            // a graph in flight keeps running; destroy it once the stream is idle
+            (void)hipStreamSynchronize(c->stream);
+            (void)hipGraphExecDestroy(c->graph_pass[k]);
+            c->graph_pass[k] = nullptr;
+        }
+}
 int photon_blocks(const ppm_params* p) { return (p->photon_width * p->photon_height + 255) / 256; }
 int clear_blocks(const ppm_params* p) {
     return (std::max(p->hash_width * p->hash_height, kRayBuckets) + 255) / 256;
 }
 
-size_t footprint(const ppm_scene* s, const ppm_params* p) {
+size_t state_footprint(const ppm_params* p) {
     const size_t Q = (size_t)p->image_width * p->image_height, P = (size_t)p->photon_width * p->photon_height;
-    const size_t H = (size_t)p->hash_width * p->hash_height, N = s->num_nodes, T = s->num_triangles;
+    const size_t H = (size_t)p->hash_width * p->hash_height;
     const size_t sizes[] = {
-        6 * (N + 1) * 32, T * 48, T * 96, (size_t)s->num_materials * 32, (size_t)s->num_lights * 4,
-        (size_t)s->num_lights * 4,
         // hit points: pos, refl, nrm, rng, (tau, R), (E, N), tri, (mat, t), rgba8, eye partials
-        Q * 16, Q * 16, Q * 16, Q * 16, Q * 16, Q * 16, Q * 4, Q * 8, Q * 4, (size_t)tile_blocks(p) * 24,
+        Q * 16, Q * 16, Q * 16, Q * 16, Q * 16, Q * 16, Q * 4, Q * 8, Q * 4, (size_t)max_tile_blocks(p) * 24,
         // photons: pos, flux, dir, rng, state, slot, emitted partials
         P * 16, P * 16, P * 16, P * 16, P * 8, P * 4, (size_t)photon_blocks(p) * 4,
         // rays: org, dir, aux, key_rank, perm, buckets
@@ -413,17 +442,113 @@ size_t footprint(const ppm_scene* s, const ppm_params* p) {
     return tot;
 }
 
-int create_context(ppm_ctx* c, const ppm_scene* s, const ppm_params* p, hipStream_t stream, hipStream_t upload_st,
-                   Arena* arena, uint32_t passes_left) {
+size_t scene_footprint(int T, int N, int M, int NL) {
+    const size_t sizes[] = {(size_t)6 * (N + 1) * 32, (size_t)T * 48, (size_t)T * 96, (size_t)M * 32, (size_t)NL * 4,
+                            (size_t)NL * 4};
+    size_t tot = 0;
+    for (size_t b : sizes) tot += Arena::round_size(b ? b : 16);
+    return tot;
+}
+
+size_t PackedScene::device_bytes() const { return scene_footprint(T, N, M, NL); }
+
+SceneDev::~SceneDev() {
+    for (void* p : allocs) {
+        if (arena) arena->release(p, last_use);
+        else (void)hipFree(p);
+    }
+}
+
+int pack_scene(const ppm_scene* s, PackedScene* out) {
+    int rc;
+    if ((rc = pack_nodes(s, out->nodes))) return rc;
+    pack_triangles(s, out->isect, out->shade);
+    out->T = s->num_triangles;
+    out->N = s->num_nodes;
+    out->M = s->num_materials;
+    out->NL = s->num_lights;
+    out->mats.resize((size_t)out->M * 2);
+    for (int m = 0; m < out->M; m++) {
+        const ppm_material& mt = s->materials[m];
+        out->mats[2 * m] = make_float4(mt.color[0], mt.color[1], mt.color[2], mt.eta);
+        out->mats[2 * m + 1] = make_float4((float)mt.brdf, mt.g, 0.0f, 0.0f);
+    }
+    out->light_tri.assign(s->light_triangle, s->light_triangle + out->NL);
+    out->light_cdf.assign(s->light_cdf, s->light_cdf + out->NL);
+    out->light_area = s->light_area;
+    for (int k = 0; k < 4; k++) out->bsphere[k] = s->bsphere[k];
+    out->camera = s->camera;
+    return PPM_OK;
+}
+
+int upload_scene(const PackedScene& ps, Arena* arena, hipStream_t up, hipStream_t last_use,
+                 std::shared_ptr<SceneDev>* out, std::string* err) {
+    auto sd = std::make_shared<SceneDev>();
+    sd->arena = arena;
+    sd->last_use = last_use;
+    sd->T = ps.T;
+    sd->N = ps.N;
+    sd->M = ps.M;
+    sd->NL = ps.NL;
+    auto put = [&](const void* src, size_t bytes, void** dst) -> int {
+        const size_t n = bytes ? bytes : 16;
+        hipError_t e = arena ? arena->alloc(dst, n) : hipMalloc(dst, n);
+        if (e == hipSuccess) {
+            sd->allocs.push_back(*dst);
+            sd->bytes += arena ? Arena::round_size(n) : n;
+            if (bytes) e = hipMemcpyAsync(*dst, src, bytes, hipMemcpyHostToDevice, up);
+        }
+        if (e != hipSuccess) {
+            *err = std::string("scene upload: ") + hipGetErrorString(e);
+            return e == hipErrorOutOfMemory ? PPM_E_OOM : PPM_E_HIP;
+        }
+        return PPM_OK;
+    };
+    void *d_nodes, *d_isect, *d_shade, *d_mats, *d_ltri, *d_lcdf;
+    int rc;
+    if ((rc = put(ps.nodes.data(), ps.nodes.size() * 16, &d_nodes)) ||
+        (rc = put(ps.isect.data(), ps.isect.size() * 16, &d_isect)) ||
+        (rc = put(ps.shade.data(), ps.shade.size() * 16, &d_shade)) ||
+        (rc = put(ps.mats.data(), ps.mats.size() * 16, &d_mats)) ||
+        (rc = put(ps.light_tri.data(), ps.light_tri.size() * 4, &d_ltri)) ||
+        (rc = put(ps.light_cdf.data(), ps.light_cdf.size() * 4, &d_lcdf)))
+        return rc;
+    DevScene& sc = sd->s;
+    std::memset(&sc, 0, sizeof(sc));
+    sc.nodes = (const float4*)d_nodes;
+    sc.tri_isect = (const float4*)d_isect;
+    sc.tri_shade = (const float4*)d_shade;
+    sc.mats = (const float4*)d_mats;
+    sc.light_tri = (const int*)d_ltri;
+    sc.light_cdf = (const float*)d_lcdf;
+    sc.num_nodes = ps.N;
+    sc.num_lights = ps.NL;
+    sc.light_area = ps.light_area;
+    sc.bsphere_x = ps.bsphere[0];
+    sc.bsphere_y = ps.bsphere[1];
+    sc.bsphere_z = ps.bsphere[2];
+    sc.bsphere_r = ps.bsphere[3];
+    // bounding cube of the scene for the photon ray buckets
+    sc.cube_lo_x = ps.bsphere[0] - ps.bsphere[3];
+    sc.cube_lo_y = ps.bsphere[1] - ps.bsphere[3];
+    sc.cube_lo_z = ps.bsphere[2] - ps.bsphere[3];
+    sc.cube_cells = ps.bsphere[3] > 0.0f ? (float)(1 << kRayCellBits) / (2.0f * ps.bsphere[3]) : 0.0f;
+    *out = sd;
+    return PPM_OK;
+}
+
+int create_context(ppm_ctx* c, std::shared_ptr<SceneDev> scene, const ppm_camera& k, const ppm_params* p,
+                   hipStream_t stream, hipStream_t upload_st, Arena* arena, uint32_t passes_left) {
     c->prm = *p;
     c->arena = arena;
+    c->scene = scene;
     c->Q = p->image_width * p->image_height;
     c->P = p->photon_width * p->photon_height;
     c->H = p->hash_width * p->hash_height;
-    c->T = s->num_triangles;
-    c->N = s->num_nodes;
-    c->M = s->num_materials;
-    c->NL = s->num_lights;
+    c->T = scene->T;
+    c->N = scene->N;
+    c->M = scene->M;
+    c->NL = scene->NL;
     uint32_t xs[4];
     std::memcpy(xs, p->xorshift, sizeof(xs));
 
@@ -444,45 +569,9 @@ int create_context(ppm_ctx* c, const ppm_scene* s, const ppm_params* p, hipStrea
     HIP_TRY(c, hipEventCreate(&c->ev1));
 
     int rc;
-    std::vector<float4> nodes, tri_isect, tri_shade, mats((size_t)c->M * 2);
-    if ((rc = pack_nodes(s, nodes))) return set_err(c, rc, g_create_error);
-    pack_triangles(s, tri_isect, tri_shade);
-    for (int m = 0; m < c->M; m++) {
-        const ppm_material& mt = s->materials[m];
-        mats[2 * m] = make_float4(mt.color[0], mt.color[1], mt.color[2], mt.eta);
-        mats[2 * m + 1] = make_float4((float)mt.brdf, mt.g, 0.0f, 0.0f);
-    }
-    std::vector<int> ltri(s->light_triangle, s->light_triangle + c->NL);
-    std::vector<float> lcdf(s->light_cdf, s->light_cdf + c->NL);
     Job& J = c->job;
     std::memset(&J, 0, sizeof(J));
-    float4 *d_nodes, *d_isect, *d_shade, *d_mats;
-    int* d_ltri;
-    float* d_lcdf;
-    if ((rc = upload(c, &d_nodes, nodes, up)) || (rc = upload(c, &d_isect, tri_isect, up)) ||
-        (rc = upload(c, &d_shade, tri_shade, up)) || (rc = upload(c, &d_mats, mats, up)) ||
-        (rc = upload(c, &d_ltri, ltri, up)) || (rc = upload(c, &d_lcdf, lcdf, up)))
-        return rc;
-    DevScene& sc = J.s;
-    sc.nodes = d_nodes;
-    sc.tri_isect = d_isect;
-    sc.tri_shade = d_shade;
-    sc.mats = d_mats;
-    sc.light_tri = d_ltri;
-    sc.light_cdf = d_lcdf;
-    sc.num_nodes = c->N;
-    sc.num_lights = c->NL;
-    sc.light_area = s->light_area;
-    sc.bsphere_x = s->bsphere[0];
-    sc.bsphere_y = s->bsphere[1];
-    sc.bsphere_z = s->bsphere[2];
-    sc.bsphere_r = s->bsphere[3];
-    // bounding cube of the scene for the photon ray buckets
-    sc.cube_lo_x = s->bsphere[0] - s->bsphere[3];
-    sc.cube_lo_y = s->bsphere[1] - s->bsphere[3];
-    sc.cube_lo_z = s->bsphere[2] - s->bsphere[3];
-    sc.cube_cells = s->bsphere[3] > 0.0f ? (float)(1 << kRayCellBits) / (2.0f * s->bsphere[3]) : 0.0f;
-    const ppm_camera& k = s->camera;
+    J.s = scene->s;
     J.cam = DevCamera{k.origin[0], k.origin[1], k.origin[2], k.u[0], k.u[1], k.u[2], k.v[0], k.v[1], k.v[2],
                       k.w[0],      k.w[1],      k.w[2],      k.half_width, k.half_height, k.distance};
     J.ep.width = p->image_width;
@@ -502,10 +591,10 @@ int create_context(ppm_ctx* c, const ppm_scene* s, const ppm_params* p, hipStrea
     J.gp.draw = (p->flags & PPM_FLAG_NO_DRAW) ? 0 : 1;
     J.tl.width = p->image_width;
     J.tl.height = p->image_height;
-    J.tl.tiles_x = (p->image_width + 15) / 16;
-    J.tile_blocks = tile_blocks(p);
     J.photon_blocks = photon_blocks(p);
     J.clear_blocks = clear_blocks(p);
+    set_launch_shape(c, 256, 3);  // 8x8-pixel waves; the tuner may change both
+    c->use_graph = (p->flags & PPM_FLAG_GRAPH) != 0;
 
     const size_t Q = c->Q, P = c->P, H = c->H;
     HitState& hs = J.hs;
@@ -514,7 +603,7 @@ int create_context(ppm_ctx* c, const ppm_scene* s, const ppm_params* p, hipStrea
     if ((rc = dev_alloc(c, &hs.pos, Q)) || (rc = dev_alloc(c, &hs.refl, Q)) || (rc = dev_alloc(c, &hs.nrm, Q)) ||
         (rc = dev_alloc(c, &hs.rng, Q)) || (rc = dev_alloc(c, &hs.tau_r, Q)) || (rc = dev_alloc(c, &hs.e_n, Q)) ||
         (rc = dev_alloc(c, &hs.tri, Q)) || (rc = dev_alloc(c, &hs.mat_t, Q)) || (rc = dev_alloc(c, &hs.rgba8, Q)) ||
-        (rc = dev_alloc(c, &hs.partial, (size_t)J.tile_blocks * 6)) || (rc = dev_alloc(c, &ps.pos, P)) ||
+        (rc = dev_alloc(c, &hs.partial, (size_t)max_tile_blocks(p) * 6)) || (rc = dev_alloc(c, &ps.pos, P)) ||
         (rc = dev_alloc(c, &ps.flux, P)) || (rc = dev_alloc(c, &ps.dir, P)) || (rc = dev_alloc(c, &ps.rng, P)) ||
         (rc = dev_alloc(c, &ps.state, P)) || (rc = dev_alloc(c, &ps.slot, P)) ||
         (rc = dev_alloc(c, &ps.partial, (size_t)J.photon_blocks)) || (rc = dev_alloc(c, &rq.org, P)) ||
@@ -566,6 +655,7 @@ void release_context(ppm_ctx* c) {
     }
     c->allocs.clear();
     c->device_bytes = 0;
+    c->scene.reset();
     if (c->ev0) (void)hipEventDestroy(c->ev0);
     if (c->ev1) (void)hipEventDestroy(c->ev1);
     if (c->stream && c->own_stream) (void)hipStreamDestroy(c->stream);
@@ -604,7 +694,32 @@ int ppm_create(const ppm_scene* scene, const ppm_params* params, ppm_ctx** out) 
     if ((rc = ppm_internal::validate(scene, params))) return rc;
     ppm_ctx* c = new (std::nothrow) ppm_ctx();
     if (!c) return set_err(nullptr, PPM_E_OOM, "out of host memory");
-    rc = ppm_internal::create_context(c, scene, params, nullptr, nullptr, nullptr, 0xffffffffu);
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) {
+        delete c;
+        return set_err(nullptr, PPM_E_NODEV, "no HIP device");
+    }
+    if (params->device >= ndev || hipSetDevice(params->device) != hipSuccess) {
+        delete c;
+        return set_err(nullptr, PPM_E_INVAL, "device ordinal out of range");
+    }
+    ppm_internal::PackedScene ps;
+    if ((rc = ppm_internal::pack_scene(scene, &ps))) {
+        delete c;
+        return rc;
+    }
+    std::shared_ptr<ppm_internal::SceneDev> sd;
+    std::string err;
+    rc = ppm_internal::upload_scene(ps, nullptr, nullptr, nullptr, &sd, &err);
+    if (rc == PPM_OK && hipStreamSynchronize(nullptr) != hipSuccess) {
+        rc = PPM_E_HIP;
+        err = "scene upload failed";
+    }
+    if (rc != PPM_OK) {
+        delete c;
+        return set_err(nullptr, rc, err);
+    }
+    rc = ppm_internal::create_context(c, sd, scene->camera, params, nullptr, nullptr, nullptr, 0xffffffffu);
     if (rc != PPM_OK) {
         g_create_error = c->err.empty() ? g_create_error : c->err;
         ppm_internal::release_context(c);
@@ -621,7 +736,7 @@ int ppm_pass(ppm_ctx* c, uint32_t npasses) {
     HIP_TRY(c, hipSetDevice(c->device));
     if (npasses == 0) return PPM_OK;
     const uint64_t L = (uint64_t)c->prm.max_path_length;
-    const bool graph = (c->prm.flags & PPM_FLAG_GRAPH) && !c->profiling;
+    const bool graph = c->use_graph && !c->profiling;
     if (graph) {
         int rc;
         if ((rc = capture_pass(c, 0)) || (rc = capture_pass(c, 1))) return rc;
@@ -812,6 +927,44 @@ int ppm_profile_read(ppm_ctx* c, double* ms, uint64_t* launches) {
         if (launches) launches[k] = c->prof_n[k];
     }
     return PPM_OK;
+}
+
+int ppm_get_tuning(ppm_ctx* c, ppm_tuning* t) {
+    if (!c || !t) return PPM_E_INVAL;
+    t->trace_block = c->trace_block;
+    t->tile_wave_width = 1 << c->job.tl.wave_shift;
+    t->passes_per_graph = c->use_graph ? 1u : 0u;
+    t->ms_per_pass = 0.0f;
+    return PPM_OK;
+}
+
+int ppm_set_tuning(ppm_ctx* c, const ppm_tuning* t) {
+    if (!c) return PPM_E_INVAL;
+    int sh = 0;
+    if (ppm_internal::check_tuning(t, &sh) || t->passes_per_graph > 1)
+        return set_err(c, PPM_E_INVAL, "tuning out of range");
+    ppm_internal::set_launch_shape(c, t->trace_block, sh);
+    c->use_graph = t->passes_per_graph > 0;
+    return PPM_OK;
+}
+
+int ppm_autotune(ppm_ctx* c, uint32_t passes_per_trial, ppm_tuning* best) {
+    if (!c) return PPM_E_INVAL;
+    if (!c->own_stream) return set_err(c, PPM_E_INVAL, "this context belongs to a batch; use ppm_batch_autotune");
+    if (passes_per_trial == 0) passes_per_trial = 2u * (uint32_t)c->prm.max_path_length;
+    ppm_tuning start;
+    ppm_get_tuning(c, &start);
+    auto trial = [&](const ppm_tuning& t, float* ms) -> int {
+        int rc;
+        if ((rc = ppm_set_tuning(c, &t))) return rc;
+        if ((rc = ppm_pass(c, 1))) return rc;  // capture and first-launch costs stay out of the timing
+        if ((rc = ppm_pass(c, passes_per_trial))) return rc;
+        const float v = ppm_last_pass_ms(c);
+        if (v < 0.0f) return set_err(c, PPM_E_HIP, "trial timing failed");
+        *ms = v / (float)passes_per_trial;
+        return PPM_OK;
+    };
+    return ppm_internal::autotune(start, false, trial, best);
 }
 
 void ppm_destroy(ppm_ctx* c) {

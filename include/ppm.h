@@ -196,6 +196,24 @@ float ppm_last_pass_ms(ppm_ctx* ctx);
 int ppm_profile_enable(ppm_ctx* ctx, int enable);
 int ppm_profile_read(ppm_ctx* ctx, double* ms, uint64_t* launches);
 
+/* Launch shapes.  No counterpart in the reference (GL rasterises one fragment per texel).  Every
+ * shape gives bit-identical results; only the speed differs.  ppm_autotune runs real passes of
+ * the context (they count as passes) under each candidate value of one knob at a time -- photon
+ * walk block size, pixel tile shape, graph replay -- and keeps the fastest; call it before the
+ * passes whose speed matters. */
+typedef struct ppm_tuning {
+    int32_t trace_block;       /* threads per block of the photon walk: 64, 128, ..., 512 */
+    int32_t tile_wave_width;   /* a wave's eye/gather tile is w x 64/w pixels: 1, 2, 4, ..., 64 */
+    uint32_t passes_per_graph; /* plain loop: 0 launches, 1 hipGraph per pass (PPM_FLAG_GRAPH);
+                                  batch: passes per hipGraph, 0 for plain launches */
+    float ms_per_pass;         /* ppm_autotune / ppm_batch_autotune: time of the chosen shape */
+} ppm_tuning;
+
+int ppm_get_tuning(ppm_ctx* ctx, ppm_tuning* t);
+int ppm_set_tuning(ppm_ctx* ctx, const ppm_tuning* t);
+/* passes_per_trial 0: twice max_path_length. */
+int ppm_autotune(ppm_ctx* ctx, uint32_t passes_per_trial, ppm_tuning* chosen);
+
 void ppm_destroy(ppm_ctx* ctx);
 const char* ppm_last_error(const ppm_ctx* ctx);
 
@@ -258,8 +276,92 @@ uint64_t ppm_batch_device_bytes(const ppm_batch* b);
 /* HBM a job of this scene and these parameters allocates in a batch. */
 uint64_t ppm_scene_footprint(const ppm_scene* scene, const ppm_params* params);
 
+/* Launch shapes of the whole batch (ppm_tuning); ppm_batch_autotune times real batched passes,
+ * which count against the jobs' budgets. */
+int ppm_batch_get_tuning(ppm_batch* b, ppm_tuning* t);
+int ppm_batch_set_tuning(ppm_batch* b, const ppm_tuning* t);
+int ppm_batch_autotune(ppm_batch* b, uint32_t passes_per_trial, ppm_tuning* chosen);
+
 void ppm_batch_destroy(ppm_batch* b);
 const char* ppm_batch_last_error(const ppm_batch* b);
+
+/* ---------------- serving loop ----------------
+ *
+ * No counterpart in the reference.  A server owns one batch and a worker thread that runs it:
+ * render jobs (a registered scene, a camera, parameters and a target pass count) arrive from
+ * any thread at any time, wait until device memory and a slot are free, are admitted into the
+ * running batch between steps of passes_per_step batched passes, keep their hit points, radii,
+ * flux and generator state resident in HBM until their target is reached, and leave the batch
+ * when their results have been copied out.  Admission and result copies overlap the passes of the
+ * other jobs.  Every job's results equal those of the same scene, camera and parameters run
+ * alone through ppm_create / ppm_pass for its target pass count, bit for bit.
+ *
+ * Scenes are registered once (packed on the calling thread) and shared by every job that names
+ * them; a scene is uploaded with its first admitted job and stays resident while jobs use it. */
+
+typedef struct ppm_server ppm_server;
+
+typedef struct ppm_server_config {
+    int32_t device;
+    uint64_t hbm_budget;       /* device bytes for job state and scenes (default 200 GB) */
+    int32_t max_jobs;          /* resident jobs at most (default 256) */
+    uint32_t passes_per_step;  /* batched passes between admission points (default 4) */
+    uint32_t passes_per_graph; /* the batch's hipGraph length (default 1) */
+} ppm_server_config;
+
+#define PPM_JOB_QUEUED 0
+#define PPM_JOB_RUNNING 1
+#define PPM_JOB_DONE 2
+#define PPM_JOB_FAILED 3
+
+typedef struct ppm_job_info {
+    int32_t state;             /* PPM_JOB_* */
+    uint32_t target_passes;
+    uint64_t passes_done;      /* as of the last step the server observed */
+    uint64_t footprint;        /* device bytes of the job's state */
+    double t_submit;           /* seconds on the server clock (ppm_server_now) */
+    double t_admit;            /* -1 until admitted */
+    double t_first_pass;       /* first observation of a finished first pass; -1 before */
+    double t_done;             /* results copied out; -1 before */
+} ppm_job_info;
+
+typedef struct ppm_server_stats {
+    int32_t queued, resident, done, failed;
+    uint64_t device_bytes;     /* job state and resident scenes */
+    uint64_t steps;            /* batched steps run */
+    uint64_t photon_segments;  /* of finished jobs: P x target passes */
+    double busy_seconds;       /* server clock time with at least one job resident */
+} ppm_server_stats;
+
+void ppm_server_config_default(ppm_server_config* cfg);
+int ppm_server_create(const ppm_server_config* cfg, ppm_server** out);
+
+/* Registers a scene (validated and packed on the calling thread); returns its id. */
+int ppm_server_add_scene(ppm_server* s, const ppm_scene* scene, int64_t* scene_id);
+/* Forgets a scene once no queued or running job uses it. */
+int ppm_server_remove_scene(ppm_server* s, int64_t scene_id);
+
+/* Queues a job.  camera may be NULL (the scene's own).  Returns at once with the job id. */
+int ppm_server_submit(ppm_server* s, int64_t scene_id, const ppm_camera* camera, const ppm_params* params,
+                      uint32_t target_passes, int64_t* job);
+
+/* Waits until the job is done or failed, at most timeout_s seconds (< 0: no limit).  Returns
+ * PPM_OK when done, 1 on time-out, the job's error code when it failed. */
+int ppm_server_wait(ppm_server* s, int64_t job, double timeout_s);
+int ppm_server_info(ppm_server* s, int64_t job, ppm_job_info* info);
+
+/* A finished job's results: PPM_BUF_RADIANCE, PPM_BUF_RGBA8, PPM_BUF_FLUX_RADIUS,
+ * PPM_BUF_EMISSION_COUNT or PPM_BUF_COUNTERS; bytes must equal the buffer size. */
+int ppm_server_result(ppm_server* s, int64_t job, ppm_buf which, void* dst, size_t bytes);
+/* Forgets a finished job and its host copies. */
+int ppm_server_release(ppm_server* s, int64_t job);
+
+int ppm_server_stats_get(ppm_server* s, ppm_server_stats* stats);
+double ppm_server_now(const ppm_server* s);
+
+/* Stops the worker after its current step; jobs not done fail with "server stopped". */
+void ppm_server_destroy(ppm_server* s);
+const char* ppm_server_last_error(const ppm_server* s);
 
 /* ---------------- host scene layer (mesh.cpp, camera.cpp, bvh.cpp) ---------------- */
 

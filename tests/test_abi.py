@@ -163,3 +163,52 @@ def test_scene_footprint_counts_every_state_array():
     exact = 144 * Q + 136 * P + 8 * H
     assert exact <= big <= exact * 1.125 + 64 * 65536
     assert small < big
+
+
+def test_tuning_calls_check_their_arguments():
+    lib = g.library()
+    t = g.Tuning()
+    assert lib.ppm_get_tuning(None, ctypes.byref(t)) == -1
+    assert lib.ppm_set_tuning(None, ctypes.byref(t)) == -1
+    assert lib.ppm_autotune(None, 1, ctypes.byref(t)) == -1
+    assert lib.ppm_batch_get_tuning(None, ctypes.byref(t)) == -1
+    assert lib.ppm_batch_set_tuning(None, ctypes.byref(t)) == -1
+    assert lib.ppm_batch_autotune(None, 1, ctypes.byref(t)) == -1
+
+
+def test_null_server_calls_fail_cleanly():
+    lib = g.library()
+    sid = ctypes.c_int64()
+    assert lib.ppm_server_add_scene(None, None, ctypes.byref(sid)) == -1
+    assert lib.ppm_server_remove_scene(None, 1) == -1
+    assert lib.ppm_server_submit(None, 1, None, None, 1, ctypes.byref(sid)) == -1
+    assert lib.ppm_server_wait(None, 1, 0.0) == -1
+    info = g.JobInfo()
+    assert lib.ppm_server_info(None, 1, ctypes.byref(info)) == -1
+    buf = np.zeros(16, np.uint8)
+    assert lib.ppm_server_result(None, 1, 0, buf.ctypes.data, 16) == -1
+    assert lib.ppm_server_release(None, 1) == -1
+    st = g.ServerStats()
+    assert lib.ppm_server_stats_get(None, ctypes.byref(st)) == -1
+    assert lib.ppm_server_now(None) == -1.0
+    lib.ppm_server_destroy(None)
+
+
+def test_server_config_defaults_and_no_device():
+    cfg = g.ServerConfig()
+    g.library().ppm_server_config_default(ctypes.byref(cfg))
+    assert cfg.hbm_budget == 200 << 30 and cfg.max_jobs == 256 and cfg.passes_per_step == 4
+    bad = g.ServerConfig()
+    g.library().ppm_server_config_default(ctypes.byref(bad))
+    bad.passes_per_step = 0
+    h = ctypes.c_void_p()
+    assert g.library().ppm_server_create(ctypes.byref(bad), ctypes.byref(h)) == -1
+    count = ctypes.c_int(0)
+    try:
+        hip = ctypes.CDLL("libamdhip64.so")
+        rc = hip.hipGetDeviceCount(ctypes.byref(count))
+    except OSError:
+        rc = 1
+    if rc == 0 and count.value > 0:
+        pytest.skip("a GPU is present")
+    assert g.library().ppm_server_create(ctypes.byref(cfg), ctypes.byref(h)) == -5

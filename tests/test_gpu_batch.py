@@ -123,3 +123,47 @@ def test_plain_loop_graph_replay_equals_launches():
     b.run(5)
     rep = compare_all(b, a, STATE_BUFFERS)
     assert not format_report(rep), format_report(rep)
+
+
+@pytest.mark.parametrize("shape", [dict(trace_block=64), dict(trace_block=512), dict(tile_wave_width=1),
+                                   dict(tile_wave_width=4), dict(tile_wave_width=64),
+                                   dict(trace_block=128, tile_wave_width=16, passes_per_graph=1)])
+def test_launch_shapes_do_not_change_results(shape):
+    """Odd image sizes so that partial tiles and partial ray blocks occur under every shape."""
+    print("shape %s" % shape, flush=True)
+    m = spheres(5, 4, 2, 50, 37)
+    p = g.default_params(64, image_width=50, image_height=37, photon_width=40, photon_height=33, max_path_length=6)
+    a = g.Renderer(m, p)
+    b = g.Renderer(m, p)
+    b.set_tuning(**shape)
+    a.run(13)
+    b.run(13)
+    rep = compare_all(b, a, STATE_BUFFERS)
+    assert not format_report(rep), format_report(rep)
+
+
+def test_autotune_keeps_results_and_picks_a_valid_shape():
+    m = spheres(9, 6, 2, 64, 48)
+    p = g.default_params(64, image_height=48, max_path_length=5)
+    r = g.Renderer(m, p)
+    t = r.autotune(passes_per_trial=3)
+    assert t["trace_block"] in (64, 128, 256, 512) and t["tile_wave_width"] in (1, 2, 4, 8, 16, 32, 64)
+    assert t["ms_per_pass"] > 0
+    done = r.counters()["frame_count"]
+    # 10 trials (the start, 3 + 4 + 1 alternatives, the winner), each 1 + 3 passes
+    assert done == 10 * 4
+    a = g.Renderer(m, p)
+    a.run(done)
+    rep = compare_all(r, a, STATE_BUFFERS)
+    assert not format_report(rep), format_report(rep)
+
+    bt = g.Batch()
+    specs = job_specs()[:3]
+    ids = [bt.add(mm, pp, g.PASSES_UNLIMITED) for (_, mm, pp, _) in specs]
+    tb = bt.autotune(passes_per_trial=2)
+    assert tb["ms_per_pass"] > 0
+    bt.run(3)
+    n = bt.progress(ids[0])[0]
+    for (label, mm, pp, _), j in zip(specs, ids):
+        check_equal_alone(JobView(bt, j), mm, pp, n, label)
+    bt.close()
