@@ -36,6 +36,8 @@ HBM_PEAK_GBS = 8000.0
 #     16 B, triangle 4 B, (material, t) 8 B, (E, N) 16 B = 124 B
 #   slot clear (k_clear_slots): 8 B per slot
 BYTES_PER = {"photon": 156.0, "gather": 108.0, "eye": 124.0, "clear": 8.0}
+KERNEL_NAMES = {"photon": "photon stage (k_photon_gen + k_bucket_scan + k_bucket_scatter + k_photon_trace)",
+                "gather": "k_gather", "eye": "eye stage (k_eye + k_grid)", "clear": "k_clear_slots"}
 
 WORKLOADS = {
     "spheres-1024": dict(kind=1, seed=0x67736C70, count=40, detail=4, res=1024, L=10, camera=((0, 0, 1.667), (0, 0, 0), 45.0),
@@ -415,7 +417,7 @@ def main():
         "m_paths_per_s": (emitted + Q * eye_passes) / wall / (1024.0 * 1024.0),
         "kernel_ms_per_launch": {k: (v[0] / v[1] if v[1] else None) for k, v in prof.items()},
         "tuning": tuning if tuning is not None else r.tuning(),
-        "roofline": {"bound": "hbm", "kernel": "k_" + dom, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+        "roofline": {"bound": "hbm", "kernel": KERNEL_NAMES[dom], "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
                      "bytes_per_launch": bytes_per_launch, "avg_launch_ms": avg_ms},
         "cpu_baseline": cpu,
