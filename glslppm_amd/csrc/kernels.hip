@@ -37,62 +37,115 @@ struct Hit {
     float bx, by, bz;  // Barycentric = uvt.ywx
 };
 
-// Box test of node i and, at a leaf, the triangle test (eyeraytrace.fs:92-147,
-// photontrace.fs:148-205); returns the next position: i + 1 on a box hit or after any leaf,
-// else the miss link.  Eye rays cull back faces on their first segment; photon rays skip light
-// triangles.
-template <bool PHOTON>
-PPM_DEV int visit(const DevScene& s, f3 org, f3 dir, f3 rcp, f3 t0, bool cull, float4 a, float4 b, int i, Hit& h) {
+// Slab test of one node's box (eyeraytrace.fs:92-116, photontrace.fs:148-172) against the
+// closest hit so far; tmin is returned for the deferred re-check of leaves.
+PPM_DEV bool box_test(f3 rcp, f3 t0, float4 a, float4 b, float ht, float& tmin_out) {
     const float lx = rcp.x * a.x + t0.x, ly = rcp.y * a.y + t0.y, lz = rcp.z * a.z + t0.z;
     const float hx = rcp.x * b.x + t0.x, hy = rcp.y * b.y + t0.y, hz = rcp.z * b.z + t0.z;
     const float tmin = max1(max1(min1(lx, hx), min1(ly, hy)), min1(lz, hz));
     const float tmax = min1(min1(max1(lx, hx), max1(ly, hy)), max1(lz, hz));
-    const bool box_hit = (tmin <= tmax) && (tmin <= h.t) && (tmax >= 0.0f);
-    const int tri = __float_as_int(b.w);
-    if (tri >= 0 && box_hit) {
-        const float4 q0 = s.tri_isect[3 * tri + 0];
-        const float4 q1 = s.tri_isect[3 * tri + 1];
-        const float4 q2 = s.tri_isect[3 * tri + 2];
-        const f3 e0 = xyz(q1), e1 = xyz(q2);
-        const f3 pv = cross3(dir, e1);
-        const float det = dot3(e0, pv);
-        if (!cull || det > 1e-10f) {
-            const f3 tv = org - xyz(q0);
-            const f3 qv = cross3(tv, e0);
-            const float u = dot3(tv, pv) / det;
-            const float v = dot3(dir, qv) / det;
-            const float t = dot3(e1, qv) / det;
-            const float w = (1.0f - u) - v;
-            bool ok = (u >= 0.0f) && (v >= 0.0f) && (t >= 0.0f) && (w >= 0.0f) && (t < h.t);
-            if (PHOTON) ok = ok && (__float_as_int(q0.w) != -1);
-            if (ok) {
-                h.tri = tri;
-                h.t = t;
-                h.bx = v;
-                h.by = w;
-                h.bz = u;
-            }
-        }
-    }
-    return box_hit ? i + 1 : __float_as_int(a.w);
+    tmin_out = tmin;
+    return (tmin <= tmax) && (tmin <= ht) && (tmax >= 0.0f);
 }
 
+// Triangle test of a leaf (eyeraytrace.fs:118-147, photontrace.fs:174-205).  Eye rays cull back
+// faces on their first segment; photon rays skip light triangles.
+template <bool PHOTON>
+PPM_DEV void tri_test(const DevScene& s, f3 org, f3 dir, bool cull, int tri, Hit& h) {
+    const float4 q0 = s.tri_isect[3 * tri + 0];
+    const float4 q1 = s.tri_isect[3 * tri + 1];
+    const float4 q2 = s.tri_isect[3 * tri + 2];
+    const f3 e0 = xyz(q1), e1 = xyz(q2);
+    const f3 pv = cross3(dir, e1);
+    const float det = dot3(e0, pv);
+    if (!cull || det > 1e-10f) {
+        const f3 tv = org - xyz(q0);
+        const f3 qv = cross3(tv, e0);
+        const float u = dot3(tv, pv) / det;
+        const float v = dot3(dir, qv) / det;
+        const float t = dot3(e1, qv) / det;
+        const float w = (1.0f - u) - v;
+        bool ok = (u >= 0.0f) && (v >= 0.0f) && (t >= 0.0f) && (w >= 0.0f) && (t < h.t);
+        if (PHOTON) ok = ok && (__float_as_int(q0.w) != -1);
+        if (ok) {
+            h.tri = tri;
+            h.t = t;
+            h.bx = v;
+            h.by = w;
+            h.bz = u;
+        }
+    }
+}
+
+// Leaves a lane may hold before its triangle tests run.
+constexpr int kLeafQueue = 2;
+
 // Closest hit over the threaded BVH of one cube face (eyeraytrace.fs:67-254,
-// photontrace.fs:123-218).  Nodes i and i + 1 are fetched together; when the walk continues at
-// i + 1 (55% of steps on the sphere scene) that node is visited without waiting for another
-// load.  The order of visits, and so every result including ties, is the reference's.  Each
-// face array ends with a padding node, so the second fetch stays in bounds.
+// photontrace.fs:123-218): box tests walk the pre-order (hit: next position, miss: the stored
+// link); a leaf whose box is hit gets its triangle tested.
+//
+// The walk runs in two alternating phases so that a wave's triangle tests (three correctly
+// rounded divisions each) run with many lanes at once instead of whenever any lane meets a leaf.
+// In the box phase a lane that meets a leaf with a hit box queues it, with the box's entry
+// distance tmin, and walks on; the wave leaves the box phase when no lane can step or when more
+// than a quarter of its live lanes hold full queues.  The leaf phase then tests each lane's
+// queued leaves in the order they were met, skipping a leaf whose tmin now exceeds the closest
+// hit.  Boxes walked while a leaf was queued were tested against an older, larger closest-hit
+// distance, so the lane may enter boxes the reference skips; it never skips one the reference
+// enters.  A leaf inside a box the reference skips has tmin >= that box's tmin (nested boxes,
+// slab test monotone in the bounds, including the NaN axes of zero direction components), so the
+// re-check drops it; every other leaf is tested with the closest hit the reference has at that
+// point.  The triangle tests, their order and so every result, ties included, are the
+// reference's.  Nodes i and i + 1 are fetched together (55% of steps continue at i + 1); each face
+// array ends with a padding node, so the second fetch stays in bounds.
 template <bool PHOTON>
 PPM_DEV void walk(const DevScene& s, int face, f3 org, f3 dir, f3 rcp, f3 t0, bool cull, Hit& h) {
     const int n = s.num_nodes;
     const float4* __restrict__ nd = s.nodes + (size_t)face * (size_t)(n + 1) * 2;
     int i = 0;
-    while (i < n) {
-        const float4 a0 = nd[2 * i], b0 = nd[2 * i + 1];
-        const float4 a1 = nd[2 * i + 2], b1 = nd[2 * i + 3];
-        int next = visit<PHOTON>(s, org, dir, rcp, t0, cull, a0, b0, i, h);
-        if (next == i + 1 && next < n) next = visit<PHOTON>(s, org, dir, rcp, t0, cull, a1, b1, next, h);
-        i = next;
+    int q0 = -1, q1 = -1;
+    float m0 = 0.0f, m1 = 0.0f;
+    int nq = 0;
+    for (;;) {
+        for (;;) {
+            const bool can = i < n && nq < kLeafQueue;
+            if (__ballot(can) == 0ull) break;
+            const int live = __popcll(__ballot(i < n || nq > 0));
+            const int full = __popcll(__ballot(nq == kLeafQueue));
+            if (4 * full > live) break;
+            if (can) {
+                const float4 a0 = nd[2 * i], b0 = nd[2 * i + 1];
+                const float4 a1 = nd[2 * i + 2], b1 = nd[2 * i + 3];
+                float tm;
+                bool hit = box_test(rcp, t0, a0, b0, h.t, tm);
+                int tri = __float_as_int(b0.w);
+                int next = hit ? i + 1 : __float_as_int(a0.w);
+                if (hit && tri >= 0) {
+                    q1 = nq == 1 ? tri : q1;
+                    m1 = nq == 1 ? tm : m1;
+                    q0 = nq == 0 ? tri : q0;
+                    m0 = nq == 0 ? tm : m0;
+                    nq++;
+                }
+                if (next == i + 1 && next < n && nq < kLeafQueue) {
+                    hit = box_test(rcp, t0, a1, b1, h.t, tm);
+                    tri = __float_as_int(b1.w);
+                    next = hit ? i + 2 : __float_as_int(a1.w);
+                    if (hit && tri >= 0) {
+                        q1 = nq == 1 ? tri : q1;
+                        m1 = nq == 1 ? tm : m1;
+                        q0 = nq == 0 ? tri : q0;
+                        m0 = nq == 0 ? tm : m0;
+                        nq++;
+                    }
+                }
+                i = next;
+            }
+        }
+        if (__ballot(nq > 0) == 0ull) break;
+        if (nq > 0 && m0 <= h.t) tri_test<PHOTON>(s, org, dir, cull, q0, h);
+        if (nq > 1 && m1 <= h.t) tri_test<PHOTON>(s, org, dir, cull, q1, h);
+        nq = 0;
     }
 }
 
