@@ -45,6 +45,10 @@ WORKLOADS = {
     "cornell-256": dict(kind=0, seed=0, count=0, detail=0, res=256, L=8, camera=((0, 0, 1.667), (0, 0, 0), 45.0),
                         fixed_radius=0.02,
                         desc="Cornell room, 36 triangles: Q=P=H=256^2, 8 bounces, radius 1/50 of the room"),
+    "terrain-c2": dict(kind=2, seed=0x67736C72, count=8, detail=1024, res=2048, L=12,
+                       camera=((0.0, 0.55, 1.9), (0.0, -0.25, 0.0), 50.0), image=(1920, 1080), hash=(4096, 4096),
+                       desc="1024^2-cell value-noise heightfield + 8 glass icospheres (subdiv 5), 2,260,994 "
+                            "triangles, small area light: Q=1920x1080, P=2048^2, H=2^24, 12 bounces"),
 }
 
 
@@ -275,7 +279,8 @@ def build_scene(w):
     import glslppm_amd as g
     m = g.Mesh.generate(w["kind"], seed=w["seed"], count=w["count"], detail=w["detail"])
     (org, at, fov) = w["camera"]
-    m.set_camera(org, at, w["res"], w["res"], fov)
+    iw, ih = w.get("image", (w["res"], w["res"]))
+    m.set_camera(org, at, iw, ih, fov)
     t = time.time()
     m.build()
     return m, time.time() - t
@@ -286,6 +291,10 @@ def params_for(w):
     kw = dict(max_path_length=w["L"])
     if w.get("fixed_radius"):
         kw["fixed_radius"] = w["fixed_radius"]
+    if "image" in w:
+        kw["image_width"], kw["image_height"] = w["image"]
+    if "hash" in w:
+        kw["hash_width"], kw["hash_height"] = w["hash"]
     return g.default_params(w["res"], **kw)
 
 

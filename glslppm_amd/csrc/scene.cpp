@@ -12,12 +12,16 @@
 #include "../../include/ppm.h"
 
 #include <algorithm>
+#include <atomic>
 #include <cmath>
+#include <condition_variable>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
 #include <map>
+#include <mutex>
 #include <string>
+#include <thread>
 #include <vector>
 
 namespace {
@@ -134,123 +138,285 @@ void sort_by_key(int* idx, const float* key, int lo, int hi, std::vector<std::pa
     }
 }
 
+// sort_by_key for large ranges: after each partition step the two ranges are disjoint and sorted
+// independently, so the right one goes to another thread; the permutation is the same.
+void sort_by_key_par(int* idx, const float* key, int lo, int hi, int depth) {
+    if (depth <= 0 || hi - lo < 131072) {
+        std::vector<std::pair<int, int>> st;
+        sort_by_key(idx, key, lo, hi, st);
+        return;
+    }
+    int i = lo, j = hi;
+    const float pivot = key[idx[(lo + hi) / 2]];
+    for (;;) {
+        while (key[idx[i]] < pivot) i++;
+        while (key[idx[j]] > pivot) j--;
+        if (i >= j) break;
+        int t = idx[i];
+        idx[i] = idx[j];
+        idx[j] = t;
+        i++;
+        j--;
+    }
+    std::thread right;
+    if ((j + 1) < hi) right = std::thread(sort_by_key_par, idx, key, j + 1, hi, depth - 1);
+    if (lo < (i - 1)) sort_by_key_par(idx, key, lo, i - 1, depth - 1);
+    if (right.joinable()) right.join();
+}
+
 struct BuildTask {
     int lo, n;       // range into the index pool
     Box box;         // box handed down by the parent's sweep
     int parent;      // -1 for the root
-    int side;        // 0 = first child, 1 = second child
+    int side;        // 0 = first child in the canonical order, 1 = second
 };
 
+// Per-thread buffers of the SAH sweep; sort_depth > 0 lets the node's sorts use extra threads.
+struct Scratch {
+    int sort_depth = 0;
+    std::vector<int> sorted;
+    std::vector<float> left_area;
+    std::vector<Box> lbox;
+    std::vector<std::pair<int, int>> sstack;
+    void fit(int n) {
+        if ((int)sorted.size() < n) {
+            sorted.resize(n);
+            left_area.resize(n);
+            lbox.resize(n);
+        }
+    }
+};
+
+struct BuildInput {
+    std::vector<Box> tbox;
+    std::vector<float> cen[3];
+    std::vector<int> pool;
+};
+
+// One node of splitBVH (bvh.cpp:68-210): an exact SAH sweep over the three axes, each axis sorting
+// the range as the previous axis left it (the order of equal keys matters), then the range
+// reordered as the best axis sorted it.  Returns false for a leaf; otherwise the two children in
+// the canonical order (the side with the smaller min.x first, bvh.cpp:195).
+bool split_node(BuildInput& in, Scratch& sc, const BuildTask& tk, int id, BuildTask& first, BuildTask& second) {
+    if (tk.n <= 1) return false;
+    int* idx = &in.pool[tk.lo];
+    const int n = tk.n;
+    int best_index = 0;
+    Box best_l, best_r;
+    best_l.lo = best_l.hi = mk(0.0f, 0.0f, 0.0f);
+    best_r = best_l;
+    int nl, nr;
+    if (n == 2) {
+        best_l = in.tbox[idx[0]];
+        best_r = in.tbox[idx[1]];
+        nl = 1;
+        nr = 1;
+    } else {
+        sc.fit(n);
+        int best_axis = 0;
+        float best_cost = 1e+30f;
+        for (int k = 0; k <= 2; k++) {
+            if (sc.sort_depth > 0 && n >= 262144) sort_by_key_par(idx, in.cen[k].data(), 0, n - 1, sc.sort_depth);
+            else sort_by_key(idx, in.cen[k].data(), 0, n - 1, sc.sstack);
+            Box bl;
+            bl.reset();
+            for (int i = 0; i < n; i++) {
+                bl.grow(in.tbox[idx[i]].lo);
+                bl.grow(in.tbox[idx[i]].hi);
+                sc.left_area[i] = bl.area();
+                sc.lbox[i] = bl;
+            }
+            int tri_num = n - 1;
+            Box br;
+            br.reset();
+            for (int j = n - 2; j >= 0; j--) {
+                const int ii = idx[j + 1];
+                br.grow(in.tbox[ii].lo);
+                br.grow(in.tbox[ii].hi);
+                const float cost = (float)tri_num * sc.left_area[j] + (float)(n - tri_num) * br.area();
+                if (cost < best_cost) {
+                    best_cost = cost;
+                    best_axis = k;
+                    best_index = j;
+                    best_l = sc.lbox[best_index];
+                    best_r = br;
+                }
+                tri_num--;
+            }
+            if (best_axis == k) std::memcpy(sc.sorted.data(), idx, sizeof(int) * n);
+        }
+        std::memcpy(idx, sc.sorted.data(), sizeof(int) * n);
+        nl = best_index + 1;
+        nr = n - nl;
+    }
+    BuildTask lt = {tk.lo, nl, best_l, id, 0};
+    BuildTask rt = {tk.lo + nl, nr, best_r, id, 0};
+    if (best_l.lo.x < best_r.lo.x) {
+        first = lt;
+        second = rt;
+    } else {
+        first = rt;
+        second = lt;
+    }
+    first.side = 0;
+    second.side = 1;
+    return true;
+}
+
+// A node as the parallel build records it: ids are handed out in completion order and turned
+// into the canonical pre-order at the end.
+struct RawNode {
+    Box box;
+    int tri;            // leaf triangle, -1 for an internal node
+    int child[2];       // canonical first and second child
+};
+
+int build_threads() {
+    const char* e = std::getenv("PPM_BUILD_THREADS");
+    int n = e ? std::atoi(e) : (int)std::thread::hardware_concurrency();
+    return std::max(1, std::min(n, 64));
+}
+
+// Builds the subtree of `root` depth first on the calling thread (ranges below the cutoff).
+void build_serial(BuildInput& in, Scratch& sc, const BuildTask& root, std::vector<RawNode>& raw,
+                  std::atomic<int>& next_id) {
+    std::vector<BuildTask> stack;
+    stack.push_back(root);
+    while (!stack.empty()) {
+        BuildTask tk = stack.back();
+        stack.pop_back();
+        const int id = next_id.fetch_add(1);
+        RawNode& r = raw[id];
+        r.box = tk.box;
+        r.child[0] = r.child[1] = -1;
+        if (tk.parent >= 0) raw[tk.parent].child[tk.side] = id;
+        BuildTask a, b;
+        if (!split_node(in, sc, tk, id, a, b)) {
+            r.tri = tk.n ? in.pool[tk.lo] : -1;
+            continue;
+        }
+        r.tri = -1;
+        stack.push_back(b);
+        stack.push_back(a);
+    }
+}
+
+// The canonical tree (face 0 pre-order ids, left = first child) of splitBVH.  Every node's split
+// depends only on its own range of the index pool, as its parent left it, and on its box, so
+// disjoint subtrees are built on separate threads: large ranges are split by a pool of workers,
+// ranges below a cutoff are finished depth first by one worker.  The result does not depend on
+// the number of threads.
 void build_canonical(const std::vector<Tri>& tris, const Box& root_box, std::vector<Node>& nodes) {
     const int T = (int)tris.size();
-    std::vector<Box> tbox(T);
-    std::vector<float> cen[3];
-    for (int k = 0; k < 3; k++) cen[k].resize(T);
+    BuildInput in;
+    in.tbox.resize(T);
+    for (int k = 0; k < 3; k++) in.cen[k].resize(T);
     for (int i = 0; i < T; i++) {
         Box b;
         b.reset();
         b.grow(tris[i].p[0]);
         b.grow(tris[i].p[1]);
         b.grow(tris[i].p[2]);
-        tbox[i] = b;
+        in.tbox[i] = b;
         V3 c = muls(add(add(tris[i].p[0], tris[i].p[1]), tris[i].p[2]), 1.0f / 3.0f);
-        cen[0][i] = c.x;
-        cen[1][i] = c.y;
-        cen[2][i] = c.z;
+        in.cen[0][i] = c.x;
+        in.cen[1][i] = c.y;
+        in.cen[2][i] = c.z;
     }
-    std::vector<int> pool(T);
-    for (int i = 0; i < T; i++) pool[i] = i;
-    std::vector<int> sorted(T > 0 ? T : 1);
-    std::vector<float> left_area(T > 0 ? T : 1);
-    std::vector<Box> lbox(T > 0 ? T : 1);
-    std::vector<std::pair<int, int>> sstack;
-    nodes.clear();
-    nodes.reserve(T > 0 ? 2 * T : 1);
+    in.pool.resize(T);
+    for (int i = 0; i < T; i++) in.pool[i] = i;
+    const int total = T > 0 ? 2 * T - 1 : 1;
+    std::vector<RawNode> raw(total);
+    std::atomic<int> next_id(0);
 
-    std::vector<BuildTask> stack;
-    stack.push_back({0, T, root_box, -1, 0});
-    while (!stack.empty()) {
-        BuildTask tk = stack.back();
-        stack.pop_back();
-        const int id = (int)nodes.size();
-        nodes.push_back(Node());
-        Node& nd = nodes.back();
-        nd.box = tk.box;
-        nd.left = nd.right = -1;
-        if (tk.parent >= 0) {
-            if (tk.side == 0) nodes[tk.parent].left = id;
-            else nodes[tk.parent].right = id;
-        }
-        if (tk.n <= 1) {
-            nodes[id].leaf = 1;
-            nodes[id].tri = tk.n ? pool[tk.lo] : -1;
-            continue;
-        }
-        nodes[id].leaf = 0;
-        nodes[id].tri = -1;
-        int* idx = &pool[tk.lo];
-        const int n = tk.n;
-        int best_index = 0;
-        Box best_l, best_r;
-        best_l.lo = best_l.hi = mk(0.0f, 0.0f, 0.0f);
-        best_r = best_l;
-        int nl, nr;
-        if (n == 2) {
-            best_l = tbox[idx[0]];
-            best_r = tbox[idx[1]];
-            nl = 1;
-            nr = 1;
-        } else {
-            int best_axis = 0;
-            float best_cost = 1e+30f;
-            for (int k = 0; k <= 2; k++) {
-                sort_by_key(idx, cen[k].data(), 0, n - 1, sstack);
-                Box bl;
-                bl.reset();
-                for (int i = 0; i < n; i++) {
-                    bl.grow(tbox[idx[i]].lo);
-                    bl.grow(tbox[idx[i]].hi);
-                    left_area[i] = bl.area();
-                    lbox[i] = bl;
+    const int threads = build_threads();
+    const int cutoff = std::max(1024, T / (8 * threads));
+    if (threads == 1 || T <= cutoff) {
+        Scratch sc;
+        build_serial(in, sc, BuildTask{0, T, root_box, -1, 0}, raw, next_id);
+    } else {
+        std::mutex mu;
+        std::condition_variable cv;
+        std::vector<BuildTask> queue;
+        queue.push_back(BuildTask{0, T, root_box, -1, 0});
+        int busy = 0;
+        int depth = 0;
+        while ((2 << depth) <= threads) depth++;
+        auto worker = [&]() {
+            Scratch sc;
+            sc.sort_depth = depth;
+            for (;;) {
+                BuildTask tk;
+                {
+                    std::unique_lock<std::mutex> lk(mu);
+                    cv.wait(lk, [&] { return !queue.empty() || busy == 0; });
+                    if (queue.empty()) return;
+                    // the largest waiting range first keeps the critical path short
+                    auto it = std::max_element(queue.begin(), queue.end(),
+                                               [](const BuildTask& x, const BuildTask& y) { return x.n < y.n; });
+                    tk = *it;
+                    queue.erase(it);
+                    busy++;
                 }
-                int tri_num = n - 1;
-                Box br;
-                br.reset();
-                for (int j = n - 2; j >= 0; j--) {
-                    const int ii = idx[j + 1];
-                    br.grow(tbox[ii].lo);
-                    br.grow(tbox[ii].hi);
-                    const float cost = (float)tri_num * left_area[j] + (float)(n - tri_num) * br.area();
-                    if (cost < best_cost) {
-                        best_cost = cost;
-                        best_axis = k;
-                        best_index = j;
-                        best_l = lbox[best_index];
-                        best_r = br;
+                if (tk.n <= cutoff) {
+                    build_serial(in, sc, tk, raw, next_id);
+                } else {
+                    const int id = next_id.fetch_add(1);
+                    RawNode& r = raw[id];
+                    r.box = tk.box;
+                    r.tri = -1;
+                    r.child[0] = r.child[1] = -1;
+                    BuildTask a, b;
+                    const bool inner = split_node(in, sc, tk, id, a, b);
+                    {
+                        std::lock_guard<std::mutex> lk(mu);
+                        if (tk.parent >= 0) raw[tk.parent].child[tk.side] = id;
+                        if (inner) {
+                            queue.push_back(a);
+                            queue.push_back(b);
+                        } else {
+                            r.tri = tk.n ? in.pool[tk.lo] : -1;
+                        }
                     }
-                    tri_num--;
                 }
-                if (best_axis == k) std::memcpy(sorted.data(), idx, sizeof(int) * n);
+                {
+                    std::lock_guard<std::mutex> lk(mu);
+                    busy--;
+                }
+                cv.notify_all();
             }
-            std::memcpy(idx, sorted.data(), sizeof(int) * n);
-            nl = best_index + 1;
-            nr = n - nl;
+        };
+        std::vector<std::thread> pool;
+        for (int t = 0; t < threads; t++) pool.emplace_back(worker);
+        for (auto& th : pool) th.join();
+    }
+
+    // canonical pre-order: node, first child's subtree, second child's subtree
+    const int N = next_id.load();
+    nodes.assign(N, Node());
+    std::vector<int> order;
+    order.reserve(N);
+    std::vector<int> st;
+    if (N > 0) st.push_back(0);
+    std::vector<int> canon(N, -1);
+    while (!st.empty()) {
+        const int k = st.back();
+        st.pop_back();
+        canon[k] = (int)order.size();
+        order.push_back(k);
+        if (raw[k].child[0] >= 0) {
+            st.push_back(raw[k].child[1]);
+            st.push_back(raw[k].child[0]);
         }
-        BuildTask lt = {tk.lo, nl, best_l, id, 0};
-        BuildTask rt = {tk.lo + nl, nr, best_r, id, 0};
-        // canonical condition: the side with the smaller min.x is visited first (bvh.cpp:195)
-        BuildTask first, second;
-        if (best_l.lo.x < best_r.lo.x) {
-            first = lt;
-            second = rt;
-        } else {
-            first = rt;
-            second = lt;
-        }
-        first.side = 0;
-        second.side = 1;
-        stack.push_back(second);
-        stack.push_back(first);
+    }
+    for (int c = 0; c < N; c++) {
+        const RawNode& r = raw[order[c]];
+        Node& nd = nodes[c];
+        nd.box = r.box;
+        nd.leaf = r.child[0] < 0 ? 1 : 0;
+        nd.tri = nd.leaf ? r.tri : -1;
+        nd.left = nd.leaf ? -1 : canon[r.child[0]];
+        nd.right = nd.leaf ? -1 : canon[r.child[1]];
     }
 }
 
@@ -273,10 +439,10 @@ void thread_links(const std::vector<Node>& nodes, std::vector<int32_t>& links) {
     std::vector<int> size(N, 1);
     for (int k = N - 1; k >= 0; k--)
         if (!nodes[k].leaf) size[k] = 1 + size[nodes[k].left] + size[nodes[k].right];
-    std::vector<int> order(N), pos(N), st;
-    for (int f = 0; f < 6; f++) {
+    // the six orders are independent: one thread each for large trees
+    auto face_links = [&](int f) {
+        std::vector<int> order(N), pos(N), st;
         int c = 0;
-        st.clear();
         if (N > 0) st.push_back(0);
         while (!st.empty()) {
             int k = st.back();
@@ -298,6 +464,13 @@ void thread_links(const std::vector<Node>& nodes, std::vector<int32_t>& links) {
             links[((size_t)f * N + k) * 2 + 0] = hit;
             links[((size_t)f * N + k) * 2 + 1] = miss;
         }
+    };
+    if (N < 65536 || build_threads() == 1) {
+        for (int f = 0; f < 6; f++) face_links(f);
+    } else {
+        std::vector<std::thread> th;
+        for (int f = 0; f < 6; f++) th.emplace_back(face_links, f);
+        for (auto& t : th) t.join();
     }
 }
 

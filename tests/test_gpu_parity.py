@@ -93,3 +93,51 @@ def test_reference_scene_file_layout_square_512():
     """The reference's own configuration: Q = P = H = 512^2, 10 bounces, footprint radius rule."""
     m = g.Mesh.generate(g.SCENE_CORNELL).set_camera((0, 0, 1.667), (0, 0, 0), 512, 512, 45.0).build()
     _check(m, g.default_params(512), 12, "square-512")
+
+
+def c2_terrain():
+    """C2: 1024x1024-cell value-noise heightfield plus 8 glass icospheres (subdivision 5),
+    2,260,994 triangles, small area light; 1920x1080 eye rays, 2048^2 photons, 2^24 hash slots,
+    12 bounces."""
+    m = g.Mesh.generate(g.SCENE_TERRAIN, seed=0x67736C72, count=8, detail=1024)
+    m.set_camera((0.0, 0.55, 1.9), (0.0, -0.25, 0.0), 1920, 1080, 50.0).build()
+    p = g.default_params(2048, image_width=1920, image_height=1080, hash_width=4096, hash_height=4096,
+                         max_path_length=12)
+    return m, p
+
+
+def test_c2_terrain_full_size():
+    m, p = c2_terrain()
+    assert m.num_triangles == 2260994
+    # bit-exact against the oracle over the first two passes (one eye pass, two photon passes)
+    _check(m, p, 2, "c2-terrain")
+    # size-independent properties over a full eye cycle and two more passes, GPU only
+    r = g.Renderer(m, p)
+    r.run(12)
+    tr12 = r.readback("flux_radius").copy()
+    en12 = r.readback("emission_count").copy()
+    r.run(2)
+    slot = r.readback("photon_slot")
+    count = r.readback("slot_count")
+    rep = r.readback("slot_rep")
+    H = p.hash_width * p.hash_height
+    valid = slot >= 0
+    # correction: per-slot counts are the histogram of the photons' slots
+    assert np.array_equal(np.bincount(slot[valid], minlength=H).astype(np.uint32), count)
+    # scatter: the representative is the smallest eligible photon index of its slot
+    P = p.photon_width * p.photon_height
+    limit = P - p.photon_width // 2
+    idx = np.nonzero(valid & (np.arange(P) < limit))[0]
+    best = np.full(H, -1, np.int64)
+    order = np.argsort(slot[idx], kind="stable")
+    s_sorted, i_sorted = slot[idx][order], idx[order]
+    first = np.r_[True, s_sorted[1:] != s_sorted[:-1]]
+    best[s_sorted[first]] = i_sorted[first]
+    assert np.array_equal(best.astype(np.int32), rep)
+    # progressive update: R never grows, N never shrinks, between passes 12 and 14 (no eye pass)
+    tr14, en14 = r.readback("flux_radius"), r.readback("emission_count")
+    hit = r.readback("hit_triangle")[:, 0] >= 0
+    assert np.all(tr14[hit, 3] <= tr12[hit, 3]) and np.all(en14[hit, 3] >= en12[hit, 3])
+    c = r.counters()
+    assert c["frame_count"] == 14 and c["eye_samples"] == 2 and c["photon_segments"] == 14 * P
+    assert 0 < c["photons_emitted"] <= 14 * P
