@@ -177,6 +177,8 @@ def library():
         "ppm_batch_get_tuning": (ctypes.c_int, [vp, ctypes.POINTER(Tuning)]),
         "ppm_batch_set_tuning": (ctypes.c_int, [vp, ctypes.POINTER(Tuning)]),
         "ppm_batch_autotune": (ctypes.c_int, [vp, ctypes.c_uint32, ctypes.POINTER(Tuning)]),
+        "ppm_write_png": (ctypes.c_int, [ctypes.c_char_p, vp, ctypes.c_int32, ctypes.c_int32]),
+        "ppm_write_pfm": (ctypes.c_int, [ctypes.c_char_p, vp, ctypes.c_int32, ctypes.c_int32]),
         "ppm_batch_config_default": (None, [ctypes.POINTER(BatchConfig)]),
         "ppm_server_config_default": (None, [ctypes.POINTER(ServerConfig)]),
         "ppm_server_create": (ctypes.c_int, [ctypes.POINTER(ServerConfig), ctypes.POINTER(vp)]),
@@ -408,6 +410,14 @@ class Renderer:
         self._check(library().ppm_readback(self._h, bid, out.ctypes.data, out.nbytes))
         return out
 
+    def save(self, path):
+        """Writes the current image: .png (tone-mapped RGBA8) or .pfm (linear radiance)."""
+        w, h = self.params.image_width, self.params.image_height
+        if str(path).lower().endswith(".pfm"):
+            write_pfm(path, self.readback("radiance"), w, h)
+        else:
+            write_png(path, self.readback("rgba8"), w, h)
+
     def counters(self):
         c = Counters()
         self._check(library().ppm_readback(self._h, BUF_COUNTERS, ctypes.addressof(c), ctypes.sizeof(c)))
@@ -429,6 +439,26 @@ class Renderer:
             self.close()
         except Exception:
             pass
+
+
+def write_png(path, rgba8, width, height):
+    """The RGBA8 readback (rows bottom-up) as a PNG file."""
+    a = np.ascontiguousarray(rgba8, dtype=np.uint8)
+    if a.size != width * height * 4:
+        raise PPMError("rgba8 must hold width x height x 4 bytes")
+    rc = library().ppm_write_png(os.fsencode(path), a.ctypes.data, int(width), int(height))
+    if rc != 0:
+        raise PPMError("ppm_write_png failed (%d)" % rc)
+
+
+def write_pfm(path, radiance, width, height):
+    """The linear radiance readback (float RGBA, rows bottom-up) as a colour PFM file."""
+    a = np.ascontiguousarray(radiance, dtype=np.float32)
+    if a.size != width * height * 4:
+        raise PPMError("radiance must hold width x height x 4 floats")
+    rc = library().ppm_write_pfm(os.fsencode(path), a.ctypes.data, int(width), int(height))
+    if rc != 0:
+        raise PPMError("ppm_write_pfm failed (%d)" % rc)
 
 
 def scene_footprint(mesh, params):

@@ -51,9 +51,10 @@ def build(verbose=False):
         if verbose and out.strip():
             print(out)
         objs.append(obj)
-    obj = os.path.join(BUILD, "scene.o")
-    _run(["g++"] + CXX_FLAGS + ["-c", os.path.join(CSRC, "scene.cpp"), "-o", obj])
-    objs.append(obj)
+    for src in ("scene.cpp", "image.cpp"):
+        obj = os.path.join(BUILD, src.replace(".cpp", ".o"))
+        _run(["g++"] + CXX_FLAGS + ["-c", os.path.join(CSRC, src), "-o", obj])
+        objs.append(obj)
     tmp = LIB + ".tmp"
     _run([hipcc, "--offload-arch=" + ARCH, "-shared", "-fPIC", "-o", tmp] + objs)
     os.replace(tmp, LIB)

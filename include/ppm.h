@@ -363,6 +363,15 @@ double ppm_server_now(const ppm_server* s);
 void ppm_server_destroy(ppm_server* s);
 const char* ppm_server_last_error(const ppm_server* s);
 
+/* ---------------- image files ----------------
+ *
+ * The reference only shows its image in a window (glslppm.cpp:506-526); these write the same
+ * buffers to files.  Input rows are bottom-up, as every readback is.
+ * ppm_write_png: PPM_BUF_RGBA8 as an 8-bit RGBA PNG (rows flipped to PNG's top-down order).
+ * ppm_write_pfm: PPM_BUF_RADIANCE (float RGBA) as a colour PFM (RGB, little-endian, bottom-up). */
+int ppm_write_png(const char* path, const uint8_t* rgba, int32_t width, int32_t height);
+int ppm_write_pfm(const char* path, const float* rgba, int32_t width, int32_t height);
+
 /* ---------------- host scene layer (mesh.cpp, camera.cpp, bvh.cpp) ---------------- */
 
 typedef struct ppm_mesh ppm_mesh;
