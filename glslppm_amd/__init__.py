@@ -57,7 +57,8 @@ class Scene(ctypes.Structure):
                 ("num_materials", ctypes.c_int32), ("materials", ctypes.POINTER(Material)),
                 ("num_nodes", ctypes.c_int32), ("node_bounds", _fp), ("node_triangle", _ip), ("node_links", _ip),
                 ("num_lights", ctypes.c_int32), ("light_triangle", _ip), ("light_cdf", _fp),
-                ("light_area", ctypes.c_float), ("bsphere", ctypes.c_float * 4), ("camera", Camera)]
+                ("light_area", ctypes.c_float), ("bsphere", ctypes.c_float * 4), ("camera", Camera),
+                ("texcoords", _fp), ("texture_size", ctypes.c_int32), ("textures", ctypes.POINTER(ctypes.c_uint8))]
 
 
 class Params(ctypes.Structure):
@@ -166,6 +167,8 @@ def library():
         "ppm_mesh_generate": (ctypes.c_int, [ctypes.c_int32, ctypes.c_uint64, ctypes.c_int32, ctypes.c_int32,
                                              ctypes.POINTER(vp)]),
         "ppm_mesh_set_camera": (ctypes.c_int, [vp, _fp, _fp, ctypes.c_int32, ctypes.c_int32, ctypes.c_float]),
+        "ppm_mesh_set_texcoords": (ctypes.c_int, [vp, _fp]),
+        "ppm_mesh_set_texture": (ctypes.c_int, [vp, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32, vp]),
         "ppm_mesh_build": (ctypes.c_int, [vp]),
         "ppm_mesh_scene": (ctypes.POINTER(Scene), [vp]),
         "ppm_mesh_num_triangles": (ctypes.c_int32, [vp]),
@@ -287,6 +290,25 @@ class Mesh:
         self._check(library().ppm_mesh_build(self._h))
         return self
 
+    def set_texcoords(self, uv):
+        """Per-vertex texture coordinates [T, 3, 2] (None clears them); rebuild afterwards."""
+        if uv is None:
+            self._check(library().ppm_mesh_set_texcoords(self._h, None))
+        else:
+            a = np.ascontiguousarray(uv, dtype=np.float32).reshape(-1)
+            self._check(library().ppm_mesh_set_texcoords(self._h, a.ctypes.data_as(_fp)))
+        return self
+
+    def set_texture(self, material, image):
+        """An RGB8 image [height, width, 3] for a material (None: untextured); rebuild afterwards."""
+        if image is None:
+            self._check(library().ppm_mesh_set_texture(self._h, int(material), 0, 0, None))
+        else:
+            a = np.ascontiguousarray(image, dtype=np.uint8)
+            self._check(library().ppm_mesh_set_texture(self._h, int(material), a.shape[1], a.shape[0],
+                                                       a.ctypes.data))
+        return self
+
     @property
     def num_triangles(self):
         return int(library().ppm_mesh_num_triangles(self._h))
@@ -324,6 +346,11 @@ class Mesh:
             "light_cdf": arr(s.light_cdf, L, np.float32),
             "light_area": np.float32(s.light_area),
             "bsphere": np.array(list(s.bsphere), np.float32),
+            "texcoords": (arr(s.texcoords, T * 6, np.float32).reshape(T, 3, 2) if s.texcoords
+                          else None),
+            "texture_size": int(s.texture_size),
+            "textures": (arr(s.textures, M * s.texture_size * s.texture_size * 3, np.uint8).reshape(
+                M, s.texture_size, s.texture_size, 3) if s.textures else None),
             "camera": np.array(list(c.origin) + list(c.u) + list(c.v) + list(c.w) +
                                [c.half_width, c.half_height, c.distance], np.float32),
         }

@@ -377,8 +377,10 @@ uint64_t ppm_batch_device_bytes(const ppm_batch* b) { return b && b->arena ? b->
 
 uint64_t ppm_scene_footprint(const ppm_scene* scene, const ppm_params* params) {
     if (!scene || !params) return 0;
+    const int tex = (scene->texcoords && scene->textures) ? scene->texture_size : 0;
     return ppm_internal::state_footprint(params) +
-           ppm_internal::scene_footprint(scene->num_triangles, scene->num_nodes, scene->num_materials, scene->num_lights);
+           ppm_internal::scene_footprint(scene->num_triangles, scene->num_nodes, scene->num_materials, scene->num_lights,
+                                         tex);
 }
 
 int ppm_batch_get_tuning(ppm_batch* b, ppm_tuning* t) {

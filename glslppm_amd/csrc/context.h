@@ -48,7 +48,9 @@ private:
 
 // A scene in the form the kernels read (CGPURT::CreateTextures, gpurt.cpp:47-552), on the host.
 struct PackedScene {
-    std::vector<float4> nodes, isect, shade, mats;
+    std::vector<float4> nodes, isect, shade, mats, tex;
+    std::vector<uchar4> volume;
+    int tex_size = 0;
     std::vector<int> light_tri;
     std::vector<float> light_cdf;
     int T = 0, N = 0, M = 0, NL = 0;
@@ -167,7 +169,7 @@ int create_context(ppm_ctx* c, std::shared_ptr<SceneDev> scene, const ppm_camera
 void release_context(ppm_ctx* c);
 // HBM bytes of a context's pass state, and of a scene, as the arena allocates them.
 size_t state_footprint(const ppm_params* p);
-size_t scene_footprint(int T, int N, int M, int NL);
+size_t scene_footprint(int T, int N, int M, int NL, int tex_size);
 int set_error(ppm_ctx* c, int code, const std::string& msg);
 const std::string& create_error();
 // Blocks of each kind for one job.

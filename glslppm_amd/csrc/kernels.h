@@ -9,6 +9,8 @@
 //           tri_shade[T][6] float4 (v0, n0.x) (v1, n0.y) (v2, n0.z) (n1, n2.x)
 //                                  (n2.y, n2.z, gn.x, gn.y) (gn.z, material, 0, 0)
 //           mats[M][2] float4      (rgb, eta), (brdf, g, 0, 0)
+//           tri_tex[T][2] float4   (u0, v0, u1, v1), (u2, v2, 0, 0)     only with textures
+//           volume[M][S][S] uchar4 RGB8 + pad of every material (S = 512)  only with textures
 //   hit points (Q): pos, refl, nrm, rng, (tau, R), (E, N) float4; tri int; (mat, t) float2
 //   photons (P):    pos, flux, dir, rng float4 (.w of pos/flux/dir = hit normal);
 //                   state int2 (triangle, bounce | emitted << 16); slot int
@@ -30,6 +32,15 @@ struct DevScene {
     float bsphere_x, bsphere_y, bsphere_z, bsphere_r;
     float cube_lo_x, cube_lo_y, cube_lo_z;  // bounding cube of the scene for the ray buckets
     float cube_cells;                        // cells per unit length
+    // textures (null when no triangle is textured): per triangle (u0, v0, u1, v1), (u2, v2, 0, 0);
+    // volume [M][S][S] RGBA8 of the materials' images, sampled like a GL_LINEAR / GL_REPEAT 3D
+    // texture; material_rcp = 1.0f / M (the MaterialNumRcp uniform)
+    const float4* __restrict__ tri_tex;
+    const uchar4* __restrict__ volume;
+    int tex_size;
+    int num_materials;
+    float material_rcp;
+    int pad_tex;
 };
 
 struct DevCamera {

@@ -227,6 +227,46 @@ PPM_DEV f3 glossy_reflect(f3 d, f3 n, float g, float4& rndv) {
     return result;
 }
 
+// One axis of a GL_LINEAR, GL_REPEAT lookup: texel coordinate coord * size - 1/2, its floor
+// wrapped into [0, size) and the next texel, and the fraction as the weight of the next texel.
+// A coordinate too large for an exact floor (or NaN) reads texel 0 with weight 0.
+PPM_DEV float tex_axis(float coord, int size, int& i0, int& i1) {
+    const float a = coord * (float)size - 0.5f;
+    if (!(fabsf(a) <= 1.0e18f)) {
+        i0 = 0;
+        i1 = size > 1 ? 1 : 0;
+        return 0.0f;
+    }
+    const float fl = floorf(a);
+    long long k = (long long)fl % (long long)size;
+    if (k < 0) k += size;
+    i0 = (int)k;
+    i1 = i0 + 1 == size ? 0 : i0 + 1;
+    return a - fl;
+}
+
+PPM_DEV f3 texel(const DevScene& s, int i, int j, int k) {
+    const uchar4 c = s.volume[((size_t)k * s.tex_size + j) * s.tex_size + i];
+    return mk3((float)c.x / 255.0f, (float)c.y / 255.0f, (float)c.z / 255.0f);
+}
+
+PPM_DEV f3 lerp3(f3 p, f3 q, float w) { return p * (1.0f - w) + q * w; }
+
+// texture3D(VolumeTextureTextures, (u, v, r)).rgb (gpurt.cpp:158-198: RGB8, 512 x 512 x M,
+// GL_LINEAR, GL_REPEAT): trilinear weights of the GL specification, interpolated along u, then v,
+// then r, in fp32.
+PPM_DEV f3 tex3d(const DevScene& s, float u, float v, float r) {
+    int i0, i1, j0, j1, k0, k1;
+    const float a = tex_axis(u, s.tex_size, i0, i1);
+    const float b = tex_axis(v, s.tex_size, j0, j1);
+    const float c = tex_axis(r, s.num_materials, k0, k1);
+    const f3 y00 = lerp3(texel(s, i0, j0, k0), texel(s, i1, j0, k0), a);
+    const f3 y10 = lerp3(texel(s, i0, j1, k0), texel(s, i1, j1, k0), a);
+    const f3 y01 = lerp3(texel(s, i0, j0, k1), texel(s, i1, j0, k1), a);
+    const f3 y11 = lerp3(texel(s, i0, j1, k1), texel(s, i1, j1, k1), a);
+    return lerp3(lerp3(y00, y10, b), lerp3(y01, y11, b), c);
+}
+
 template <typename T>
 PPM_DEV T wave_min(T v) {
     for (int o = 32; o > 0; o >>= 1) v = min1(v, __shfl_xor(v, o, 64));
@@ -381,19 +421,28 @@ PPM_DEV void eye(const Job& J, int blk) {
             last_mat = i.mat;
             pos = i.pos;
             nrm = i.nrm;
+            // Result.tex = T2 B.x + T0 B.y + T1 B.z; textured surfaces scale their colour
+            // (eyeraytrace.fs:277-280, 443-446)
+            f3 icol = i.col;
+            if (s.tri_tex) {
+                const float4 ta = s.tri_tex[2 * h.tri], tb = s.tri_tex[2 * h.tri + 1];
+                const float tu = (tb.x * h.bx + ta.x * h.by) + ta.z * h.bz;
+                const float tv = (tb.y * h.bx + ta.y * h.by) + ta.w * h.bz;
+                if (fabsf(tu) < 1e10f) icol = icol * tex3d(s, tu, tv, s.material_rcp * ((float)i.mat + 0.5f));
+            }
             if (i.brdf == -1) {
                 nrm = mk3(0.0f, 0.0f, 0.0f);
-                emi = col * i.col;
+                emi = col * icol;
                 break;
             } else if (i.brdf == 0) {
-                col = col * i.col;
+                col = col * icol;
                 break;
             } else if (i.brdf == 1 || i.brdf == 4) {
                 org = pos + eps * i.gnrm;
                 if (i.brdf == 4) nrm = glossy_reflect(nrm, nrm, i.g, rndv);
                 dir = reflect3(dir, nrm);
                 if (dot3(dir, i.gnrm) < 0.0f) dir = -dir;
-                col = col * i.col;
+                col = col * icol;
             } else if (i.brdf == 2 || i.brdf == 5) {
                 if (i.brdf == 5) nrm = glossy_reflect(nrm, nrm, i.g, rndv);
                 const float ln = dot3(nrm, dir);
@@ -405,14 +454,14 @@ PPM_DEV void eye(const Job& J, int blk) {
                         dir = reflect3(dir, nrm);
                         if (dot3(dir, i.gnrm) < 0.0f) dir = -dir;
                     } else {
-                        col = col * i.col;
+                        col = col * icol;
                         org = pos - eps * i.gnrm;
                         dir = refract3(dir, nrm, 1.0f / eta);
                         if (dot3(dir, i.gnrm) > 0.0f) dir = -dir;
                     }
                 } else {
                     const float re = fresnel(-dir, -nrm, eta, 1.0f);
-                    col = col * i.col;
+                    col = col * icol;
                     if (gpu_rnd(rndv) < re) {
                         org = pos - eps * i.gnrm;
                         dir = reflect3(dir, -nrm);
@@ -431,7 +480,7 @@ PPM_DEV void eye(const Job& J, int blk) {
                     dir = reflect3(dir, nrm);
                     if (dot3(dir, i.gnrm) < 0.0f) dir = -dir;
                 } else {
-                    col = col * i.col;
+                    col = col * icol;
                     break;
                 }
             }
@@ -548,6 +597,13 @@ PPM_DEV void light_sample(const DevScene& s, float4& rndv, f3& flux, f3& dir, f3
     const float4 m0 = s.mats[2 * mat];
     const float le = s.light_area * 3.141592f;
     flux = mk3(m0.x * le, m0.y * le, m0.z * le);
+    // T = T2 c + T0 a + T1 b on textured lights (photontrace.fs:413-422)
+    if (s.tri_tex) {
+        const float4 ta = s.tri_tex[2 * li], tb = s.tri_tex[2 * li + 1];
+        const float tu = (tb.x * c + ta.x * a) + ta.z * b;
+        const float tv = (tb.y * c + ta.y * a) + ta.w * b;
+        if (fabsf(tu) < 1e10f) flux = flux * tex3d(s, tu, tv, s.material_rcp * ((float)mat + 0.75f));
+    }
     org = (V0 * a + V1 * b) + V2 * c;
     const f3 nrm = normalize3((N0 * a + N1 * b) + N2 * c);
     const f3 gnrm = mk3(q4.z, q4.w, q5.x);
@@ -630,6 +686,20 @@ PPM_DEV bool photon_gen(const DevScene& s, const PhotonParams& pp, const PhotonS
         const int brdf = (int)m1.x;
         const float eta = m0.w, gl = m1.y;
         const f3 ipos = xyz(ppos);
+        if (s.tri_tex) {
+            // barycentrics of the stored hit from the three sub-triangle areas, then
+            // i.tex = T2 B.x + T0 B.y + T1 B.z (photontrace.fs:519-544, 562-565)
+            const f3 V0 = xyz(tr[0]), V1 = xyz(tr[1]), V2 = xyz(tr[2]);
+            const float bx = length3(cross3(V1 - V0, ipos - V0));
+            const float by = length3(cross3(V2 - V1, ipos - V1));
+            const float bz = length3(cross3(V0 - V2, ipos - V2));
+            const float bw = (bx + by) + bz;
+            const float wx = bx / bw, wy = by / bw, wz = bz / bw;
+            const float4 ta = s.tri_tex[2 * st.x], tb = s.tri_tex[2 * st.x + 1];
+            const float tu = (tb.x * wx + ta.x * wy) + ta.z * wz;
+            const float tv = (tb.y * wx + ta.y * wy) + ta.w * wz;
+            if (fabsf(tu) < 1e10f) col = col * tex3d(s, tu, tv, s.material_rcp * (((float)mat + 0.5f) + 0.25f));
+        }
         f3 inrm = prev_nrm;
         const f3 pd = xyz(pdir);
         const float eps = 1e-6f;

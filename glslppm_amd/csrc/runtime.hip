@@ -103,6 +103,8 @@ int check_scene(const ppm_scene* s) {
     for (int l = 0; l < s->num_lights; l++)
         if (s->light_triangle[l] < 0 || s->light_triangle[l] >= s->num_triangles)
             return set_err(nullptr, PPM_E_INVAL, "light triangle out of range");
+    if (s->texcoords && (!s->textures || s->texture_size <= 0 || s->texture_size > 8192))
+        return set_err(nullptr, PPM_E_INVAL, "texture coordinates need a texture volume of size 1..8192");
     return PPM_OK;
 }
 
@@ -442,15 +444,16 @@ size_t state_footprint(const ppm_params* p) {
     return tot;
 }
 
-size_t scene_footprint(int T, int N, int M, int NL) {
+size_t scene_footprint(int T, int N, int M, int NL, int tex_size) {
+    const size_t S = (size_t)tex_size;
     const size_t sizes[] = {(size_t)6 * (N + 1) * 32, (size_t)T * 48, (size_t)T * 96, (size_t)M * 32, (size_t)NL * 4,
-                            (size_t)NL * 4};
+                            (size_t)NL * 4, S ? (size_t)T * 32 : 0, (size_t)M * S * S * 4};
     size_t tot = 0;
     for (size_t b : sizes) tot += Arena::round_size(b ? b : 16);
     return tot;
 }
 
-size_t PackedScene::device_bytes() const { return scene_footprint(T, N, M, NL); }
+size_t PackedScene::device_bytes() const { return scene_footprint(T, N, M, NL, tex_size); }
 
 SceneDev::~SceneDev() {
     for (void* p : allocs) {
@@ -478,6 +481,23 @@ int pack_scene(const ppm_scene* s, PackedScene* out) {
     out->light_area = s->light_area;
     for (int k = 0; k < 4; k++) out->bsphere[k] = s->bsphere[k];
     out->camera = s->camera;
+    out->tex.clear();
+    out->volume.clear();
+    out->tex_size = 0;
+    if (s->texcoords && s->textures && s->texture_size > 0) {
+        const int T = s->num_triangles, S = s->texture_size;
+        out->tex_size = S;
+        out->tex.resize((size_t)T * 2);
+        for (int t = 0; t < T; t++) {
+            const float* u = s->texcoords + (size_t)t * 6;
+            out->tex[(size_t)t * 2] = make_float4(u[0], u[1], u[2], u[3]);
+            out->tex[(size_t)t * 2 + 1] = make_float4(u[4], u[5], 0.0f, 0.0f);
+        }
+        const size_t n = (size_t)out->M * S * S;
+        out->volume.resize(n);
+        for (size_t i = 0; i < n; i++)
+            out->volume[i] = make_uchar4(s->textures[3 * i], s->textures[3 * i + 1], s->textures[3 * i + 2], 255);
+    }
     return PPM_OK;
 }
 
@@ -504,7 +524,7 @@ int upload_scene(const PackedScene& ps, Arena* arena, hipStream_t up, hipStream_
         }
         return PPM_OK;
     };
-    void *d_nodes, *d_isect, *d_shade, *d_mats, *d_ltri, *d_lcdf;
+    void *d_nodes, *d_isect, *d_shade, *d_mats, *d_ltri, *d_lcdf, *d_tex = nullptr, *d_vol = nullptr;
     int rc;
     if ((rc = put(ps.nodes.data(), ps.nodes.size() * 16, &d_nodes)) ||
         (rc = put(ps.isect.data(), ps.isect.size() * 16, &d_isect)) ||
@@ -512,6 +532,9 @@ int upload_scene(const PackedScene& ps, Arena* arena, hipStream_t up, hipStream_
         (rc = put(ps.mats.data(), ps.mats.size() * 16, &d_mats)) ||
         (rc = put(ps.light_tri.data(), ps.light_tri.size() * 4, &d_ltri)) ||
         (rc = put(ps.light_cdf.data(), ps.light_cdf.size() * 4, &d_lcdf)))
+        return rc;
+    if (ps.tex_size > 0 && ((rc = put(ps.tex.data(), ps.tex.size() * 16, &d_tex)) ||
+                            (rc = put(ps.volume.data(), ps.volume.size() * 4, &d_vol))))
         return rc;
     DevScene& sc = sd->s;
     std::memset(&sc, 0, sizeof(sc));
@@ -533,6 +556,11 @@ int upload_scene(const PackedScene& ps, Arena* arena, hipStream_t up, hipStream_
     sc.cube_lo_y = ps.bsphere[1] - ps.bsphere[3];
     sc.cube_lo_z = ps.bsphere[2] - ps.bsphere[3];
     sc.cube_cells = ps.bsphere[3] > 0.0f ? (float)(1 << kRayCellBits) / (2.0f * ps.bsphere[3]) : 0.0f;
+    sc.tri_tex = (const float4*)d_tex;
+    sc.volume = (const uchar4*)d_vol;
+    sc.tex_size = ps.tex_size;
+    sc.num_materials = ps.M;
+    sc.material_rcp = 1.0f / (float)ps.M;
     *out = sd;
     return PPM_OK;
 }

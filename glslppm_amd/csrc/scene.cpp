@@ -12,6 +12,7 @@
 #include "../../include/ppm.h"
 
 #include <algorithm>
+#include <cctype>
 #include <atomic>
 #include <cmath>
 #include <condition_variable>
@@ -69,6 +70,7 @@ struct Tri {
     V3 p[3];
     V3 n[3];
     int mat;
+    float uv[3][2] = {{1.0e30f, 1.0e30f}, {1.0e30f, 1.0e30f}, {1.0e30f, 1.0e30f}};  // as given
 };
 
 struct Node {
@@ -85,6 +87,9 @@ struct ppm_mesh {
     std::vector<ppm_material> mats;
     ppm_camera cam;
     bool has_camera = false;
+    bool has_uv = false;                        // every triangle carries texture coordinates
+    std::vector<std::vector<uint8_t>> tex;      // per material: RGB8 image, empty if untextured
+    std::vector<int> tex_w, tex_h;
 
     // derived by build
     Box bbox;
@@ -95,6 +100,8 @@ struct ppm_mesh {
     std::vector<int32_t> light_triangle;
     std::vector<float> light_cdf;
     float light_area = 0.0f;
+    std::vector<float> texcoords;
+    std::vector<uint8_t> volume;
     ppm_scene scene;
     bool built = false;
 };
@@ -705,7 +712,46 @@ struct MtlRec {
     std::string name;
     V3 ka{0, 0, 0}, kd{0, 0, 0}, ks{0, 0, 0};
     float ns = 0.0f;
+    bool textured = false;
+    int tw = 0, th = 0;
+    std::vector<uint8_t> tex;
 };
+
+// LoadPPM (mesh.cpp:37-55): "P6", width, height, "255", each followed by white space as
+// fscanf's "\n" skips it (so are leading white-space bytes of the pixel data, as there), then
+// width x height x 3 bytes; a short file leaves the rest zero.  Comments are not read.
+bool load_ppm_image(const std::string& path, int& w, int& h, std::vector<uint8_t>& rgb) {
+    FILE* fp = std::fopen(path.c_str(), "rb");
+    if (!fp) return false;
+    std::vector<uint8_t> d;
+    int ch;
+    while ((ch = std::fgetc(fp)) != EOF) d.push_back((uint8_t)ch);
+    std::fclose(fp);
+    size_t i = 0;
+    auto skip_ws = [&]() {
+        while (i < d.size() && std::isspace(d[i])) i++;
+    };
+    auto read_int = [&](int& v) -> bool {
+        skip_ws();
+        size_t j = i;
+        long long x = 0;
+        while (i < d.size() && d[i] >= '0' && d[i] <= '9' && x < (1ll << 31)) x = x * 10 + (d[i++] - '0');
+        v = (int)x;
+        return i > j;
+    };
+    if (d.size() < 2 || d[0] != 'P' || d[1] != '6') return false;
+    i = 2;
+    skip_ws();
+    int maxv = 0;
+    if (!read_int(w) || !read_int(h) || !read_int(maxv) || maxv != 255 || w <= 0 || h <= 0 ||
+        (long long)w * h > (1ll << 28))
+        return false;
+    skip_ws();
+    rgb.assign((size_t)w * h * 3, 0);
+    const size_t n = std::min(rgb.size(), d.size() - i);
+    if (n) std::memcpy(rgb.data(), d.data() + i, n);
+    return true;
+}
 
 bool starts(const std::string& s, const char* p) { return s.compare(0, std::strlen(p), p) == 0; }
 
@@ -739,6 +785,7 @@ bool load_mtl(const std::string& path, std::vector<MtlRec>& mtls) {
         float r = 0, g = 0, b = 0, s = 0;
         if (starts(ln, "newmtl")) {
             cur.name = ln.size() > 7 ? ln.substr(7) : std::string();
+            cur.textured = false;
         } else if (starts(ln, "Ka") || starts(ln, "Ke")) {
             std::sscanf(ln.c_str() + 3, "%f %f %f", &r, &g, &b);
             cur.ka = mk(r, g, b);
@@ -752,6 +799,19 @@ bool load_mtl(const std::string& path, std::vector<MtlRec>& mtls) {
             std::sscanf(ln.c_str() + 3, "%f", &s);
             cur.ns = s;
             mtls.push_back(cur);
+            mtls.back().textured = false;
+            mtls.back().tex.clear();
+        } else if (starts(ln, "map_Kd")) {
+            // the image belongs to the last material committed so far (mesh.cpp:117-122);
+            // the name is resolved next to the MTL file first, then as given
+            std::string name = ln.size() > 7 ? ln.substr(7) : std::string();
+            while (!name.empty() && (name.back() == '\n' || name.back() == '\r')) name.pop_back();
+            if (mtls.empty()) continue;
+            MtlRec& m = mtls.back();
+            if (!load_ppm_image(dir_of(path) + name, m.tw, m.th, m.tex) &&
+                !load_ppm_image(name, m.tw, m.th, m.tex))
+                return false;
+            m.textured = true;
         }
     }
     return true;
@@ -812,12 +872,13 @@ int load_obj(const char* path, const float position[3], float scale, ppm_mesh* m
     if (!read_lines(path, lines)) return fail(PPM_E_IO, std::string("cannot read ") + path);
     std::vector<MtlRec> mtls;
     std::vector<V3> v, vn;
+    std::vector<float> vt;
     struct Face {
-        int v[3], n[3];
+        int v[3], t[3], n[3];
         int mat;
     };
     std::vector<Face> faces;
-    bool no_normals = false, no_materials = true;
+    bool no_normals = false, no_materials = true, no_texcoords = false;
     int cmat = 0;
     for (const std::string& ln : lines) {
         if (starts(ln, "mtllib")) {
@@ -833,7 +894,10 @@ int load_obj(const char* path, const float position[3], float scale, ppm_mesh* m
                 float l = std::sqrt(x * x + y * y + z * z);
                 vn.push_back(mk(x / l, y / l, z / l));
             } else if (ln[1] == 't') {
-                // texture coordinates are not used: textured materials are not supported
+                float u = 0, w = 0;
+                std::sscanf(ln.c_str() + 2, "%f %f", &u, &w);
+                vt.push_back(u);
+                vt.push_back(w);
             } else {
                 std::sscanf(ln.c_str() + 1, "%f %f %f", &x, &y, &z);
                 v.push_back(mk(x, y, z));
@@ -856,8 +920,10 @@ int load_obj(const char* path, const float position[3], float scale, ppm_mesh* m
                 int vi, ti, ni;
                 parse_corner(w[q], vi, ti, ni);
                 fc.v[q] = vi - 1;
+                fc.t[q] = ti - 1;
                 fc.n[q] = ni - 1;
                 if (!ni) no_normals = true;
+                if (!ti) no_texcoords = true;
             }
             fc.mat = cmat;
             faces.push_back(fc);
@@ -865,7 +931,12 @@ int load_obj(const char* path, const float position[3], float scale, ppm_mesh* m
     }
     const V3 pos = mk(position[0], position[1], position[2]);
     if (!no_materials) {
-        for (const MtlRec& r : mtls) m->mats.push_back(classify(r));
+        for (const MtlRec& r : mtls) {
+            m->mats.push_back(classify(r));
+            m->tex.push_back(r.textured ? r.tex : std::vector<uint8_t>());
+            m->tex_w.push_back(r.textured ? r.tw : 0);
+            m->tex_h.push_back(r.textured ? r.th : 0);
+        }
     } else {
         // mesh.cpp:613-621: a default material whose colour is left at zero
         ppm_material d;
@@ -874,7 +945,11 @@ int load_obj(const char* path, const float position[3], float scale, ppm_mesh* m
         d.brdf = PPM_BRDF_DIFFUSE;
         d.g = 0.0f;
         m->mats.push_back(d);
+        m->tex.emplace_back();
+        m->tex_w.push_back(0);
+        m->tex_h.push_back(0);
     }
+    m->has_uv = !no_texcoords && !faces.empty();
     for (const Face& fc : faces) {
         Tri t;
         for (int q = 0; q < 3; q++) {
@@ -891,6 +966,15 @@ int load_obj(const char* path, const float position[3], float scale, ppm_mesh* m
             t.n[0] = t.n[1] = t.n[2] = n;
         }
         t.mat = no_materials ? 0 : fc.mat;
+        for (int q = 0; q < 3; q++) {
+            t.uv[q][0] = t.uv[q][1] = 1.0e30f;
+            if (m->has_uv) {
+                if (fc.t[q] < 0 || 2 * (size_t)fc.t[q] + 1 >= vt.size())
+                    return fail(PPM_E_INVAL, "bad texture coordinate index");
+                t.uv[q][0] = vt[2 * (size_t)fc.t[q]];
+                t.uv[q][1] = vt[2 * (size_t)fc.t[q] + 1];
+            }
+        }
         m->tris.push_back(t);
     }
     return PPM_OK;
@@ -952,6 +1036,34 @@ int ppm_mesh_from_arrays(int32_t num_triangles, const float* positions, const fl
         }
     }
     *out = m;
+    return PPM_OK;
+}
+
+int ppm_mesh_set_texcoords(ppm_mesh* m, const float* texcoords) {
+    if (!m) return fail(PPM_E_INVAL, "null mesh");
+    for (size_t i = 0; i < m->tris.size(); i++)
+        for (int q = 0; q < 3; q++) {
+            m->tris[i].uv[q][0] = texcoords ? texcoords[i * 6 + q * 2] : 1.0e30f;
+            m->tris[i].uv[q][1] = texcoords ? texcoords[i * 6 + q * 2 + 1] : 1.0e30f;
+        }
+    m->has_uv = texcoords != nullptr;
+    m->built = false;
+    return PPM_OK;
+}
+
+int ppm_mesh_set_texture(ppm_mesh* m, int32_t material, int32_t width, int32_t height, const uint8_t* rgb) {
+    if (!m) return fail(PPM_E_INVAL, "null mesh");
+    if (material < 0 || material >= (int)m->mats.size()) return fail(PPM_E_INVAL, "material out of range");
+    if (rgb && (width <= 0 || height <= 0 || (long long)width * height > (1ll << 28)))
+        return fail(PPM_E_INVAL, "bad texture size");
+    m->tex.resize(m->mats.size());
+    m->tex_w.resize(m->mats.size(), 0);
+    m->tex_h.resize(m->mats.size(), 0);
+    if (rgb) m->tex[material].assign(rgb, rgb + (size_t)width * height * 3);
+    else m->tex[material].clear();
+    m->tex_w[material] = rgb ? width : 0;
+    m->tex_h[material] = rgb ? height : 0;
+    m->built = false;
     return PPM_OK;
 }
 
@@ -1047,6 +1159,48 @@ int ppm_mesh_build(ppm_mesh* m) {
         }
         thread_links(nodes, m->node_links);
 
+        // TMesh::LoadOBJ (mesh.cpp:656-669): a triangle keeps its texture coordinates only when
+        // its material is textured; then the volume of gpurt.cpp:158-198 (nearest resampling to
+        // 512^2, magenta for untextured materials)
+        const int M = (int)m->mats.size();
+        m->tex.resize(M);
+        m->tex_w.resize(M, 0);
+        m->tex_h.resize(M, 0);
+        bool any_tex = false;
+        m->texcoords.assign((size_t)T * 6, 1.0e30f);
+        if (m->has_uv)
+            for (int i = 0; i < T; i++)
+                if (!m->tex[m->tris[i].mat].empty()) {
+                    any_tex = true;
+                    for (int q = 0; q < 3; q++) {
+                        m->texcoords[(size_t)i * 6 + q * 2] = m->tris[i].uv[q][0];
+                        m->texcoords[(size_t)i * 6 + q * 2 + 1] = m->tris[i].uv[q][1];
+                    }
+                }
+        const int S = 512;
+        m->volume.clear();
+        if (any_tex) {
+            m->volume.resize((size_t)M * S * S * 3);
+            for (int z = 0; z < M; z++)
+                for (int y = 0; y < S; y++)
+                    for (int x = 0; x < S; x++) {
+                        uint8_t* d = &m->volume[3 * ((size_t)x + (size_t)y * S + (size_t)z * S * S)];
+                        if (!m->tex[z].empty()) {
+                            const int tw = m->tex_w[z], th = m->tex_h[z];
+                            const size_t src = 3 * ((size_t)(x * tw / S) + (size_t)(y * th / S) * tw);
+                            d[0] = m->tex[z][src];
+                            d[1] = m->tex[z][src + 1];
+                            d[2] = m->tex[z][src + 2];
+                        } else {
+                            d[0] = 255;
+                            d[1] = 0;
+                            d[2] = 255;
+                        }
+                    }
+        } else {
+            m->texcoords.clear();
+        }
+
         m->positions.resize((size_t)T * 9);
         m->normals.resize((size_t)T * 9);
         m->material.resize(T);
@@ -1088,6 +1242,9 @@ int ppm_mesh_build(ppm_mesh* m) {
             ppm_mesh_set_camera(m, o, at, 256, 256, 45.0f);
         }
         s.camera = m->cam;
+        s.texcoords = m->texcoords.empty() ? nullptr : m->texcoords.data();
+        s.texture_size = m->volume.empty() ? 0 : 512;
+        s.textures = m->volume.empty() ? nullptr : m->volume.data();
         m->built = true;
     } catch (const std::bad_alloc&) {
         return fail(PPM_E_OOM, "out of memory building the BVH");

@@ -92,6 +92,14 @@ typedef struct ppm_scene {
     float light_area;             /* LightSummedArea */
     float bsphere[4];             /* SceneBSphere: bbox centre, half diagonal (glslppm.cpp:388-392) */
     ppm_camera camera;
+    /* Textures (mesh.cpp:37-127, 656-669; gpurt.cpp:158-198).  texcoords holds each triangle
+     * vertex's (u, v) for triangles of textured materials and 1e30 elsewhere; NULL when no
+     * triangle is textured (then no lookup ever happens).  textures is the RGB8 volume
+     * [num_materials][texture_size][texture_size][3]: a textured material's image resampled to
+     * texture_size^2 by nearest texel, an untextured material's slice magenta (255, 0, 255). */
+    const float* texcoords;       /* [T][3][2] or NULL */
+    int32_t texture_size;         /* 512 (gpurt.cpp:170), 0 without textures */
+    const uint8_t* textures;      /* [M][S][S][3] or NULL */
 } ppm_scene;
 
 /* Pass-loop parameters.  The reference couples Q = P = H = 512^2 (glslppm.cpp:36-40); here the
@@ -393,6 +401,14 @@ int ppm_mesh_from_arrays(int32_t num_triangles, const float* positions, const fl
 #define PPM_SCENE_SPHERES 1
 #define PPM_SCENE_TERRAIN 2
 int ppm_mesh_generate(int32_t kind, uint64_t seed, int32_t count, int32_t detail, ppm_mesh** out);
+
+/* Texture coordinates of a mesh made from arrays, [T][3 vertices][u, v] (NULL clears them), and
+ * the image of one material (RGB8, rows as stored in a PPM file, which makes the material
+ * textured; NULL makes it untextured).  OBJ files get both from vt lines and map_Kd PPM images.
+ * As in TMesh::LoadOBJ (mesh.cpp:656-669), a triangle uses its texture coordinates only when its
+ * material is textured. */
+int ppm_mesh_set_texcoords(ppm_mesh* mesh, const float* texcoords);
+int ppm_mesh_set_texture(ppm_mesh* mesh, int32_t material, int32_t width, int32_t height, const uint8_t* rgb);
 
 /* TCamera::Set (camera.cpp:4-18). */
 int ppm_mesh_set_camera(ppm_mesh* mesh, const float origin[3], const float lookat[3],

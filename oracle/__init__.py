@@ -56,6 +56,8 @@ def library():
         lib.oracle_num_threads.restype = ctypes.c_int
         lib.oracle_update.restype = None
         lib.oracle_update.argtypes = [fp, fp, fp, fp, ctypes.c_float, fp, ctypes.c_float]
+        lib.oracle_texture_sample.restype = None
+        lib.oracle_texture_sample.argtypes = [vp, ctypes.c_float, ctypes.c_float, ctypes.c_float, fp]
         _lib = lib
     return _lib
 
@@ -119,6 +121,13 @@ def trace(scene_ptr, org, direction, cull=False, photon=False):
     tri = library().oracle_trace(ctypes.cast(scene_ptr, ctypes.c_void_p), _fptr(o), _fptr(d), int(cull), int(photon),
                                  _fptr(t), _fptr(pos))
     return tri, t[0], pos
+
+
+def texture_sample(scene_ptr, u, v, r):
+    """texture3D of the scene's RGB8 volume (GL_LINEAR, GL_REPEAT) as the oracle evaluates it."""
+    out = np.zeros(3, np.float32)
+    library().oracle_texture_sample(ctypes.cast(scene_ptr, ctypes.c_void_p), float(u), float(v), float(r), _fptr(out))
+    return out
 
 
 def radiance(flux_radius, emission, total_photons):
@@ -232,6 +241,12 @@ def ref_library():
         lib.ref_bvh.restype = None
         lib.ref_camera.argtypes = [fp, fp, ctypes.c_int, ctypes.c_int, ctypes.c_float, fp]
         lib.ref_camera.restype = None
+        lib.ref_texcoords.argtypes = [fp]
+        lib.ref_texcoords.restype = None
+        lib.ref_texture_info.argtypes = [ctypes.c_int, ip, ip]
+        lib.ref_texture_info.restype = ctypes.c_int
+        lib.ref_texture.argtypes = [ctypes.c_int, ctypes.c_void_p]
+        lib.ref_texture.restype = None
         _ref = lib
     return _ref
 

@@ -241,9 +241,72 @@ static v3 normal_rebuilt(const ppm_scene* s, int tri, int k) {
     return V3(n[0], n[1], z);
 }
 
+/* Texture coordinate k (0: u, 1: v) of a triangle vertex: texcoords or, without any, 1e30
+ * (mesh.cpp:664-668). */
+static float texcoord(const ppm_scene* s, int tri, int vert, int k) {
+    return s->texcoords ? s->texcoords[(size_t)tri * 6 + vert * 2 + k] : 1.0e30f;
+}
+
+/* texture3D of the RGB8 volume (gpurt.cpp:158-198), GL_LINEAR and GL_REPEAT on all three axes
+ * (GL specification, linear filtering: i0 = wrap(floor(u - 1/2)), weight frac(u - 1/2)); the
+ * restatement interpolates along u, then v, then r, in fp32.  Coordinates too large for an
+ * exact floor, or NaN, read texel 0 with weight 0.  Pinned against a float64 evaluation of the
+ * same formula and against texel centres in tests/test_oracle_texture.py. */
+static float tex_axis(float coord, int size, int* i0, int* i1) {
+    float a = coord * (float)size - 0.5f;
+    if (!(fabsf(a) <= 1.0e18f)) {
+        *i0 = 0;
+        *i1 = size > 1 ? 1 : 0;
+        return 0.0f;
+    }
+    float fl = floorf(a);
+    long long k = (long long)fl % (long long)size;
+    if (k < 0) k += size;
+    *i0 = (int)k;
+    *i1 = *i0 + 1 == size ? 0 : *i0 + 1;
+    return a - fl;
+}
+
+static v3 texel(const ppm_scene* s, int i, int j, int k) {
+    const size_t S = (size_t)s->texture_size;
+    const uint8_t* c = s->textures + 3 * (((size_t)k * S + (size_t)j) * S + (size_t)i);
+    return V3((float)c[0] / 255.0f, (float)c[1] / 255.0f, (float)c[2] / 255.0f);
+}
+
+static v3 lerp3(v3 p, v3 q, float w) {
+    float r = 1.0f - w;
+    return V3(p.x * r + q.x * w, p.y * r + q.y * w, p.z * r + q.z * w);
+}
+
+v3 oracle_texture3d(const ppm_scene* s, float u, float v, float r) {
+    int i0, i1, j0, j1, k0, k1;
+    float a = tex_axis(u, s->texture_size, &i0, &i1);
+    float b = tex_axis(v, s->texture_size, &j0, &j1);
+    float c = tex_axis(r, s->num_materials, &k0, &k1);
+    v3 y00 = lerp3(texel(s, i0, j0, k0), texel(s, i1, j0, k0), a);
+    v3 y10 = lerp3(texel(s, i0, j1, k0), texel(s, i1, j1, k0), a);
+    v3 y01 = lerp3(texel(s, i0, j0, k1), texel(s, i1, j0, k1), a);
+    v3 y11 = lerp3(texel(s, i0, j1, k1), texel(s, i1, j1, k1), a);
+    return lerp3(lerp3(y00, y10, b), lerp3(y01, y11, b), c);
+}
+
+/* unit entry point for the texture pins */
+void oracle_texture_sample(const ppm_scene* s, float u, float v, float r, float* out) {
+    v3 c = oracle_texture3d(s, u, v, r);
+    out[0] = c.x;
+    out[1] = c.y;
+    out[2] = c.z;
+}
+
+static int textured(const ppm_scene* s) { return s->texcoords && s->textures && s->texture_size > 0; }
+
+/* MaterialNumRcp (glslppm.cpp:256, 376) */
+static float material_rcp(const ppm_scene* s) { return 1.0f / (float)s->num_materials; }
+
 typedef struct {
     float t;
     v3 pos, nrm, gnrm, col;
+    float tex_u, tex_v; /* Result.tex */
     int brdf;
     float eta, g;
     int matid;
@@ -324,6 +387,9 @@ static Isect raytrace(const ppm_scene* s, v3 org, v3 dir, int cull, int photon) 
         R.pos = vadd(vadd(vscale(V2, bary.x), vscale(V0, bary.y)), vscale(V1, bary.z));
         R.nrm = vnormalize(vadd(vadd(vscale(N2, bary.x), vscale(N0, bary.y)), vscale(N1, bary.z)));
         R.gnrm = vnormalize(vcross(vsub(V1, V0), vsub(V2, V0)));
+        /* Result.tex = T2 * Barycentric.x + T0 * Barycentric.y + T1 * Barycentric.z */
+        R.tex_u = (texcoord(s, tri, 2, 0) * bary.x + texcoord(s, tri, 0, 0) * bary.y) + texcoord(s, tri, 1, 0) * bary.z;
+        R.tex_v = (texcoord(s, tri, 2, 1) * bary.x + texcoord(s, tri, 0, 1) * bary.y) + texcoord(s, tri, 1, 1) * bary.z;
         const ppm_material* m = &s->materials[(int)isect_mat];
         R.col = V3(m->color[0], m->color[1], m->color[2]);
         R.brdf = m->brdf;
@@ -496,6 +562,9 @@ static void eye_fragment(Oracle* o, int i, int j, float aax, float aay, int num_
         }
         pos = I.pos;
         nrm = I.nrm;
+        /* eyeraytrace.fs:443-446 */
+        if (textured(s) && fabsf(I.tex_u) < 1e+10f)
+            I.col = vmul(I.col, oracle_texture3d(s, I.tex_u, I.tex_v, material_rcp(s) * ((float)I.matid + 0.5f)));
         if (I.brdf == -1) {
             nrm = V3(0, 0, 0);
             emi = vmul(col, I.col);
@@ -599,6 +668,13 @@ static void light_sample(const ppm_scene* s, v4* rndv, v3* flux, v3* dir, v3* or
     *flux = V3(m->color[0], m->color[1], m->color[2]);
     float le = s->light_area * 3.141592f;
     *flux = vmul(*flux, V3(le, le, le));
+    /* photontrace.fs:413-422: T = T2 * c + T0 * a + T1 * b */
+    if (textured(s)) {
+        float tu = (texcoord(s, tri, 2, 0) * c + texcoord(s, tri, 0, 0) * a) + texcoord(s, tri, 1, 0) * b;
+        float tv = (texcoord(s, tri, 2, 1) * c + texcoord(s, tri, 0, 1) * a) + texcoord(s, tri, 1, 1) * b;
+        if (fabsf(tu) < 1e+10f)
+            *flux = vmul(*flux, oracle_texture3d(s, tu, tv, material_rcp(s) * ((float)s->material[tri] + 0.75f)));
+    }
     v3 o = vadd(vadd(vscale(V0, a), vscale(V1, b)), vscale(V2, c));
     v3 nrm = vnormalize(vadd(vadd(vscale(N0, a), vscale(N1, b)), vscale(N2, c)));
     v3 gnrm = vnormalize(vcross(vsub(V1, V0), vsub(V2, V0)));
@@ -667,6 +743,19 @@ static int photon_fragment(Oracle* o, int px) {
         v3 inrm = pnrm;
         v3 pd = V3(PD.x, PD.y, PD.z);
         const float eps = 1e-6f;
+        /* photontrace.fs:519-544, 562-565: barycentrics from sub-triangle areas, then i.tex */
+        if (textured(s)) {
+            float bx = vlength(vcross(vsub(V1, V0), vsub(ipos, V0)));
+            float by = vlength(vcross(vsub(V2, V1), vsub(ipos, V1)));
+            float bz = vlength(vcross(vsub(V0, V2), vsub(ipos, V2)));
+            float bw = (bx + by) + bz;
+            float wx = bx / bw, wy = by / bw, wz = bz / bw;
+            float tu = (texcoord(s, tri, 2, 0) * wx + texcoord(s, tri, 0, 0) * wy) + texcoord(s, tri, 1, 0) * wz;
+            float tv = (texcoord(s, tri, 2, 1) * wx + texcoord(s, tri, 0, 1) * wy) + texcoord(s, tri, 1, 1) * wz;
+            if (fabsf(tu) < 1e+10f)
+                icol = vmul(icol, oracle_texture3d(s, tu, tv,
+                                                   material_rcp(s) * (((float)s->material[tri] + 0.5f) + 0.25f)));
+        }
         if (brdf == 0) {
             float r0 = 6.283184f * gpurnd4(&rndv);
             float r1 = sqrtf(gpurnd4(&rndv));

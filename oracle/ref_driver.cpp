@@ -69,6 +69,27 @@ int ref_mesh_from_arrays(int T, const float* pos, const float* nrm, const int* m
     return T;
 }
 
+// Texture coordinates [T][3][2] and the textures of the loaded mesh (TMesh::LoadOBJ, LoadPPM).
+void ref_texcoords(float* out) {
+    for (size_t i = 0; i < g_mesh->triangles.size(); i++)
+        for (int q = 0; q < 3; q++) {
+            out[i * 6 + q * 2] = g_mesh->triangles[i].texcoords[q].x;
+            out[i * 6 + q * 2 + 1] = g_mesh->triangles[i].texcoords[q].y;
+        }
+}
+
+int ref_texture_info(int m, int* width, int* height) {
+    const TMaterial& t = g_mesh->materials[m];
+    *width = t.isTextured ? t.textureWidth : 0;
+    *height = t.isTextured ? t.textureHeight : 0;
+    return t.isTextured ? 1 : 0;
+}
+
+void ref_texture(int m, unsigned char* out) {
+    const TMaterial& t = g_mesh->materials[m];
+    if (t.isTextured) std::memcpy(out, t.texture.data(), t.texture.size());
+}
+
 int ref_num_triangles() { return g_mesh ? (int)g_mesh->triangles.size() : 0; }
 int ref_num_materials() { return g_mesh ? (int)g_mesh->materials.size() : 0; }
 int ref_num_lights() { return g_mesh ? (int)g_mesh->lightsCDF.size() : 0; }

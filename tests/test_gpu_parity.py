@@ -141,3 +141,19 @@ def test_c2_terrain_full_size():
     c = r.counters()
     assert c["frame_count"] == 14 and c["eye_samples"] == 2 and c["photon_segments"] == 14 * P
     assert 0 < c["photons_emitted"] <= 14 * P
+
+
+def test_textured_materials():
+    """Textured white walls, red wall, light and mirror box (texture coordinates on every
+    triangle, images of odd sizes): the eye lookup, the light-emission lookup and the photon
+    bounce lookup with its quarter-slice blend, against the oracle."""
+    rs = np.random.default_rng(17)
+    m = cornell(64)
+    T = m.num_triangles
+    m.set_texcoords(rs.uniform(-1.5, 2.5, (T, 3, 2)).astype(np.float32))
+    for mat, (h, w) in {0: (37, 23), 1: (8, 8), 3: (5, 9), 4: (130, 70)}.items():
+        m.set_texture(mat, rs.integers(0, 256, (h, w, 3), dtype=np.uint8))
+    m.build()
+    a = m.arrays()
+    assert a["texture_size"] == 512 and np.all(np.abs(a["texcoords"][a["material"] == 2]) == np.float32(1e30))
+    _check(m, g.default_params(64, max_path_length=8), 17, "textured")
