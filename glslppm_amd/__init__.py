@@ -212,6 +212,7 @@ def library():
         "ppm_batch_num_jobs": (ctypes.c_int32, [vp]),
         "ppm_batch_device_bytes": (ctypes.c_uint64, [vp]),
         "ppm_scene_footprint": (ctypes.c_uint64, [ctypes.POINTER(Scene), ctypes.POINTER(Params)]),
+        "ppm_state_footprint": (ctypes.c_uint64, [ctypes.POINTER(Params)]),
         "ppm_batch_destroy": (None, [vp]),
         "ppm_batch_last_error": (ctypes.c_char_p, [vp]),
     }
@@ -491,6 +492,11 @@ def write_pfm(path, radiance, width, height):
 def scene_footprint(mesh, params):
     """HBM bytes a job of this scene and these parameters allocates (ppm_scene_footprint)."""
     return int(library().ppm_scene_footprint(mesh.scene, ctypes.byref(params)))
+
+
+def state_footprint(params):
+    """HBM bytes of one job's pass state (ppm_state_footprint)."""
+    return int(library().ppm_state_footprint(ctypes.byref(params)))
 
 
 class Batch:

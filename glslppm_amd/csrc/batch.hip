@@ -429,6 +429,10 @@ int ppm_batch_autotune(ppm_batch* b, uint32_t passes_per_trial, ppm_tuning* best
     return ppm_internal::autotune(start, true, trial, best);
 }
 
+uint64_t ppm_state_footprint(const ppm_params* params) {
+    return params ? ppm_internal::state_footprint(params) : 0;
+}
+
 void ppm_batch_destroy(ppm_batch* b) {
     if (!b) return;
     (void)hipSetDevice(b->device);

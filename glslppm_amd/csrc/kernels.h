@@ -14,7 +14,7 @@
 //   hit points (Q): pos, refl, nrm, rng, (tau, R), (E, N) float4; tri int; (mat, t) float2
 //   photons (P):    pos, flux, dir, rng float4 (.w of pos/flux/dir = hit normal);
 //                   state int2 (triangle, bounce | emitted << 16); slot int
-//   hash (H):       count u32, representative u32 (0xffffffff = none)
+//   hash (H):       (count u32, representative u32) per slot (0xffffffff = none)
 #pragma once
 
 #include "fmath.h"
@@ -30,8 +30,6 @@ struct DevScene {
     int num_lights;
     float light_area;
     float bsphere_x, bsphere_y, bsphere_z, bsphere_r;
-    float cube_lo_x, cube_lo_y, cube_lo_z;  // bounding cube of the scene for the ray buckets
-    float cube_cells;                        // cells per unit length
     // textures (null when no triangle is textured): per triangle (u0, v0, u1, v1), (u2, v2, 0, 0);
     // volume [M][S][S] RGBA8 of the materials' images, sampled like a GL_LINEAR / GL_REPEAT 3D
     // texture; material_rcp = 1.0f / M (the MaterialNumRcp uniform)
@@ -100,25 +98,21 @@ struct PhotonState {
     unsigned int* partial;  // [blocks] emitted counts
 };
 
-// Rays of the photons that continue this pass, compacted in queue order.
+// Rays of the photons that continue this pass, compacted in slot order: each block of 256 slots
+// counts its continuing rays (ranks from wave ballots), one scan turns the counts into first
+// positions, and every ray is placed at its block's position plus its rank.
 struct RayQueue {
     float4* org;   // [P] by photon slot: origin, flux.x
     float4* dir;   // [P] direction, flux.y
     int4* aux;     // [P] flux.z bits, bounce | emitted << 16, photon slot, 0
-    int2* key_rank;         // [P] coherence bucket (-1: photon ended) and rank inside it
-    int* perm;              // [rays] photon slots in bucket order
-    unsigned int* bucket;   // [kRayBuckets] counts, then (after the scan) first positions
+    int* rank;             // [P] rank of a continuing ray among its block's, -1 when the photon ended
+    int* perm;             // [rays] photon slots of the continuing rays
+    unsigned int* block;   // [photon blocks] continuing rays per block, then (after the scan) first positions
 };
 
-// Coherence buckets of the photon rays: cube face x 8^3 cells of the scene's bounding cube.
-// Rays of one bucket start close together and walk the same face's tree, so a wave of them
-// reads far fewer distinct nodes than a wave of arbitrary photons.
-constexpr int kRayCellBits = 3;
-constexpr int kRayBuckets = 6 << (3 * kRayCellBits);
-
 struct HashState {
-    unsigned int* count;
-    unsigned int* rep;
+    unsigned int* slots;   // [H][2]: count, representative (0xffffffff = none), one 8-byte record
+                           // per slot so that a gather reads one line per cell
     int size;          // H
     int width;         // hash_width (power of two)
     int height;

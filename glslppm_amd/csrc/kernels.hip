@@ -3,12 +3,11 @@
 //   k_eye      every max_path passes: eyeraytrace.fs + per-block max.fs/min.fs partials
 //   k_grid     the hit-point box and the grid uniforms of glslppm.cpp:292-311 (one block)
 //   k_clear_slots
-//   k_photon_gen    photontrace.fs emission/bounce; continuing rays are bucketed by cube face and
-//                   origin cell (counting sort: LDS histogram per block)
-//   k_bucket_scan   bucket offsets + sum.fs and NumPhotons (one block)
-//   k_bucket_scatter
+//   k_photon_gen    photontrace.fs emission/bounce; continuing rays ranked within their block
+//   k_bucket_scan   per-block ray offsets + sum.fs and NumPhotons (one block)
+//   k_bucket_scatter  compaction of the continuing rays
 //   k_photon_trace  photontrace.fs walk + deposit + hash.fs + scatter (atomic min of the photon
-//                   index) + correction (atomic count), one thread per ray in bucket order
+//                   index) + correction (atomic count), one thread per continuing ray
 //   k_gather   progressive.fs (8-cell gather and the N/R/tau update) + re.fs draw to RGBA8
 // Arithmetic follows the GLSL statement by statement in strict fp32 (see fmath.h).
 //
@@ -313,10 +312,8 @@ __global__ void k_init_state(Job J) {
         J.ps.slot[i] = -1;
     }
     if (i < J.h.size) {
-        J.h.count[i] = 0u;
-        J.h.rep[i] = 0xffffffffu;
+        reinterpret_cast<uint2*>(J.h.slots)[i] = make_uint2(0u, 0xffffffffu);
     }
-    if (i < kRayBuckets) J.rq.bucket[i] = 0u;
 }
 
 namespace {
@@ -361,10 +358,8 @@ PPM_DEV void pass_begin(const Job& J) {
 PPM_DEV void clear_slots(const Job& J, int blk) {
     if (!J.g->active) return;
     const int i = blk * kBlock + threadIdx.x;
-    if (i < kRayBuckets) J.rq.bucket[i] = 0u;
     if (i < J.h.size) {
-        J.h.count[i] = 0u;
-        J.h.rep[i] = 0xffffffffu;
+        reinterpret_cast<uint2*>(J.h.slots)[i] = make_uint2(0u, 0xffffffffu);
     }
 }
 
@@ -647,12 +642,13 @@ PPM_DEV void ibl_sample(const DevScene& s, float4& rndv, f3& flux, f3& dir, f3& 
 //
 // k_photon_gen runs one thread per photon slot: emission or bounce with Russian roulette
 // (photontrace.fs:466-686).  A terminated photon writes its final state at once; a continuing
-// one writes its ray and is counted into a coherence bucket (cube face of the direction x cell of
-// the origin).  k_bucket_scan turns the counts into bucket offsets, k_bucket_scatter places the
-// rays in bucket order, and k_photon_trace runs one thread per continuing ray in that order: the
-// walk, then the deposit (photontrace.fs:690-710), hash.fs, scatter and correction.  Each
-// photon's arithmetic is independent of which lane runs it and the slot table only sees
-// commutative atomics, so the result equals one thread per slot.
+// one writes its ray and its rank among its block's continuing rays.  k_bucket_scan turns the
+// per-block counts into first positions, k_bucket_scatter compacts the rays in slot order, and
+// k_photon_trace runs one thread per continuing ray: the walk, then the deposit
+// (photontrace.fs:690-710), hash.fs, scatter and correction.  (Sorting the rays by direction and
+// origin cell instead made the postponed-leaf walk slower: its waves then reach their leaves in
+// lock step.)  Each photon's arithmetic is independent of which lane runs it and the slot table
+// only sees commutative atomics, so the result equals one thread per slot.
 
 namespace {
 
@@ -829,28 +825,14 @@ PPM_DEV void photon_finish(const DevScene& s, const PhotonParams& pp, const Phot
         // a slot past the last row lands outside the hash viewport and is clipped
         if (sf < h.num_f) {
             slot = (int)sf;
-            atomicAdd(h.count + slot, 1u);
-            if (p < pp.rep_limit) atomicMin(h.rep + slot, (unsigned int)p);
+            atomicAdd(h.slots + 2 * slot, 1u);
+            if (p < pp.rep_limit) atomicMin(h.slots + 2 * slot + 1, (unsigned int)p);
         }
     }
     ps.slot[p] = slot;
 }
 
-// Cube face and origin cell of a ray, interleaved as face | morton(cell).
-PPM_DEV int ray_bucket(const DevScene& s, f3 org, f3 dir) {
-    const int n = 1 << kRayCellBits;
-    int c[3];
-    const float o[3] = {(org.x - s.cube_lo_x) * s.cube_cells, (org.y - s.cube_lo_y) * s.cube_cells,
-                        (org.z - s.cube_lo_z) * s.cube_cells};
-    for (int k = 0; k < 3; k++) {
-        const int v = (int)floorf(o[k]);
-        c[k] = v < 0 ? 0 : (v > n - 1 ? n - 1 : v);
-    }
-    int code = 0;
-    for (int b = 0; b < kRayCellBits; b++)
-        code |= (((c[0] >> b) & 1) << (3 * b)) | (((c[1] >> b) & 1) << (3 * b + 1)) | (((c[2] >> b) & 1) << (3 * b + 2));
-    return (cube_face(dir) << (3 * kRayCellBits)) | code;
-}
+PPM_DEV unsigned long long lanes_below(int lane) { return lane == 0 ? 0ull : (~0ull >> (64 - lane)); }
 
 PPM_DEV void photon_gen_blk(const Job& J, int blk) {
     if (!J.g->active) return;
@@ -864,66 +846,60 @@ PPM_DEV void photon_gen_blk(const Job& J, int blk) {
     f3 org, dir, flux;
     int out_bits = 0;
     if (p < pp.count) live = photon_gen(s, pp, ps, p, org, dir, flux, out_bits, emitted);
-    int key = -1;
     if (live) {
         rq.org[p] = make_float4(org.x, org.y, org.z, flux.x);
         rq.dir[p] = make_float4(dir.x, dir.y, dir.z, flux.y);
         rq.aux[p] = make_int4(__float_as_int(flux.z), out_bits, p, 0);
-        key = ray_bucket(s, org, dir);
     }
-    // counting sort of the continuing rays by coherence bucket: ranks inside the block from an
-    // LDS histogram, then one global atomic per non-empty bucket per block (spread over the
-    // buckets; no counter is shared by all blocks)
-    __shared__ unsigned int hist[kRayBuckets];
-    __shared__ unsigned int red[kWaves];
-    for (int b = threadIdx.x; b < kRayBuckets; b += kBlock) hist[b] = 0u;
-    __syncthreads();
-    unsigned int local = 0;
-    if (live) local = atomicAdd(&hist[key], 1u);
-    __syncthreads();
-    for (int b = threadIdx.x; b < kRayBuckets; b += kBlock) {
-        const unsigned int c = hist[b];
-        if (c) hist[b] = atomicAdd(&rq.bucket[b], c);
+    // rank of each continuing ray in its block, and per-block counts of continuing rays and of
+    // emitted flags (sum.fs), added up by k_bucket_scan
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const unsigned long long m = __ballot(live);
+    const unsigned int es = wave_sum(emitted);
+    __shared__ unsigned int wl[kWaves], we[kWaves];
+    if (lane == 0) {
+        wl[wave] = (unsigned int)__popcll(m);
+        we[wave] = es;
     }
     __syncthreads();
-    if (p < pp.count) rq.key_rank[p] = make_int2(key, live ? (int)(hist[key] + local) : 0);
-    // sum.fs: one partial per block, added up by k_bucket_scan
-    const unsigned int ws = wave_sum(emitted);
-    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = ws;
-    __syncthreads();
+    unsigned int before = 0;
+    for (int w = 0; w < wave; w++) before += wl[w];
+    if (p < pp.count) rq.rank[p] = live ? (int)(before + (unsigned int)__popcll(m & lanes_below(lane))) : -1;
     if (threadIdx.x == 0) {
-        unsigned int tot = 0;
-        for (int w = 0; w < kWaves; w++) tot += red[w];
-        ps.partial[blk] = tot;
+        unsigned int nl = 0, ne = 0;
+        for (int w = 0; w < kWaves; w++) {
+            nl += wl[w];
+            ne += we[w];
+        }
+        rq.block[blk] = nl;
+        ps.partial[blk] = ne;
     }
 }
 
-// Exclusive scan of the bucket counts into first positions, the number of continuing rays, and
-// NumPhotons += int(sum of emitted flags) (glslppm.cpp:396-397); one block of 1024 threads.
+// Exclusive scan of the per-block ray counts into first positions, the number of continuing
+// rays, and NumPhotons += int(sum of emitted flags) (glslppm.cpp:396-397); one block of 1024
+// threads, each owning a contiguous run of blocks.
 PPM_DEV void bucket_scan(const Job& J) {
     DevGlobals* g = J.g;
     if (!g->active) return;
     const RayQueue& rq = J.rq;
     const unsigned int* partial = J.ps.partial;
-    const int n_partial = J.photon_blocks;
-    constexpr int per = (kRayBuckets + 1023) / 1024;
+    const int n = J.photon_blocks;
+    const int per = (n + 1023) / 1024;
     __shared__ unsigned int wsum[16];
     __shared__ unsigned int esum[16];
     const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
-    unsigned int v[per];
-    unsigned int local = 0;
-    for (int k = 0; k < per; k++) {
-        const int b = t * per + k;
-        v[k] = b < kRayBuckets ? rq.bucket[b] : 0u;
-        local += v[k];
+    const int lo = t * per, hi = min(n, lo + per);
+    unsigned int local = 0, e = 0;
+    for (int b = lo; b < hi; b++) {
+        local += rq.block[b];
+        e += partial[b];
     }
     unsigned int inc = local;
     for (int o = 1; o < 64; o <<= 1) {
         const unsigned int y = __shfl_up(inc, o, 64);
         if (lane >= o) inc += y;
     }
-    unsigned int e = 0;
-    for (int b = t; b < n_partial; b += 1024) e += partial[b];
     e = wave_sum(e);
     if (lane == 63) wsum[wave] = inc;
     if (lane == 0) esum[wave] = e;
@@ -931,10 +907,10 @@ PPM_DEV void bucket_scan(const Job& J) {
     unsigned int before = 0;
     for (int w = 0; w < wave; w++) before += wsum[w];
     unsigned int run = before + inc - local;
-    for (int k = 0; k < per; k++) {
-        const int b = t * per + k;
-        if (b < kRayBuckets) rq.bucket[b] = run;
-        run += v[k];
+    for (int b = lo; b < hi; b++) {
+        const unsigned int c = rq.block[b];
+        rq.block[b] = run;
+        run += c;
     }
     if (t == 0) {
         unsigned int total = 0, emitted = 0;
@@ -949,14 +925,14 @@ PPM_DEV void bucket_scan(const Job& J) {
     }
 }
 
-// Places every continuing ray at its bucket position.
+// Places every continuing ray at its block's first position plus its rank.
 PPM_DEV void bucket_scatter(const Job& J, int blk) {
     if (!J.g->active) return;
     const RayQueue& rq = J.rq;
     const int p = blk * kBlock + threadIdx.x;
     if (p >= J.pp.count) return;
-    const int2 kr = rq.key_rank[p];
-    if (kr.x >= 0) rq.perm[rq.bucket[kr.x] + (unsigned int)kr.y] = p;
+    const int r = rq.rank[p];
+    if (r >= 0) rq.perm[rq.block[blk] + (unsigned int)r] = p;
 }
 
 PPM_DEV void photon_trace(const Job& J, int blk) {
@@ -1042,9 +1018,10 @@ PPM_DEV void gather(const Job& J, int blk) {
                     const float sf = cell_hash(cell, gs, hsc, h.num_f);
                     // texel fetch clamps a slot past the last row to the start of that row
                     const int slot = sf < h.num_f ? (int)sf : clamp_slot;
-                    const unsigned int cnt = h.count[slot];
+                    const uint2 rec = reinterpret_cast<const uint2*>(h.slots)[slot];
+                    const unsigned int cnt = rec.x;
                     if (cnt == 0u) continue;  // adds exactly +0 to flux and count
-                    unsigned int rep = h.rep[slot];
+                    unsigned int rep = rec.y;
                     if (rep == 0xffffffffu) rep = 0u;  // cleared slot reads photon texel (0, 0)
                     const f3 ppos = xyz(ps.pos[rep]);
                     const f3 rmin = cell / gs + bbmin;

@@ -424,7 +424,7 @@ void set_launch_shape(ppm_ctx* c, int trace_block, int wave_shift) {
 }
 int photon_blocks(const ppm_params* p) { return (p->photon_width * p->photon_height + 255) / 256; }
 int clear_blocks(const ppm_params* p) {
-    return (std::max(p->hash_width * p->hash_height, kRayBuckets) + 255) / 256;
+    return (p->hash_width * p->hash_height + 255) / 256;
 }
 
 size_t state_footprint(const ppm_params* p) {
@@ -435,10 +435,10 @@ size_t state_footprint(const ppm_params* p) {
         Q * 16, Q * 16, Q * 16, Q * 16, Q * 16, Q * 16, Q * 4, Q * 8, Q * 4, (size_t)max_tile_blocks(p) * 24,
         // photons: pos, flux, dir, rng, state, slot, emitted partials
         P * 16, P * 16, P * 16, P * 16, P * 8, P * 4, (size_t)photon_blocks(p) * 4,
-        // rays: org, dir, aux, key_rank, perm, buckets
-        P * 16, P * 16, P * 16, P * 8, P * 4, (size_t)kRayBuckets * 4,
-        // hash count, representative; globals; resolve targets
-        H * 4, H * 4, sizeof(DevGlobals), Q * 16, Q * 16};
+        // rays: org, dir, aux, rank, perm, per-block counts
+        P * 16, P * 16, P * 16, P * 4, P * 4, (size_t)photon_blocks(p) * 4,
+        // hash (count, representative) records; globals; resolve targets
+        H * 8, sizeof(DevGlobals), Q * 16, Q * 16};
     size_t tot = 0;
     for (size_t b : sizes) tot += Arena::round_size(b ? b : 16);
     return tot;
@@ -551,11 +551,6 @@ int upload_scene(const PackedScene& ps, Arena* arena, hipStream_t up, hipStream_
     sc.bsphere_y = ps.bsphere[1];
     sc.bsphere_z = ps.bsphere[2];
     sc.bsphere_r = ps.bsphere[3];
-    // bounding cube of the scene for the photon ray buckets
-    sc.cube_lo_x = ps.bsphere[0] - ps.bsphere[3];
-    sc.cube_lo_y = ps.bsphere[1] - ps.bsphere[3];
-    sc.cube_lo_z = ps.bsphere[2] - ps.bsphere[3];
-    sc.cube_cells = ps.bsphere[3] > 0.0f ? (float)(1 << kRayCellBits) / (2.0f * ps.bsphere[3]) : 0.0f;
     sc.tri_tex = (const float4*)d_tex;
     sc.volume = (const uchar4*)d_vol;
     sc.tex_size = ps.tex_size;
@@ -635,9 +630,9 @@ int create_context(ppm_ctx* c, std::shared_ptr<SceneDev> scene, const ppm_camera
         (rc = dev_alloc(c, &ps.flux, P)) || (rc = dev_alloc(c, &ps.dir, P)) || (rc = dev_alloc(c, &ps.rng, P)) ||
         (rc = dev_alloc(c, &ps.state, P)) || (rc = dev_alloc(c, &ps.slot, P)) ||
         (rc = dev_alloc(c, &ps.partial, (size_t)J.photon_blocks)) || (rc = dev_alloc(c, &rq.org, P)) ||
-        (rc = dev_alloc(c, &rq.dir, P)) || (rc = dev_alloc(c, &rq.aux, P)) || (rc = dev_alloc(c, &rq.key_rank, P)) ||
-        (rc = dev_alloc(c, &rq.perm, P)) || (rc = dev_alloc(c, &rq.bucket, (size_t)kRayBuckets)) ||
-        (rc = dev_alloc(c, &J.h.count, H)) || (rc = dev_alloc(c, &J.h.rep, H)) || (rc = dev_alloc(c, &J.g, 1)) ||
+        (rc = dev_alloc(c, &rq.dir, P)) || (rc = dev_alloc(c, &rq.aux, P)) || (rc = dev_alloc(c, &rq.rank, P)) ||
+        (rc = dev_alloc(c, &rq.perm, P)) || (rc = dev_alloc(c, &rq.block, (size_t)J.photon_blocks)) ||
+        (rc = dev_alloc(c, &J.h.slots, 2 * H)) || (rc = dev_alloc(c, &J.g, 1)) ||
         (rc = dev_alloc(c, &c->resolve_lin, Q)) || (rc = dev_alloc(c, &c->resolve_map, Q)))
         return rc;
     J.h.size = c->H;
@@ -656,7 +651,7 @@ int create_context(ppm_ctx* c, std::shared_ptr<SceneDev> scene, const ppm_camera
     std::memcpy(g0.xs, xs, sizeof(xs));
     g0.passes_left = passes_left;
     HIP_TRY(c, hipMemcpyAsync(J.g, &g0, sizeof(g0), hipMemcpyHostToDevice, up));
-    const int n = (int)std::max(std::max(Q, P), std::max(H, (size_t)kRayBuckets));
+    const int n = (int)std::max(std::max(Q, P), H);
     hipLaunchKernelGGL(k_init_state, dim3((n + 255) / 256), dim3(256), 0, up, J);
     HIP_TRY(c, hipGetLastError());
     HIP_TRY(c, hipStreamSynchronize(up));
@@ -893,8 +888,15 @@ int ppm_readback(ppm_ctx* c, ppm_buf which, void* dst, size_t bytes) {
             return PPM_OK;
         }
         case PPM_BUF_PHOTON_SLOT: return d2h(J.ps.slot, P * 4);
-        case PPM_BUF_SLOT_COUNT: return d2h(J.h.count, H * 4);
-        case PPM_BUF_SLOT_REP: return d2h(J.h.rep, H * 4);
+        case PPM_BUF_SLOT_COUNT:
+        case PPM_BUF_SLOT_REP: {
+            std::vector<uint32_t> rec(2 * H);
+            HIP_TRY(c, hipMemcpy(rec.data(), J.h.slots, H * 8, hipMemcpyDeviceToHost));
+            uint32_t* o = (uint32_t*)dst;
+            const size_t k = which == PPM_BUF_SLOT_COUNT ? 0 : 1;
+            for (size_t i = 0; i < H; i++) o[i] = rec[2 * i + k];
+            return PPM_OK;
+        }
         case PPM_BUF_COUNTERS: {
             ppm_counters k;
             std::memset(&k, 0, sizeof(k));

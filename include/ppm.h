@@ -281,8 +281,10 @@ int32_t ppm_batch_num_jobs(const ppm_batch* b);
 /* HBM held by the batch's pool (jobs in use plus cached blocks). */
 uint64_t ppm_batch_device_bytes(const ppm_batch* b);
 
-/* HBM a job of this scene and these parameters allocates in a batch. */
+/* HBM a job of this scene and these parameters allocates in a batch (scene plus pass state), and
+ * the pass state alone (what each further job on an already resident scene adds). */
 uint64_t ppm_scene_footprint(const ppm_scene* scene, const ppm_params* params);
+uint64_t ppm_state_footprint(const ppm_params* params);
 
 /* Launch shapes of the whole batch (ppm_tuning); ppm_batch_autotune times real batched passes,
  * which count against the jobs' budgets. */

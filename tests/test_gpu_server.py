@@ -83,9 +83,11 @@ def test_jobs_in_the_mix_equal_jobs_alone():
 def test_admission_waits_for_memory_and_keeps_results():
     meshes = scenes()
     jobs = job_list()
-    one = max(g.scene_footprint(meshes[si], p) for (si, _, p, _) in jobs)
-    # room for two jobs at a time (with both scenes resident), not three
-    budget = int(2.2 * one)
+    states = [g.state_footprint(p) for (_, _, p, _) in jobs]
+    scene_bytes = sum(g.scene_footprint(m, jobs[0][2]) - g.state_footprint(jobs[0][2]) for m in meshes)
+    # room for both scenes and two jobs at a time, not three
+    budget = scene_bytes + 2 * max(states) + min(states) // 2
+    assert scene_bytes + 3 * min(states) > budget
     srv = g.Server(hbm_budget=budget, passes_per_step=2)
     sids = [srv.add_scene(m) for m in meshes]
     ids = [(srv.submit(sids[si], p, n, camera=g.make_camera(*cam)), si, cam, p, n) for (si, cam, p, n) in jobs]
