@@ -50,10 +50,7 @@ PPM_DEV bool box_test(f3 rcp, f3 t0, float4 a, float4 b, float ht, float& tmin_o
 // Triangle test of a leaf (eyeraytrace.fs:118-147, photontrace.fs:174-205).  Eye rays cull back
 // faces on their first segment; photon rays skip light triangles.
 template <bool PHOTON>
-PPM_DEV void tri_test(const DevScene& s, f3 org, f3 dir, bool cull, int tri, Hit& h) {
-    const float4 q0 = s.tri_isect[3 * tri + 0];
-    const float4 q1 = s.tri_isect[3 * tri + 1];
-    const float4 q2 = s.tri_isect[3 * tri + 2];
+PPM_DEV void tri_test(float4 q0, float4 q1, float4 q2, f3 org, f3 dir, bool cull, int tri, Hit& h) {
     const f3 e0 = xyz(q1), e1 = xyz(q2);
     const f3 pv = cross3(dir, e1);
     const float det = dot3(e0, pv);
@@ -142,8 +139,23 @@ PPM_DEV void walk(const DevScene& s, int face, f3 org, f3 dir, f3 rcp, f3 t0, bo
             }
         }
         if (__ballot(nq > 0) == 0ull) break;
-        if (nq > 0 && m0 <= h.t) tri_test<PHOTON>(s, org, dir, cull, q0, h);
-        if (nq > 1 && m1 <= h.t) tri_test<PHOTON>(s, org, dir, cull, q1, h);
+        // both queued triangles are fetched before the first test: the second leaf's re-check
+        // needs the first test's result, its data does not
+        float4 x0 = make_float4(0.0f, 0.0f, 0.0f, 0.0f), x1 = x0, x2 = x0, y0 = x0, y1 = x0, y2 = x0;
+        if (nq > 0) {
+            const float4* T = s.tri_isect + 3 * q0;
+            x0 = T[0];
+            x1 = T[1];
+            x2 = T[2];
+        }
+        if (nq > 1) {
+            const float4* T = s.tri_isect + 3 * q1;
+            y0 = T[0];
+            y1 = T[1];
+            y2 = T[2];
+        }
+        if (nq > 0 && m0 <= h.t) tri_test<PHOTON>(x0, x1, x2, org, dir, cull, q0, h);
+        if (nq > 1 && m1 <= h.t) tri_test<PHOTON>(y0, y1, y2, org, dir, cull, q1, h);
         nq = 0;
     }
 }
@@ -1066,6 +1078,17 @@ PPM_DEV void gather(const Job& J, int blk) {
         const BlockRef m = map[blockIdx.x];                                                         \
         body(jobs[m.job], m.block);                                                                 \
     }
+// The eye kernel's bounce loop keeps ~100 VGPRs live; asking for 5 waves per SIMD fits it in 96
+// without spilling, one wave more than it would get, which shortens its dependent-load stalls.
+#define PPM_JOB_KERNEL_WAVES(name, body, threads, waves)                                          \
+    __global__ __launch_bounds__(threads) __attribute__((amdgpu_waves_per_eu(waves))) void name(Job J) { \
+        body(J, (int)blockIdx.x);                                                                   \
+    }                                                                                               \
+    __global__ __launch_bounds__(threads) __attribute__((amdgpu_waves_per_eu(waves))) void name##_batch( \
+        const Job* __restrict__ jobs, const BlockRef* __restrict__ map) {                           \
+        const BlockRef m = map[blockIdx.x];                                                         \
+        body(jobs[m.job], m.block);                                                                 \
+    }
 #define PPM_JOB_KERNEL1(name, body, threads)                                                      \
     __global__ __launch_bounds__(threads) void name(Job J) { body(J); }                            \
     __global__ __launch_bounds__(threads) void name##_batch(const Job* __restrict__ jobs) {        \
@@ -1073,7 +1096,7 @@ PPM_DEV void gather(const Job& J, int blk) {
     }
 
 PPM_JOB_KERNEL(k_clear_slots, clear_slots, kBlock)
-PPM_JOB_KERNEL(k_eye, eye, kBlock)
+PPM_JOB_KERNEL_WAVES(k_eye, eye, kBlock, 5)
 PPM_JOB_KERNEL1(k_grid, grid, 1024)
 PPM_JOB_KERNEL(k_photon_gen, photon_gen_blk, kBlock)
 PPM_JOB_KERNEL1(k_bucket_scan, bucket_scan, 1024)
