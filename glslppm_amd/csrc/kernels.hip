@@ -1028,8 +1028,9 @@ PPM_DEV void gather(const Job& J, int blk) {
                 for (int ix = 0; ix <= 1; ix++) {
                     const f3 cell = mk3(ii.x + (float)ix, ii.y + (float)iy, ii.z + (float)iz);
                     const float sf = cell_hash(cell, gs, hsc, h.num_f);
-                    // texel fetch clamps a slot past the last row to the start of that row
-                    const int slot = sf < h.num_f ? (int)sf : clamp_slot;
+                    // texel fetch clamps a slot past the last row to the start of that row; a
+                    // NaN slot (zero-extent grid) clamps to texel (0, 0)
+                    const int slot = sf < h.num_f ? (int)sf : (sf >= h.num_f ? clamp_slot : 0);
                     const uint2 rec = reinterpret_cast<const uint2*>(h.slots)[slot];
                     const unsigned int cnt = rec.x;
                     if (cnt == 0u) continue;  // adds exactly +0 to flux and count

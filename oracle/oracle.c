@@ -902,9 +902,10 @@ static void accumulate(Oracle* o, v3 qpos, v3 qnrm, float qr, v3 cell, v3* flux,
     float h = oracle_hash(hidx, o->grid_scale, o->hash_scale, (float)o->H);
     float u = (gl_mod(h, (float)o->Hw) + 0.5f) * (1.0f / (float)o->Hw);
     float v = (floorf(h * (1.0f / (float)o->Hw)) + 0.5f) * (1.0f / (float)o->Hh);
-    /* GL_CLAMP + GL_NEAREST */
-    u = u < 0.0f ? 0.0f : (u > 1.0f ? 1.0f : u);
-    v = v < 0.0f ? 0.0f : (v > 1.0f ? 1.0f : v);
+    /* GL_CLAMP + GL_NEAREST; the clamp rejects NaN like GLSL min/max, so a NaN coordinate (a
+     * zero-extent hit-point box gives grid_scale = inf and NaN cells) reads texel (0, 0) */
+    u = gl_min(gl_max(u, 0.0f), 1.0f);
+    v = gl_min(gl_max(v, 0.0f), 1.0f);
     int x = (int)floorf(u * (float)o->Hw), y = (int)floorf(v * (float)o->Hh);
     if (x > o->Hw - 1) x = o->Hw - 1;
     if (y > o->Hh - 1) y = o->Hh - 1;

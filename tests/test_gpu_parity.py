@@ -157,3 +157,37 @@ def test_textured_materials():
     a = m.arrays()
     assert a["texture_size"] == 512 and np.all(np.abs(a["texcoords"][a["material"] == 2]) == np.float32(1e30))
     _check(m, g.default_params(64, max_path_length=8), 17, "textured")
+
+
+def test_thin_lens_camera():
+    """Aperture > 0: the lens sample of eyeraytrace.fs:405-410 moves the ray origin."""
+    m = g.Mesh.generate(g.SCENE_SPHERES, seed=13, count=6, detail=2)
+    m.set_camera((0.05, 0.1, 1.5), (0, 0, 0), 48, 48, 50.0).build()
+    _check(m, g.default_params(48, hash_width=64, hash_height=32, max_path_length=6, aperture=0.08,
+                             focal_length=1.4), 9, "thin-lens")
+
+
+def test_degenerate_sizes():
+    """One pixel, one photon, one hash slot; a one-pixel image over a 4x4 hash (its hit-point box
+    has zero extent, so grid_scale is inf and every gather cell is NaN); 3x1 / 1x5 strips."""
+    _check(cornell(1), g.default_params(1, max_path_length=4), 9, "1x1")
+    _check(cornell(1), g.default_params(1, photon_width=2, photon_height=2, hash_width=4, hash_height=4,
+                                        max_path_length=4), 9, "1x1-hash4x4")
+    m = g.Mesh.generate(g.SCENE_CORNELL).set_camera((0, 0, 1.667), (0, 0, 0), 3, 1, 45.0).build()
+    p = g.default_params(4, image_width=3, image_height=1, photon_width=1, photon_height=5, hash_width=2,
+                         hash_height=3, max_path_length=3)
+    _check(m, p, 7, "strips")
+
+
+def test_smallest_scenes():
+    """A lone light triangle seen from below (photons leave the scene, none lands) and the same
+    light over a floor."""
+    light = [(1.0, 1.0, 1.0), 1.7, -1, 0.0]
+    floor = [(0.7, 0.7, 0.7), 1.7, 0, 0.0]
+    tri_light = np.array([[[-0.2, 0.4, -0.2], [0.2, 0.4, -0.2], [0.2, 0.4, 0.2]]], np.float32)  # faces -y
+    m = g.Mesh.from_arrays(tri_light, [light], [0]).set_camera((0, -0.5, 1.0), (0, 0.4, 0), 16, 16, 60.0).build()
+    _check(m, g.default_params(16, max_path_length=4), 5, "one-triangle")
+    tri_floor = np.array([[[-1, -0.5, -1], [1, -0.5, 1], [1, -0.5, -1]]], np.float32)  # faces +y
+    m = g.Mesh.from_arrays(np.concatenate([tri_light, tri_floor]), [light, floor], [0, 1])
+    m.set_camera((0, 0.2, 1.5), (0, -0.5, 0), 16, 16, 60.0).build()
+    _check(m, g.default_params(16, max_path_length=4), 9, "two-triangles")
