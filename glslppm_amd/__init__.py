@@ -79,6 +79,15 @@ class Tuning(ctypes.Structure):
     _fields_ = [("trace_block", ctypes.c_int32), ("tile_wave_width", ctypes.c_int32),
                 ("passes_per_graph", ctypes.c_uint32), ("ms_per_pass", ctypes.c_float)]
 
+    def update(self, **kw):
+        for k, v in kw.items():
+            if v is None:
+                continue
+            if k == "ms_per_pass" or k not in dict(self._fields_):
+                raise PPMError("unknown tuning field %s" % k)
+            setattr(self, k, int(v))
+        return self
+
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
 
@@ -403,16 +412,12 @@ class Renderer:
         self._check(library().ppm_get_tuning(self._h, ctypes.byref(t)))
         return t.as_dict()
 
-    def set_tuning(self, trace_block=None, tile_wave_width=None, passes_per_graph=None):
+    def set_tuning(self, **fields):
+        """Changes the named launch-shape fields (trace_block, tile_wave_width, passes_per_graph);
+        the others keep their values."""
         t = Tuning()
         self._check(library().ppm_get_tuning(self._h, ctypes.byref(t)))
-        if trace_block is not None:
-            t.trace_block = int(trace_block)
-        if tile_wave_width is not None:
-            t.tile_wave_width = int(tile_wave_width)
-        if passes_per_graph is not None:
-            t.passes_per_graph = int(passes_per_graph)
-        self._check(library().ppm_set_tuning(self._h, ctypes.byref(t)))
+        self._check(library().ppm_set_tuning(self._h, ctypes.byref(t.update(**fields))))
 
     def autotune(self, passes_per_trial=0):
         """Times real passes under each launch shape and keeps the fastest; returns it."""
@@ -560,16 +565,12 @@ class Batch:
         self._check(library().ppm_batch_get_tuning(self._h, ctypes.byref(t)))
         return t.as_dict()
 
-    def set_tuning(self, trace_block=None, tile_wave_width=None, passes_per_graph=None):
+    def set_tuning(self, **fields):
+        """Changes the named launch-shape fields (trace_block, tile_wave_width, passes_per_graph);
+        the others keep their values."""
         t = Tuning()
         self._check(library().ppm_batch_get_tuning(self._h, ctypes.byref(t)))
-        if trace_block is not None:
-            t.trace_block = int(trace_block)
-        if tile_wave_width is not None:
-            t.tile_wave_width = int(tile_wave_width)
-        if passes_per_graph is not None:
-            t.passes_per_graph = int(passes_per_graph)
-        self._check(library().ppm_batch_set_tuning(self._h, ctypes.byref(t)))
+        self._check(library().ppm_batch_set_tuning(self._h, ctypes.byref(t.update(**fields))))
 
     def autotune(self, passes_per_trial=0):
         """Times real passes under each launch shape and keeps the fastest; returns it."""

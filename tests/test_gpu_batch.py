@@ -167,3 +167,13 @@ def test_autotune_keeps_results_and_picks_a_valid_shape():
     for (label, mm, pp, _), j in zip(specs, ids):
         check_equal_alone(JobView(bt, j), mm, pp, n, label)
     bt.close()
+
+
+def test_tuning_out_of_range_is_refused():
+    m = spheres(5, 4, 2, 32, 32)
+    r = g.Renderer(m, g.default_params(32, max_path_length=4))
+    for bad in (dict(trace_block=96), dict(trace_block=1024), dict(tile_wave_width=3), dict(passes_per_graph=2),
+                dict(ms_per_pass=1.0), dict(no_such_field=1)):
+        with pytest.raises(g.PPMError):
+            r.set_tuning(**bad)
+    assert r.tuning()["trace_block"] == 256
