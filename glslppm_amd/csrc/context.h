@@ -48,7 +48,8 @@ private:
 
 // A scene in the form the kernels read (CGPURT::CreateTextures, gpurt.cpp:47-552), on the host.
 struct PackedScene {
-    std::vector<float4> nodes, isect, shade, mats, tex;
+    std::vector<uint4> nodes;
+    std::vector<float4> isect, shade, mats, tex;
     std::vector<uchar4> volume;
     int tex_size = 0;
     std::vector<int> light_tri;

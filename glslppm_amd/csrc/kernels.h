@@ -2,10 +2,13 @@
 //
 // HBM layout (all arrays 16-byte aligned, structure-of-arrays so that a wave's 64 lanes read
 // 1 KiB contiguous per field):
-//   scene:  nodes[6][N+1][2] float4 per cube face, in that face's pre-order (+1 padding node):
-//                                  (min.xyz, miss position), (max.xyz, leaf triangle or -1);
-//                                  the hit link is the next position, so it is not stored.
-//           tri_isect[T][3] float4 (v0, brdf), (v1 - v0, material), (v2 - v0, 0)
+//   scene:  nodes[6][N+1] uint4 per cube face, in that face's pre-order (+1 padding node): the box
+//                                  as six binary16 rounded outward (lo.x lo.y | lo.z hi.x |
+//                                  hi.y hi.z), then the miss position of an interior node or
+//                                  ~triangle of a leaf (whose miss position is the next one); the
+//                                  hit link is the next position, so it is not stored.
+//           tri_isect[T][5] float4 (v0, brdf), (v1 - v0, material), (v2 - v0, 0), exact leaf box
+//                                  (min.xyz, 0), (max.xyz, 0)
 //           tri_shade[T][6] float4 (v0, n0.x) (v1, n0.y) (v2, n0.z) (n1, n2.x)
 //                                  (n2.y, n2.z, gn.x, gn.y) (gn.z, material, 0, 0)
 //           mats[M][2] float4      (rgb, eta), (brdf, g, 0, 0)
@@ -20,7 +23,7 @@
 #include "fmath.h"
 
 struct DevScene {
-    const float4* __restrict__ nodes;
+    const uint4* __restrict__ nodes;
     const float4* __restrict__ tri_isect;
     const float4* __restrict__ tri_shade;
     const float4* __restrict__ mats;

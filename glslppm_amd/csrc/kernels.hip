@@ -36,9 +36,9 @@ struct Hit {
     float bx, by, bz;  // Barycentric = uvt.ywx
 };
 
-// Slab test of one node's box (eyeraytrace.fs:92-116, photontrace.fs:148-172) against the
-// closest hit so far; tmin is returned for the deferred re-check of leaves.
-PPM_DEV bool box_test(f3 rcp, f3 t0, float4 a, float4 b, float ht, float& tmin_out) {
+// Slab test of a box (eyeraytrace.fs:92-116, photontrace.fs:148-172) against the closest hit so
+// far; tmin is returned for the deferred re-check of leaves.
+PPM_DEV bool box_test(f3 rcp, f3 t0, f3 a, f3 b, float ht, float& tmin_out) {
     const float lx = rcp.x * a.x + t0.x, ly = rcp.y * a.y + t0.y, lz = rcp.z * a.z + t0.z;
     const float hx = rcp.x * b.x + t0.x, hy = rcp.y * b.y + t0.y, hz = rcp.z * b.z + t0.z;
     const float tmin = max1(max1(min1(lx, hx), min1(ly, hy)), min1(lz, hz));
@@ -76,6 +76,15 @@ PPM_DEV void tri_test(float4 q0, float4 q1, float4 q2, f3 org, f3 dir, bool cull
 // Leaves a lane may hold before its triangle tests run.
 constexpr int kLeafQueue = 2;
 
+PPM_DEV float half_lo(unsigned int w) { return (float)__builtin_bit_cast(_Float16, (unsigned short)(w & 0xffffu)); }
+PPM_DEV float half_hi(unsigned int w) { return (float)__builtin_bit_cast(_Float16, (unsigned short)(w >> 16)); }
+
+// The outward-rounded box of a node record.
+PPM_DEV void node_box(uint4 r, f3& lo, f3& hi) {
+    lo = mk3(half_lo(r.x), half_hi(r.x), half_lo(r.y));
+    hi = mk3(half_hi(r.y), half_lo(r.z), half_hi(r.z));
+}
+
 // Closest hit over the threaded BVH of one cube face (eyeraytrace.fs:67-254,
 // photontrace.fs:123-218): box tests walk the pre-order (hit: next position, miss: the stored
 // link); a leaf whose box is hit gets its triangle tested.
@@ -84,20 +93,27 @@ constexpr int kLeafQueue = 2;
 // rounded divisions each) run with many lanes at once instead of whenever any lane meets a leaf.
 // In the box phase a lane that meets a leaf with a hit box queues it, with the box's entry
 // distance tmin, and walks on; the wave leaves the box phase when no lane can step or when more
-// than a quarter of its live lanes hold full queues.  The leaf phase then tests each lane's
-// queued leaves in the order they were met, skipping a leaf whose tmin now exceeds the closest
-// hit.  Boxes walked while a leaf was queued were tested against an older, larger closest-hit
-// distance, so the lane may enter boxes the reference skips; it never skips one the reference
-// enters.  A leaf inside a box the reference skips has tmin >= that box's tmin (nested boxes,
-// slab test monotone in the bounds, including the NaN axes of zero direction components), so the
-// re-check drops it; every other leaf is tested with the closest hit the reference has at that
-// point.  The triangle tests, their order and so every result, ties included, are the
-// reference's.  Nodes i and i + 1 are fetched together (55% of steps continue at i + 1); each face
-// array ends with a padding node, so the second fetch stays in bounds.
+// than a quarter of its live lanes hold full queues.  The leaf phase then takes each lane's
+// queued leaves in the order they were met: a leaf whose tmin now exceeds the closest hit is
+// dropped, the others get the reference's slab test of their exact box with the current closest
+// hit, then their triangle test.
+//
+// Box-phase boxes are the node records' binary16 boxes, rounded outward.  A slab value
+// rcp * b + t0 is monotone in b, and rounding outward keeps the sign of each bound (which decides
+// the NaN axes of zero direction components), so a rounded box passes whenever the exact one
+// does, with a tmin no larger.  Boxes walked while a leaf was queued were also tested against an
+// older, larger closest-hit distance.  So the lane may enter boxes the reference skips, never
+// skips one the reference enters, and meets the reference's leaves in the reference's order.  A
+// leaf inside a box the reference skips either fails its exact test or has an exact tmin >= that
+// box's tmin (nested boxes, monotone slab test, NaN axes included) > the closest hit at that
+// point; every other leaf is tested exactly as the reference tests it, with the closest hit the
+// reference has at that point.  The triangle tests, their order and so every result, ties
+// included, are the reference's.  Nodes i and i + 1 are fetched together (55% of steps continue
+// at i + 1); each face array ends with a padding node, so the second fetch stays in bounds.
 template <bool PHOTON>
 PPM_DEV void walk(const DevScene& s, int face, f3 org, f3 dir, f3 rcp, f3 t0, bool cull, Hit& h) {
     const int n = s.num_nodes;
-    const float4* __restrict__ nd = s.nodes + (size_t)face * (size_t)(n + 1) * 2;
+    const uint4* __restrict__ nd = s.nodes + (size_t)face * (size_t)(n + 1);
     int i = 0;
     int q0 = -1, q1 = -1;
     float m0 = 0.0f, m1 = 0.0f;
@@ -110,27 +126,29 @@ PPM_DEV void walk(const DevScene& s, int face, f3 org, f3 dir, f3 rcp, f3 t0, bo
             const int full = __popcll(__ballot(nq == kLeafQueue));
             if (4 * full > live) break;
             if (can) {
-                const float4 a0 = nd[2 * i], b0 = nd[2 * i + 1];
-                const float4 a1 = nd[2 * i + 2], b1 = nd[2 * i + 3];
+                const uint4 r0 = nd[i], r1 = nd[i + 1];
+                f3 lo, hi;
+                node_box(r0, lo, hi);
                 float tm;
-                bool hit = box_test(rcp, t0, a0, b0, h.t, tm);
-                int tri = __float_as_int(b0.w);
-                int next = hit ? i + 1 : __float_as_int(a0.w);
-                if (hit && tri >= 0) {
-                    q1 = nq == 1 ? tri : q1;
+                bool hit = box_test(rcp, t0, lo, hi, h.t, tm);
+                int link = (int)r0.w;
+                int next = (hit || link < 0) ? i + 1 : link;
+                if (hit && link < 0) {
+                    q1 = nq == 1 ? ~link : q1;
                     m1 = nq == 1 ? tm : m1;
-                    q0 = nq == 0 ? tri : q0;
+                    q0 = nq == 0 ? ~link : q0;
                     m0 = nq == 0 ? tm : m0;
                     nq++;
                 }
                 if (next == i + 1 && next < n && nq < kLeafQueue) {
-                    hit = box_test(rcp, t0, a1, b1, h.t, tm);
-                    tri = __float_as_int(b1.w);
-                    next = hit ? i + 2 : __float_as_int(a1.w);
-                    if (hit && tri >= 0) {
-                        q1 = nq == 1 ? tri : q1;
+                    node_box(r1, lo, hi);
+                    hit = box_test(rcp, t0, lo, hi, h.t, tm);
+                    link = (int)r1.w;
+                    next = (hit || link < 0) ? i + 2 : link;
+                    if (hit && link < 0) {
+                        q1 = nq == 1 ? ~link : q1;
                         m1 = nq == 1 ? tm : m1;
-                        q0 = nq == 0 ? tri : q0;
+                        q0 = nq == 0 ? ~link : q0;
                         m0 = nq == 0 ? tm : m0;
                         nq++;
                     }
@@ -139,23 +157,31 @@ PPM_DEV void walk(const DevScene& s, int face, f3 org, f3 dir, f3 rcp, f3 t0, bo
             }
         }
         if (__ballot(nq > 0) == 0ull) break;
-        // both queued triangles are fetched before the first test: the second leaf's re-check
-        // needs the first test's result, its data does not
-        float4 x0 = make_float4(0.0f, 0.0f, 0.0f, 0.0f), x1 = x0, x2 = x0, y0 = x0, y1 = x0, y2 = x0;
+        // both queued leaves are fetched before the first test: the second leaf's tests need the
+        // first test's result, its data does not
+        float4 x0 = make_float4(0.0f, 0.0f, 0.0f, 0.0f), x1 = x0, x2 = x0, x3 = x0, x4 = x0;
+        float4 y0 = x0, y1 = x0, y2 = x0, y3 = x0, y4 = x0;
         if (nq > 0) {
-            const float4* T = s.tri_isect + 3 * q0;
+            const float4* T = s.tri_isect + 5 * q0;
             x0 = T[0];
             x1 = T[1];
             x2 = T[2];
+            x3 = T[3];
+            x4 = T[4];
         }
         if (nq > 1) {
-            const float4* T = s.tri_isect + 3 * q1;
+            const float4* T = s.tri_isect + 5 * q1;
             y0 = T[0];
             y1 = T[1];
             y2 = T[2];
+            y3 = T[3];
+            y4 = T[4];
         }
-        if (nq > 0 && m0 <= h.t) tri_test<PHOTON>(x0, x1, x2, org, dir, cull, q0, h);
-        if (nq > 1 && m1 <= h.t) tri_test<PHOTON>(y0, y1, y2, org, dir, cull, q1, h);
+        float tm;
+        if (nq > 0 && m0 <= h.t && box_test(rcp, t0, xyz(x3), xyz(x4), h.t, tm))
+            tri_test<PHOTON>(x0, x1, x2, org, dir, cull, q0, h);
+        if (nq > 1 && m1 <= h.t && box_test(rcp, t0, xyz(y3), xyz(y4), h.t, tm))
+            tri_test<PHOTON>(y0, y1, y2, org, dir, cull, q1, h);
         nq = 0;
     }
 }
@@ -178,7 +204,7 @@ struct Surface {
     int brdf, mat;
 };
 
-PPM_DEV int tri_material(const DevScene& s, int tri) { return __float_as_int(s.tri_isect[3 * tri + 1].w); }
+PPM_DEV int tri_material(const DevScene& s, int tri) { return __float_as_int(s.tri_isect[5 * tri + 1].w); }
 
 // Shading fetch after the walk (eyeraytrace.fs:256-287).
 PPM_DEV Surface shade(const DevScene& s, const Hit& h) {

@@ -94,9 +94,14 @@ int check_scene(const ppm_scene* s) {
     for (int i = 0; i < s->num_triangles; i++)
         if (s->material[i] < 0 || s->material[i] >= s->num_materials)
             return set_err(nullptr, PPM_E_INVAL, "triangle material out of range");
-    for (int k = 0; k < s->num_nodes; k++)
-        if (s->node_triangle[k] < -1 || s->node_triangle[k] >= s->num_triangles)
-            return set_err(nullptr, PPM_E_INVAL, "leaf triangle out of range");
+    std::vector<char> in_leaf(s->num_triangles, 0);
+    for (int k = 0; k < s->num_nodes; k++) {
+        const int t = s->node_triangle[k];
+        if (t < -1 || t >= s->num_triangles) return set_err(nullptr, PPM_E_INVAL, "leaf triangle out of range");
+        if (t >= 0 && in_leaf[t]++) return set_err(nullptr, PPM_E_INVAL, "triangle in two leaves");
+    }
+    for (int t = 0; t < s->num_triangles; t++)
+        if (!in_leaf[t]) return set_err(nullptr, PPM_E_INVAL, "triangle in no leaf");
     for (size_t k = 0; k < (size_t)6 * s->num_nodes * 2; k++)
         if (s->node_links[k] < -1 || s->node_links[k] >= s->num_nodes)
             return set_err(nullptr, PPM_E_INVAL, "BVH link out of range");
@@ -108,11 +113,39 @@ int check_scene(const ppm_scene* s) {
     return PPM_OK;
 }
 
-// Per-face pre-order node records; the hit link of position i is i + 1.  Each face array has
-// N + 1 records: the last is padding that the walk's look-ahead fetch may read but never visits.
-int pack_nodes(const ppm_scene* s, std::vector<float4>& out) {
+uint16_t half_bits(_Float16 h) {
+    uint16_t b;
+    std::memcpy(&b, &h, 2);
+    return b;
+}
+
+float half_value(uint16_t b) {
+    _Float16 h;
+    std::memcpy(&h, &b, 2);
+    return (float)h;
+}
+
+// The largest binary16 value <= x (down) or the smallest >= x (up); the sign of x is kept, so a
+// positive bound never rounds below +0 and a negative one never above -0.
+uint16_t half_outward(float x, bool down) {
+    uint16_t b = half_bits((_Float16)x);
+    for (;;) {
+        const float v = half_value(b);
+        if (down ? !(v > x) : !(v < x)) return b;
+        const bool neg = (b & 0x8000u) != 0;
+        if (down) b = (b == 0x0000u) ? 0x8001u : (neg ? b + 1 : b - 1);
+        else b = (b == 0x8000u) ? 0x0001u : (neg ? b - 1 : b + 1);
+    }
+}
+
+// Per-face pre-order node records, 16 bytes each: the box as binary16 (x = lo.x | lo.y << 16,
+// y = lo.z | hi.x << 16, z = hi.y | hi.z << 16) rounded outward, and w = the miss position of an
+// interior node or ~triangle for a leaf, whose miss position is always the next one.  The hit
+// link of position i is i + 1.  Each face array has N + 1 records: the last is padding that the
+// walk's look-ahead fetch may read but never visits.
+int pack_nodes(const ppm_scene* s, std::vector<uint4>& out) {
     const int N = s->num_nodes;
-    out.assign((size_t)6 * (N + 1) * 2, make_float4(0, 0, 0, 0));
+    out.assign((size_t)6 * (N + 1), make_uint4(0, 0, 0, 0));
     std::vector<int> order(N), pos(N);
     for (int f = 0; f < 6; f++) {
         const int32_t* L = s->node_links + (size_t)f * N * 2;
@@ -131,22 +164,30 @@ int pack_nodes(const ppm_scene* s, std::vector<float4>& out) {
             const int mpos = miss < 0 ? N : pos[miss];
             if (miss >= 0 && mpos <= i) return set_err(nullptr, PPM_E_INVAL, "BVH miss link points backwards");
             const float* b = s->node_bounds + (size_t)nk * 6;
-            out[((size_t)f * (N + 1) + i) * 2 + 0] = make_float4(b[0], b[1], b[2], as_float(mpos));
-            out[((size_t)f * (N + 1) + i) * 2 + 1] = make_float4(b[3], b[4], b[5], as_float(s->node_triangle[nk]));
+            const int tri = s->node_triangle[nk];
+            if (tri >= 0 && mpos != i + 1) return set_err(nullptr, PPM_E_INVAL, "BVH leaf miss link is not the next node");
+            uint16_t h[6];
+            for (int k = 0; k < 6; k++) h[k] = half_outward(b[k], k < 3);
+            out[(size_t)f * (N + 1) + i] =
+                make_uint4(h[0] | (uint32_t)h[1] << 16, h[2] | (uint32_t)h[3] << 16, h[4] | (uint32_t)h[5] << 16,
+                           tri >= 0 ? (uint32_t)~tri : (uint32_t)mpos);
         }
-        out[((size_t)f * (N + 1) + N) * 2 + 0] = make_float4(0.0f, 0.0f, 0.0f, as_float(N + 1));
-        out[((size_t)f * (N + 1) + N) * 2 + 1] = make_float4(0.0f, 0.0f, 0.0f, as_float(-1));
+        out[(size_t)f * (N + 1) + N] = make_uint4(0, 0, 0, (uint32_t)(N + 1));
     }
     return PPM_OK;
 }
 
 // Triangle records of textureTriangles (gpurt.cpp:115-162) in the form the kernels read:
 // normal z as the shaders rebuild it, sign(sgn(n.z)(mat+1)) sqrt(|1 - x^2 - y^2|)
-// (eyeraytrace.fs:268-270), and the geometric normal normalize(cross(v1-v0, v2-v0)).
+// (eyeraytrace.fs:268-270), and the geometric normal normalize(cross(v1-v0, v2-v0)).  The
+// intersection record also holds the exact box of the triangle's leaf.
 void pack_triangles(const ppm_scene* s, std::vector<float4>& isect, std::vector<float4>& shade) {
     const int T = s->num_triangles;
-    isect.resize((size_t)T * 3);
+    isect.resize((size_t)T * 5);
     shade.resize((size_t)T * 6);
+    std::vector<int> leaf(T, -1);
+    for (int k = 0; k < s->num_nodes; k++)
+        if (s->node_triangle[k] >= 0) leaf[s->node_triangle[k]] = k;
     for (int t = 0; t < T; t++) {
         const float* p = s->positions + (size_t)t * 9;
         const float* n = s->normals + (size_t)t * 9;
@@ -166,9 +207,12 @@ void pack_triangles(const ppm_scene* s, std::vector<float4>& isect, std::vector<
         float c[3] = {e0[1] * e1[2] - e0[2] * e1[1], e0[2] * e1[0] - e0[0] * e1[2], e0[0] * e1[1] - e0[1] * e1[0]};
         const float inv = 1.0f / std::sqrt(c[0] * c[0] + c[1] * c[1] + c[2] * c[2]);
         const float gn[3] = {c[0] * inv, c[1] * inv, c[2] * inv};
-        isect[(size_t)t * 3 + 0] = make_float4(p[0], p[1], p[2], as_float(brdf));
-        isect[(size_t)t * 3 + 1] = make_float4(e0[0], e0[1], e0[2], as_float(mat));
-        isect[(size_t)t * 3 + 2] = make_float4(e1[0], e1[1], e1[2], 0.0f);
+        const float* lb = s->node_bounds + (size_t)leaf[t] * 6;
+        isect[(size_t)t * 5 + 0] = make_float4(p[0], p[1], p[2], as_float(brdf));
+        isect[(size_t)t * 5 + 1] = make_float4(e0[0], e0[1], e0[2], as_float(mat));
+        isect[(size_t)t * 5 + 2] = make_float4(e1[0], e1[1], e1[2], 0.0f);
+        isect[(size_t)t * 5 + 3] = make_float4(lb[0], lb[1], lb[2], 0.0f);
+        isect[(size_t)t * 5 + 4] = make_float4(lb[3], lb[4], lb[5], 0.0f);
         float4* o = &shade[(size_t)t * 6];
         o[0] = make_float4(p[0], p[1], p[2], rn[0][0]);
         o[1] = make_float4(p[3], p[4], p[5], rn[0][1]);
@@ -446,7 +490,7 @@ size_t state_footprint(const ppm_params* p) {
 
 size_t scene_footprint(int T, int N, int M, int NL, int tex_size) {
     const size_t S = (size_t)tex_size;
-    const size_t sizes[] = {(size_t)6 * (N + 1) * 32, (size_t)T * 48, (size_t)T * 96, (size_t)M * 32, (size_t)NL * 4,
+    const size_t sizes[] = {(size_t)6 * (N + 1) * 16, (size_t)T * 80, (size_t)T * 96, (size_t)M * 32, (size_t)NL * 4,
                             (size_t)NL * 4, S ? (size_t)T * 32 : 0, (size_t)M * S * S * 4};
     size_t tot = 0;
     for (size_t b : sizes) tot += Arena::round_size(b ? b : 16);
@@ -538,7 +582,7 @@ int upload_scene(const PackedScene& ps, Arena* arena, hipStream_t up, hipStream_
         return rc;
     DevScene& sc = sd->s;
     std::memset(&sc, 0, sizeof(sc));
-    sc.nodes = (const float4*)d_nodes;
+    sc.nodes = (const uint4*)d_nodes;
     sc.tri_isect = (const float4*)d_isect;
     sc.tri_shade = (const float4*)d_shade;
     sc.mats = (const float4*)d_mats;
