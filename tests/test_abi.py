@@ -212,3 +212,21 @@ def test_server_config_defaults_and_no_device():
     if rc == 0 and count.value > 0:
         pytest.skip("a GPU is present")
     assert g.library().ppm_server_create(ctypes.byref(cfg), ctypes.byref(h)) == -5
+
+
+def test_scene_with_a_triangle_in_two_leaves_or_none_is_rejected():
+    """Every triangle must sit in exactly one leaf (its exact box is packed with the triangle)."""
+    m = _scene()
+    nt = m.arrays()["node_triangle"].astype(np.int32)
+    leaves = np.nonzero(nt >= 0)[0]
+    dup = nt.copy()
+    dup[leaves[1]] = dup[leaves[0]]  # triangle in two leaves, another in none
+    none = nt.copy()
+    none[leaves[0]] = -1             # a triangle in no leaf
+    for arr, msg in ((dup, b"triangle in two leaves"), (none, b"triangle in no leaf")):
+        s = g.Scene()
+        ctypes.pointer(s)[0] = m.scene.contents
+        s.node_triangle = arr.ctypes.data_as(ctypes.POINTER(ctypes.c_int32))
+        h = ctypes.c_void_p()
+        assert g.library().ppm_create(ctypes.byref(s), ctypes.byref(g.default_params(16)), ctypes.byref(h)) == -1
+        assert g.library().ppm_last_error(None) == msg
