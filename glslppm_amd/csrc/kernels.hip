@@ -76,13 +76,30 @@ PPM_DEV void tri_test(float4 q0, float4 q1, float4 q2, f3 org, f3 dir, bool cull
 // Leaves a lane may hold before its triangle tests run.
 constexpr int kLeafQueue = 2;
 
-PPM_DEV float half_lo(unsigned int w) { return (float)__builtin_bit_cast(_Float16, (unsigned short)(w & 0xffffu)); }
-PPM_DEV float half_hi(unsigned int w) { return (float)__builtin_bit_cast(_Float16, (unsigned short)(w >> 16)); }
+// float(binary16 in the low / high half of w) * r with one rounding: v_fma_mix_f32 widens the
+// half exactly and adds -0, which leaves every product (signed zeros included) unchanged, so the
+// result is bit-identical to a conversion followed by v_mul_f32, in one instruction.
+PPM_DEV float mul_half_lo(unsigned int w, float r) {
+    float d;
+    asm("v_fma_mix_f32 %0, %1, %2, neg(0) op_sel_hi:[1,0,0]" : "=v"(d) : "v"(w), "v"(r));
+    return d;
+}
+PPM_DEV float mul_half_hi(unsigned int w, float r) {
+    float d;
+    asm("v_fma_mix_f32 %0, %1, %2, neg(0) op_sel:[1,0,0] op_sel_hi:[1,0,0]" : "=v"(d) : "v"(w), "v"(r));
+    return d;
+}
 
-// The outward-rounded box of a node record.
-PPM_DEV void node_box(uint4 r, f3& lo, f3& hi) {
-    lo = mk3(half_lo(r.x), half_hi(r.x), half_lo(r.y));
-    hi = mk3(half_hi(r.y), half_lo(r.z), half_hi(r.z));
+// Slab test of a node record's outward-rounded binary16 box, in box_test's arithmetic.
+PPM_DEV bool node_test(uint4 r, f3 rcp, f3 t0, float ht, float& tmin_out) {
+    const float lx = mul_half_lo(r.x, rcp.x) + t0.x, ly = mul_half_hi(r.x, rcp.y) + t0.y;
+    const float lz = mul_half_lo(r.y, rcp.z) + t0.z;
+    const float hx = mul_half_hi(r.y, rcp.x) + t0.x, hy = mul_half_lo(r.z, rcp.y) + t0.y;
+    const float hz = mul_half_hi(r.z, rcp.z) + t0.z;
+    const float tmin = max1(max1(min1(lx, hx), min1(ly, hy)), min1(lz, hz));
+    const float tmax = min1(min1(max1(lx, hx), max1(ly, hy)), max1(lz, hz));
+    tmin_out = tmin;
+    return (tmin <= tmax) && (tmin <= ht) && (tmax >= 0.0f);
 }
 
 // Closest hit over the threaded BVH of one cube face (eyeraytrace.fs:67-254,
@@ -127,10 +144,8 @@ PPM_DEV void walk(const DevScene& s, int face, f3 org, f3 dir, f3 rcp, f3 t0, bo
             if (4 * full > live) break;
             if (can) {
                 const uint4 r0 = nd[i], r1 = nd[i + 1];
-                f3 lo, hi;
-                node_box(r0, lo, hi);
                 float tm;
-                bool hit = box_test(rcp, t0, lo, hi, h.t, tm);
+                bool hit = node_test(r0, rcp, t0, h.t, tm);
                 int link = (int)r0.w;
                 int next = (hit || link < 0) ? i + 1 : link;
                 if (hit && link < 0) {
@@ -141,8 +156,7 @@ PPM_DEV void walk(const DevScene& s, int face, f3 org, f3 dir, f3 rcp, f3 t0, bo
                     nq++;
                 }
                 if (next == i + 1 && next < n && nq < kLeafQueue) {
-                    node_box(r1, lo, hi);
-                    hit = box_test(rcp, t0, lo, hi, h.t, tm);
+                    hit = node_test(r1, rcp, t0, h.t, tm);
                     link = (int)r1.w;
                     next = (hit || link < 0) ? i + 2 : link;
                     if (hit && link < 0) {

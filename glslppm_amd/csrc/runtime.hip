@@ -125,17 +125,25 @@ float half_value(uint16_t b) {
     return (float)h;
 }
 
-// The largest binary16 value <= x (down) or the smallest >= x (up); the sign of x is kept, so a
-// positive bound never rounds below +0 and a negative one never above -0.
+// The largest binary16 value <= x (down) or the smallest >= x (up), never a subnormal (those
+// step further outward, to a signed zero or the smallest normal, so that the bounds hold whether
+// or not a unit flushes binary16 subnormals); the sign of x is kept, so a positive bound never
+// rounds below +0 and a negative one never above -0.
 uint16_t half_outward(float x, bool down) {
     uint16_t b = half_bits((_Float16)x);
     for (;;) {
         const float v = half_value(b);
-        if (down ? !(v > x) : !(v < x)) return b;
+        if (down ? !(v > x) : !(v < x)) break;
         const bool neg = (b & 0x8000u) != 0;
         if (down) b = (b == 0x0000u) ? 0x8001u : (neg ? b + 1 : b - 1);
         else b = (b == 0x8000u) ? 0x0001u : (neg ? b - 1 : b + 1);
     }
+    if ((b & 0x7c00u) == 0 && (b & 0x03ffu) != 0) {
+        const bool neg = (b & 0x8000u) != 0;
+        if (down) b = neg ? 0x8400u : 0x0000u;
+        else b = neg ? 0x8000u : 0x0400u;
+    }
+    return b;
 }
 
 // Per-face pre-order node records, 16 bytes each: the box as binary16 (x = lo.x | lo.y << 16,

@@ -13,7 +13,8 @@ f32 = np.float32
 
 
 def outward(x, down):
-    """Largest binary16 <= x (down) or smallest >= x (up), keeping the sign of x."""
+    """Largest binary16 <= x (down) or smallest >= x (up), keeping the sign of x and never a
+    subnormal (the packing rule of the node records)."""
     x = np.asarray(x, f32)
     with np.errstate(over="ignore"):
         h = x.astype(np.float16)
@@ -24,8 +25,14 @@ def outward(x, down):
                 break
             step = np.float16(-np.inf) if down else np.float16(np.inf)
             h = np.where(bad, np.nextafter(h, step), h)
-    # nextafter(+0, -inf) is the negative denormal; a positive bound stays >= +0 (and vice versa)
-    return h.astype(f32)
+        # subnormals step further outward: to a signed zero or the smallest normal (6.1e-5)
+        v = h.astype(f32)
+        sub = (v != 0) & (np.abs(v) < f32(2.0 ** -14))
+        if down:
+            v = np.where(sub, np.where(v < 0, f32(-(2.0 ** -14)), f32(0.0)), v)
+        else:
+            v = np.where(sub, np.where(v < 0, f32(-0.0), f32(2.0 ** -14)), v)
+    return v.astype(f32)
 
 
 def slab(rcp, t0, lo, hi, ht):
@@ -73,6 +80,7 @@ def test_outward_rounding_brackets_and_keeps_sign():
     # positive values never round down below +0, negative ones never up above -0
     assert np.all(lo[xs > 0] >= 0) and np.all(hi[xs < 0] <= 0)
     assert lo[0] == 0 and not np.signbit(lo[0]) and np.signbit(hi[1])
+    assert lo[4] == 0 and hi[4] == 2.0 ** -14 and lo[5] == -(2.0 ** -14) and hi[5] == 0 and np.signbit(hi[5])
     assert lo[11] == 65504.0 and hi[11] == np.inf and lo[12] == -np.inf and hi[12] == -65504.0
 
 
