@@ -110,7 +110,7 @@ PPM_DEV bool node_test(uint4 r, f3 rcp, f3 t0, float ht, float& tmin_out) {
 // rounded divisions each) run with many lanes at once instead of whenever any lane meets a leaf.
 // In the box phase a lane that meets a leaf with a hit box queues it, with the box's entry
 // distance tmin, and walks on; the wave leaves the box phase when no lane can step or when more
-// than a quarter of its live lanes hold full queues.  The leaf phase then takes each lane's
+// than a quarter of the lanes that were still walking when the phase began hold full queues.  The leaf phase then takes each lane's
 // queued leaves in the order they were met: a leaf whose tmin now exceeds the closest hit is
 // dropped, the others get the reference's slab test of their exact box with the current closest
 // hit, then their triangle test.
@@ -132,51 +132,56 @@ PPM_DEV void walk(const DevScene& s, int face, f3 org, f3 dir, f3 rcp, f3 t0, bo
     const int n = s.num_nodes;
     const uint4* __restrict__ nd = s.nodes + (size_t)face * (size_t)(n + 1);
     int i = 0;
+    // queued leaves, newest first: q0 (tmin m0), then q1 (m1) when two are queued
     int q0 = -1, q1 = -1;
     float m0 = 0.0f, m1 = 0.0f;
     int nq = 0;
     for (;;) {
+        const int live = __popcll(__ballot(i < n));  // every queue is empty here
         for (;;) {
             const bool can = i < n && nq < kLeafQueue;
             if (__ballot(can) == 0ull) break;
-            const int live = __popcll(__ballot(i < n || nq > 0));
-            const int full = __popcll(__ballot(nq == kLeafQueue));
-            if (4 * full > live) break;
+            if (4 * __popcll(__ballot(nq == kLeafQueue)) > live) break;
             if (can) {
                 const uint4 r0 = nd[i], r1 = nd[i + 1];
                 float tm;
                 bool hit = node_test(r0, rcp, t0, h.t, tm);
                 int link = (int)r0.w;
-                int next = (hit || link < 0) ? i + 1 : link;
+                bool cont = hit || link < 0;
                 if (hit && link < 0) {
-                    q1 = nq == 1 ? ~link : q1;
-                    m1 = nq == 1 ? tm : m1;
-                    q0 = nq == 0 ? ~link : q0;
-                    m0 = nq == 0 ? tm : m0;
+                    q1 = q0;
+                    m1 = m0;
+                    q0 = ~link;
+                    m0 = tm;
                     nq++;
                 }
-                if (next == i + 1 && next < n && nq < kLeafQueue) {
+                int next = cont ? i + 1 : link;
+                if (cont && i + 1 < n && nq < kLeafQueue) {
                     hit = node_test(r1, rcp, t0, h.t, tm);
                     link = (int)r1.w;
-                    next = (hit || link < 0) ? i + 2 : link;
+                    cont = hit || link < 0;
                     if (hit && link < 0) {
-                        q1 = nq == 1 ? ~link : q1;
-                        m1 = nq == 1 ? tm : m1;
-                        q0 = nq == 0 ? ~link : q0;
-                        m0 = nq == 0 ? tm : m0;
+                        q1 = q0;
+                        m1 = m0;
+                        q0 = ~link;
+                        m0 = tm;
                         nq++;
                     }
+                    next = cont ? i + 2 : link;
                 }
                 i = next;
             }
         }
         if (__ballot(nq > 0) == 0ull) break;
+        // the leaves in the order they were met: with two queued, q1 came first
+        const int qa = nq == 2 ? q1 : q0, qb = q0;
+        const float ma = nq == 2 ? m1 : m0, mb = m0;
         // both queued leaves are fetched before the first test: the second leaf's tests need the
         // first test's result, its data does not
         float4 x0 = make_float4(0.0f, 0.0f, 0.0f, 0.0f), x1 = x0, x2 = x0, x3 = x0, x4 = x0;
         float4 y0 = x0, y1 = x0, y2 = x0, y3 = x0, y4 = x0;
         if (nq > 0) {
-            const float4* T = s.tri_isect + 5 * q0;
+            const float4* T = s.tri_isect + 5 * qa;
             x0 = T[0];
             x1 = T[1];
             x2 = T[2];
@@ -184,7 +189,7 @@ PPM_DEV void walk(const DevScene& s, int face, f3 org, f3 dir, f3 rcp, f3 t0, bo
             x4 = T[4];
         }
         if (nq > 1) {
-            const float4* T = s.tri_isect + 5 * q1;
+            const float4* T = s.tri_isect + 5 * qb;
             y0 = T[0];
             y1 = T[1];
             y2 = T[2];
@@ -192,10 +197,10 @@ PPM_DEV void walk(const DevScene& s, int face, f3 org, f3 dir, f3 rcp, f3 t0, bo
             y4 = T[4];
         }
         float tm;
-        if (nq > 0 && m0 <= h.t && box_test(rcp, t0, xyz(x3), xyz(x4), h.t, tm))
-            tri_test<PHOTON>(x0, x1, x2, org, dir, cull, q0, h);
-        if (nq > 1 && m1 <= h.t && box_test(rcp, t0, xyz(y3), xyz(y4), h.t, tm))
-            tri_test<PHOTON>(y0, y1, y2, org, dir, cull, q1, h);
+        if (nq > 0 && ma <= h.t && box_test(rcp, t0, xyz(x3), xyz(x4), h.t, tm))
+            tri_test<PHOTON>(x0, x1, x2, org, dir, cull, qa, h);
+        if (nq > 1 && mb <= h.t && box_test(rcp, t0, xyz(y3), xyz(y4), h.t, tm))
+            tri_test<PHOTON>(y0, y1, y2, org, dir, cull, qb, h);
         nq = 0;
     }
 }
