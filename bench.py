@@ -24,18 +24,16 @@ sys.path.insert(0, ROOT)
 METRIC = "photons/s + eye-rays/s per pass; p50/p99 time-to-target-passes under load"
 HBM_PEAK_GBS = 8000.0
 
-# Algorithmic bytes per unit of work, each array read or written once in its minimal payload:
-#   photon segment (k_photon): state in 72 B (pos, dir, flux, hit normal 4 x 12 B, generator 16 B,
-#     triangle 4 B, bounce 4 B) + state out 72 B + slot index out 4 B + count and representative
-#     updates 8 B = 156 B
-#   hit point per pass (k_gather): reads pos, normal, reflectance 36 B, triangle 4 B, (tau, R),
-#     (E, N) 32 B, writes (tau, R), (E, N) 32 B, RGBA8 4 B = 108 B; plus 8 slots x (count 4 B +
-#     representative 4 B) and up to 8 representative photons x 36 B, which land in L2 and are
-#     not counted
-#   eye ray (k_eye): reads generator 16 B and (E, N) 16 B, writes pos, refl, nrm 48 B, generator
-#     16 B, triangle 4 B, (material, t) 8 B, (E, N) 16 B = 124 B
-#   slot clear (k_clear_slots): 8 B per slot
-BYTES_PER = {"photon": 156.0, "gather": 108.0, "eye": 124.0, "clear": 8.0}
+# Algorithmic bytes per unit of work: the compulsory state traffic of one pass, each array read and
+# written once in its minimal fp32 payload (B_pass = 216 P + 16 H + 120 Q + 104 Q / L):
+#   photon segment (photon stage): state in 72 B (pos, dir, flux, hit normal 4 x 12 B, generator
+#     16 B, triangle 4 B, bounce 4 B) + state out 72 B + hash and bin 36 B + its representative
+#     record 36 B = 216 B
+#   hit point per pass (k_gather, with the draw fused): gather and update 80 B (60 read, 20
+#     written) + draw 40 B = 120 B
+#   eye ray (eye stage, every L passes): 104 B
+#   hash slot (k_clear_slots): clear plus read of count and representative, 16 B
+BYTES_PER = {"photon": 216.0, "gather": 120.0, "eye": 104.0, "clear": 16.0}
 KERNEL_NAMES = {"photon": "photon stage (k_photon_gen + k_bucket_scan + k_bucket_scatter + k_photon_trace)",
                 "gather": "k_gather", "eye": "eye stage (k_eye + k_grid)", "clear": "k_clear_slots"}
 
