@@ -58,6 +58,17 @@ def library():
         lib.oracle_update.argtypes = [fp, fp, fp, fp, ctypes.c_float, fp, ctypes.c_float]
         lib.oracle_texture_sample.restype = None
         lib.oracle_texture_sample.argtypes = [vp, ctypes.c_float, ctypes.c_float, ctypes.c_float, fp]
+        lib.oracle_camera_ray.restype = None
+        lib.oracle_camera_ray.argtypes = [vp, vp, ctypes.c_int, ctypes.c_int, ctypes.c_float, ctypes.c_float, fp, fp,
+                                          fp, fp]
+        lib.oracle_light_sample.restype = ctypes.c_int
+        lib.oracle_light_sample.argtypes = [vp, fp, fp, fp, fp]
+        lib.oracle_ibl_sample.restype = None
+        lib.oracle_ibl_sample.argtypes = [vp, fp, fp, fp, fp]
+        lib.oracle_fresnel.restype = ctypes.c_float
+        lib.oracle_fresnel.argtypes = [fp, fp, ctypes.c_float, ctypes.c_float]
+        lib.oracle_photon_bounce.restype = ctypes.c_int
+        lib.oracle_photon_bounce.argtypes = [vp, ctypes.c_int, fp, fp, fp, fp, fp, fp, fp]
         _lib = lib
     return _lib
 
@@ -128,6 +139,54 @@ def texture_sample(scene_ptr, u, v, r):
     out = np.zeros(3, np.float32)
     library().oracle_texture_sample(ctypes.cast(scene_ptr, ctypes.c_void_p), float(u), float(v), float(r), _fptr(out))
     return out
+
+
+def camera_ray(scene_ptr, params, i, j, aa_x, aa_y, rnd):
+    """eyeraytrace.fs's camera ray of pixel (i, j): (origin, direction, filter weight, advanced rnd)."""
+    r = np.array(rnd, np.float32)
+    o = np.zeros(3, np.float32)
+    d = np.zeros(3, np.float32)
+    f = np.zeros(1, np.float32)
+    library().oracle_camera_ray(ctypes.cast(scene_ptr, ctypes.c_void_p), ctypes.cast(ctypes.pointer(params), ctypes.c_void_p),
+                                int(i), int(j), float(aa_x), float(aa_y), _fptr(r), _fptr(o), _fptr(d), _fptr(f))
+    return o, d, f[0], r
+
+
+def light_sample(scene_ptr, rnd):
+    """photontrace.fs's emission from the light triangles: (triangle, flux, direction, origin, rnd)."""
+    r = np.array(rnd, np.float32)
+    f = np.zeros(3, np.float32)
+    d = np.zeros(3, np.float32)
+    o = np.zeros(3, np.float32)
+    tri = library().oracle_light_sample(ctypes.cast(scene_ptr, ctypes.c_void_p), _fptr(r), _fptr(f), _fptr(d), _fptr(o))
+    return tri, f, d, o, r
+
+
+def ibl_sample(scene_ptr, rnd):
+    """photontrace.fs's emission from the environment: (flux, direction, origin, rnd)."""
+    r = np.array(rnd, np.float32)
+    f = np.zeros(3, np.float32)
+    d = np.zeros(3, np.float32)
+    o = np.zeros(3, np.float32)
+    library().oracle_ibl_sample(ctypes.cast(scene_ptr, ctypes.c_void_p), _fptr(r), _fptr(f), _fptr(d), _fptr(o))
+    return f, d, o, r
+
+
+def fresnel(incom, normal, ni, ne):
+    return np.float32(library().oracle_fresnel(_fptr(np.asarray(incom, np.float32)), _fptr(np.asarray(normal, np.float32)),
+                                               float(ni), float(ne)))
+
+
+def photon_bounce(scene_ptr, tri, ipos, pnrm, pdir, rnd, flux):
+    """photontrace.fs's bounce of a stored photon: (continues, flux, origin, direction, rnd)."""
+    r = np.array(rnd, np.float32)
+    f = np.array(flux, np.float32)
+    o = np.zeros(3, np.float32)
+    d = np.zeros(3, np.float32)
+    cont = library().oracle_photon_bounce(ctypes.cast(scene_ptr, ctypes.c_void_p), int(tri),
+                                          _fptr(np.asarray(ipos, np.float32)), _fptr(np.asarray(pnrm, np.float32)),
+                                          _fptr(np.asarray(pdir, np.float32)), _fptr(r), _fptr(f), _fptr(o), _fptr(d))
+    return bool(cont), f, o, d, r
 
 
 def radiance(flux_radius, emission, total_photons):

@@ -527,11 +527,11 @@ int oracle_create(const ppm_scene* s, const ppm_params* p, void** out) {
 }
 
 /* ---------------- eyeraytrace.fs main() ---------------- */
-static void eye_fragment(Oracle* o, int i, int j, float aax, float aay, int num_eye_samples) {
-    const ppm_scene* s = o->s;
+/* eyeraytrace.fs:396-420: the camera ray of pixel (i, j) with the pass's AA offset, through the
+ * thin lens (two draws even at aperture 0), and the tent filter weight. */
+static void camera_ray(const ppm_scene* s, const ppm_params* prm, int i, int j, float aax, float aay, v4* rndv,
+                       v3* org, v3* dir, float* filter) {
     const ppm_camera* c = &s->camera;
-    const int px = j * o->Qw + i;
-    v4 rndv = o->qrng[px];
     v3 C = V3(c->origin[0], c->origin[1], c->origin[2]);
     v3 U = V3(c->u[0], c->u[1], c->u[2]), V = V3(c->v[0], c->v[1], c->v[2]), W = V3(c->w[0], c->w[1], c->w[2]);
     float fcx = (float)i + 0.5f, fcy = (float)j + 0.5f;
@@ -539,14 +539,22 @@ static void eye_fragment(Oracle* o, int i, int j, float aax, float aay, int num_
     float ppy = (fcy + aay) - c->half_height;
     v3 rel = vadd(vadd(vscale(U, ppx), vscale(V, ppy)), vscale(W, c->distance));
     v3 rdir = vnormalize(rel);
-    v3 rorg = C;
-    float filter = (1.0f - fabsf(aax) / 1.5f) * (1.0f - fabsf(aay) / 1.5f);
-    v3 fp = vadd(C, vscale(rdir, o->prm.focal_length));
-    float radius = sqrtf(gpurnd4(&rndv)) * o->prm.aperture;
-    float theta = 6.283184f * gpurnd4(&rndv);
+    *filter = (1.0f - fabsf(aax) / 1.5f) * (1.0f - fabsf(aay) / 1.5f);
+    v3 fp = vadd(C, vscale(rdir, prm->focal_length));
+    float radius = sqrtf(gpurnd4(rndv)) * prm->aperture;
+    float theta = 6.283184f * gpurnd4(rndv);
     v3 lens = vadd(C, vscale(vadd(vscale(U, oracle_cosf(theta)), vscale(V, oracle_sinf(theta))), radius));
-    rdir = vnormalize(vsub(fp, lens));
-    rorg = lens;
+    *dir = vnormalize(vsub(fp, lens));
+    *org = lens;
+}
+
+static void eye_fragment(Oracle* o, int i, int j, float aax, float aay, int num_eye_samples) {
+    const ppm_scene* s = o->s;
+    const int px = j * o->Qw + i;
+    v4 rndv = o->qrng[px];
+    v3 rorg, rdir;
+    float filter;
+    camera_ray(s, &o->prm, i, j, aax, aay, &rndv, &rorg, &rdir, &filter);
     v3 col = vscale(V3(1.0f, 1.0f, 1.0f), filter);
     v3 nrm = V3(0, 0, 0), pos = V3(0, 0, 0), emi = V3(0, 0, 0);
     const float eps = 1e-5f;
@@ -636,7 +644,7 @@ static void eye_fragment(Oracle* o, int i, int j, float aax, float aay, int num_
 }
 
 /* ---------------- photontrace.fs ---------------- */
-static void light_sample(const ppm_scene* s, v4* rndv, v3* flux, v3* dir, v3* org) {
+static int light_sample(const ppm_scene* s, v4* rndv, v3* flux, v3* dir, v3* org) {
     float Rnd = gpurnd4(rndv);
     int low = 0;
     float stride = 1.0f / (float)s->num_lights;
@@ -685,6 +693,7 @@ static void light_sample(const ppm_scene* s, v4* rndv, v3* flux, v3* dir, v3* or
     ry = sqrtf(ry);
     *dir = onb(V3(oracle_sinf(rx) * ry, sqrtf(1.0f - ry * ry), oracle_cosf(rx) * ry), nrm);
     *org = o;
+    return tri;
 }
 
 static void ibl_sample(const ppm_scene* s, v4* rndv, v3* flux, v3* dir, v3* org) {
@@ -714,6 +723,145 @@ static void ibl_sample(const ppm_scene* s, v4* rndv, v3* flux, v3* dir, v3* org)
     *dir = vneg(d);
 }
 
+/* photontrace.fs:487-686: the bounce of a stored photon off triangle tri at ipos (shading normal
+ * pnrm, arriving along pd): BRDF sampling, then Russian roulette on max(col).  Returns 0 when the
+ * photon ends; otherwise sets the new ray and scales the flux. */
+static int photon_bounce(const ppm_scene* s, int tri, v3 ipos, v3 pnrm, v3 pd, v4* rndv, v3* flux, v3* rorg,
+                         v3* rdir) {
+    v3 V0 = vertex(s, tri, 0), V1 = vertex(s, tri, 1), V2 = vertex(s, tri, 2);
+    v3 gnrm = vnormalize(vcross(vsub(V1, V0), vsub(V2, V0)));
+    const ppm_material* m = &s->materials[s->material[tri]];
+    v3 icol = V3(m->color[0], m->color[1], m->color[2]);
+    int brdf = m->brdf;
+    float eta = m->eta, g = m->g;
+    v3 inrm = pnrm;
+    const float eps = 1e-6f;
+    /* photontrace.fs:519-544, 562-565: barycentrics from sub-triangle areas, then i.tex */
+    if (textured(s)) {
+        float bx = vlength(vcross(vsub(V1, V0), vsub(ipos, V0)));
+        float by = vlength(vcross(vsub(V2, V1), vsub(ipos, V1)));
+        float bz = vlength(vcross(vsub(V0, V2), vsub(ipos, V2)));
+        float bw = (bx + by) + bz;
+        float wx = bx / bw, wy = by / bw, wz = bz / bw;
+        float tu = (texcoord(s, tri, 2, 0) * wx + texcoord(s, tri, 0, 0) * wy) + texcoord(s, tri, 1, 0) * wz;
+        float tv = (texcoord(s, tri, 2, 1) * wx + texcoord(s, tri, 0, 1) * wy) + texcoord(s, tri, 1, 1) * wz;
+        if (fabsf(tu) < 1e+10f)
+            icol = vmul(icol, oracle_texture3d(s, tu, tv,
+                                               material_rcp(s) * (((float)s->material[tri] + 0.5f) + 0.25f)));
+    }
+    if (brdf == 0) {
+        float r0 = 6.283184f * gpurnd4(rndv);
+        float r1 = sqrtf(gpurnd4(rndv));
+        v3 v = V3(oracle_sinf(r0) * r1, sqrtf(1.0f - r1 * r1), oracle_cosf(r0) * r1);
+        *rorg = vadd(ipos, vscale(gnrm, eps));
+        *rdir = onb(v, inrm);
+        if (vdot(*rdir, gnrm) < 0.0f) *rdir = vneg(*rdir);
+    } else if (brdf == 1 || brdf == 4) {
+        *rorg = vadd(ipos, vscale(gnrm, eps));
+        if (brdf == 4) inrm = glossy_reflect(inrm, inrm, g, rndv);
+        *rdir = vreflect(pd, inrm);
+        if (vdot(*rdir, gnrm) < 0.0f) *rdir = vneg(*rdir);
+    } else if (brdf == 2 || brdf == 5) {
+        if (brdf == 5) inrm = glossy_reflect(inrm, inrm, g, rndv);
+        float ln = vdot(inrm, pd);
+        if (ln < 0.0f) {
+            float Re = fresnel(vneg(pd), inrm, 1.0f, eta);
+            if (gpurnd4(rndv) < Re) {
+                *rorg = vadd(ipos, vscale(gnrm, eps));
+                *rdir = vreflect(pd, inrm);
+                icol = V3(1.0f, 1.0f, 1.0f);
+                if (vdot(*rdir, gnrm) < 0.0f) *rdir = vneg(*rdir);
+            } else {
+                *rorg = vsub(ipos, vscale(gnrm, eps));
+                *rdir = vrefract(pd, inrm, 1.0f / eta);
+                if (vdot(*rdir, gnrm) > 0.0f) *rdir = vneg(*rdir);
+            }
+        } else {
+            float Re = fresnel(vneg(pd), vneg(inrm), eta, 1.0f);
+            if (gpurnd4(rndv) < Re) {
+                *rorg = vsub(ipos, vscale(gnrm, eps));
+                *rdir = vreflect(pd, vneg(inrm));
+                if (vdot(*rdir, vneg(gnrm)) < 0.0f) *rdir = vneg(*rdir);
+            } else {
+                *rorg = vadd(ipos, vscale(gnrm, eps));
+                *rdir = vrefract(pd, vneg(inrm), eta);
+                if (vdot(*rdir, vneg(gnrm)) > 0.0f) *rdir = vneg(*rdir);
+            }
+        }
+    } else if (brdf == 3 || brdf == 6) {
+        if (brdf == 6) inrm = glossy_reflect(inrm, inrm, g, rndv);
+        float Re = fresnel(vneg(pd), inrm, 1.0f, eta);
+        if (gpurnd4(rndv) < Re) {
+            *rorg = vadd(ipos, vscale(gnrm, eps));
+            *rdir = vreflect(pd, inrm);
+            icol = V3(1.0f, 1.0f, 1.0f);
+            if (vdot(*rdir, gnrm) < 0.0f) *rdir = vneg(*rdir);
+        } else {
+            float r0 = 6.283184f * gpurnd4(rndv);
+            float r1 = sqrtf(gpurnd4(rndv));
+            v3 v = V3(oracle_sinf(r0) * r1, sqrtf(1.0f - r1 * r1), oracle_cosf(r0) * r1);
+            *rorg = vadd(ipos, vscale(gnrm, eps));
+            *rdir = onb(v, inrm);
+            if (vdot(*rdir, gnrm) < 0.0f) *rdir = vneg(*rdir);
+        }
+    }
+    float p = gl_max(gl_max(icol.x, icol.y), icol.z);
+    if (p < gpurnd4(rndv)) {
+        return 0;
+    } else {
+        icol = V3(icol.x / p, icol.y / p, icol.z / p);
+        *flux = vmul(*flux, icol);
+    }
+    return 1;
+}
+
+/* Exported for the oracle's own tests: the camera ray, the light and IBL emission samples, the
+ * Fresnel term and the photon bounce above, on caller-held generator states (rnd: 4 floats, read
+ * and advanced). */
+void oracle_camera_ray(const ppm_scene* s, const ppm_params* p, int i, int j, float aax, float aay, float* rnd,
+                       float* org, float* dir, float* filter) {
+    v4 r = V4(rnd[0], rnd[1], rnd[2], rnd[3]);
+    v3 o, d;
+    camera_ray(s, p, i, j, aax, aay, &r, &o, &d, filter);
+    rnd[0] = r.x; rnd[1] = r.y; rnd[2] = r.z; rnd[3] = r.w;
+    org[0] = o.x; org[1] = o.y; org[2] = o.z;
+    dir[0] = d.x; dir[1] = d.y; dir[2] = d.z;
+}
+int oracle_light_sample(const ppm_scene* s, float* rnd, float* flux, float* dir, float* org) {
+    v4 r = V4(rnd[0], rnd[1], rnd[2], rnd[3]);
+    v3 f, d, o;
+    int tri = light_sample(s, &r, &f, &d, &o);
+    rnd[0] = r.x; rnd[1] = r.y; rnd[2] = r.z; rnd[3] = r.w;
+    flux[0] = f.x; flux[1] = f.y; flux[2] = f.z;
+    dir[0] = d.x; dir[1] = d.y; dir[2] = d.z;
+    org[0] = o.x; org[1] = o.y; org[2] = o.z;
+    return tri;
+}
+void oracle_ibl_sample(const ppm_scene* s, float* rnd, float* flux, float* dir, float* org) {
+    v4 r = V4(rnd[0], rnd[1], rnd[2], rnd[3]);
+    v3 f, d, o;
+    ibl_sample(s, &r, &f, &d, &o);
+    rnd[0] = r.x; rnd[1] = r.y; rnd[2] = r.z; rnd[3] = r.w;
+    flux[0] = f.x; flux[1] = f.y; flux[2] = f.z;
+    dir[0] = d.x; dir[1] = d.y; dir[2] = d.z;
+    org[0] = o.x; org[1] = o.y; org[2] = o.z;
+}
+float oracle_fresnel(const float* incom, const float* normal, float ni, float ne) {
+    return fresnel(V3(incom[0], incom[1], incom[2]), V3(normal[0], normal[1], normal[2]), ni, ne);
+}
+int oracle_photon_bounce(const ppm_scene* s, int tri, const float* ipos, const float* pnrm, const float* pd,
+                         float* rnd, float* flux, float* org, float* dir) {
+    v4 r = V4(rnd[0], rnd[1], rnd[2], rnd[3]);
+    v3 f = V3(flux[0], flux[1], flux[2]), o = V3(0, 0, 0), d = V3(0, 0, 0);
+    int cont = photon_bounce(s, tri, V3(ipos[0], ipos[1], ipos[2]), V3(pnrm[0], pnrm[1], pnrm[2]),
+                             V3(pd[0], pd[1], pd[2]), &r, &f, &o, &d);
+    rnd[0] = r.x; rnd[1] = r.y; rnd[2] = r.z; rnd[3] = r.w;
+    flux[0] = f.x; flux[1] = f.y; flux[2] = f.z;
+    org[0] = o.x; org[1] = o.y; org[2] = o.z;
+    dir[0] = d.x; dir[1] = d.y; dir[2] = d.z;
+    return cont;
+}
+
 static int photon_fragment(Oracle* o, int px) {
     const ppm_scene* s = o->s;
     const int L = o->prm.max_path_length;
@@ -732,94 +880,8 @@ static int photon_fragment(Oracle* o, int px) {
         isect_z = 0.0f;
         cont = 1;
     } else {
-        int tri = isect_tri;
-        v3 V0 = vertex(s, tri, 0), V1 = vertex(s, tri, 1), V2 = vertex(s, tri, 2);
-        v3 gnrm = vnormalize(vcross(vsub(V1, V0), vsub(V2, V0)));
-        const ppm_material* m = &s->materials[s->material[tri]];
-        v3 icol = V3(m->color[0], m->color[1], m->color[2]);
-        int brdf = m->brdf;
-        float eta = m->eta, g = m->g;
-        v3 ipos = V3(PP.x, PP.y, PP.z);
-        v3 inrm = pnrm;
-        v3 pd = V3(PD.x, PD.y, PD.z);
-        const float eps = 1e-6f;
-        /* photontrace.fs:519-544, 562-565: barycentrics from sub-triangle areas, then i.tex */
-        if (textured(s)) {
-            float bx = vlength(vcross(vsub(V1, V0), vsub(ipos, V0)));
-            float by = vlength(vcross(vsub(V2, V1), vsub(ipos, V1)));
-            float bz = vlength(vcross(vsub(V0, V2), vsub(ipos, V2)));
-            float bw = (bx + by) + bz;
-            float wx = bx / bw, wy = by / bw, wz = bz / bw;
-            float tu = (texcoord(s, tri, 2, 0) * wx + texcoord(s, tri, 0, 0) * wy) + texcoord(s, tri, 1, 0) * wz;
-            float tv = (texcoord(s, tri, 2, 1) * wx + texcoord(s, tri, 0, 1) * wy) + texcoord(s, tri, 1, 1) * wz;
-            if (fabsf(tu) < 1e+10f)
-                icol = vmul(icol, oracle_texture3d(s, tu, tv,
-                                                   material_rcp(s) * (((float)s->material[tri] + 0.5f) + 0.25f)));
-        }
-        if (brdf == 0) {
-            float r0 = 6.283184f * gpurnd4(&rndv);
-            float r1 = sqrtf(gpurnd4(&rndv));
-            v3 v = V3(oracle_sinf(r0) * r1, sqrtf(1.0f - r1 * r1), oracle_cosf(r0) * r1);
-            rorg = vadd(ipos, vscale(gnrm, eps));
-            rdir = onb(v, inrm);
-            if (vdot(rdir, gnrm) < 0.0f) rdir = vneg(rdir);
-        } else if (brdf == 1 || brdf == 4) {
-            rorg = vadd(ipos, vscale(gnrm, eps));
-            if (brdf == 4) inrm = glossy_reflect(inrm, inrm, g, &rndv);
-            rdir = vreflect(pd, inrm);
-            if (vdot(rdir, gnrm) < 0.0f) rdir = vneg(rdir);
-        } else if (brdf == 2 || brdf == 5) {
-            if (brdf == 5) inrm = glossy_reflect(inrm, inrm, g, &rndv);
-            float ln = vdot(inrm, pd);
-            if (ln < 0.0f) {
-                float Re = fresnel(vneg(pd), inrm, 1.0f, eta);
-                if (gpurnd4(&rndv) < Re) {
-                    rorg = vadd(ipos, vscale(gnrm, eps));
-                    rdir = vreflect(pd, inrm);
-                    icol = V3(1.0f, 1.0f, 1.0f);
-                    if (vdot(rdir, gnrm) < 0.0f) rdir = vneg(rdir);
-                } else {
-                    rorg = vsub(ipos, vscale(gnrm, eps));
-                    rdir = vrefract(pd, inrm, 1.0f / eta);
-                    if (vdot(rdir, gnrm) > 0.0f) rdir = vneg(rdir);
-                }
-            } else {
-                float Re = fresnel(vneg(pd), vneg(inrm), eta, 1.0f);
-                if (gpurnd4(&rndv) < Re) {
-                    rorg = vsub(ipos, vscale(gnrm, eps));
-                    rdir = vreflect(pd, vneg(inrm));
-                    if (vdot(rdir, vneg(gnrm)) < 0.0f) rdir = vneg(rdir);
-                } else {
-                    rorg = vadd(ipos, vscale(gnrm, eps));
-                    rdir = vrefract(pd, vneg(inrm), eta);
-                    if (vdot(rdir, vneg(gnrm)) > 0.0f) rdir = vneg(rdir);
-                }
-            }
-        } else if (brdf == 3 || brdf == 6) {
-            if (brdf == 6) inrm = glossy_reflect(inrm, inrm, g, &rndv);
-            float Re = fresnel(vneg(pd), inrm, 1.0f, eta);
-            if (gpurnd4(&rndv) < Re) {
-                rorg = vadd(ipos, vscale(gnrm, eps));
-                rdir = vreflect(pd, inrm);
-                icol = V3(1.0f, 1.0f, 1.0f);
-                if (vdot(rdir, gnrm) < 0.0f) rdir = vneg(rdir);
-            } else {
-                float r0 = 6.283184f * gpurnd4(&rndv);
-                float r1 = sqrtf(gpurnd4(&rndv));
-                v3 v = V3(oracle_sinf(r0) * r1, sqrtf(1.0f - r1 * r1), oracle_cosf(r0) * r1);
-                rorg = vadd(ipos, vscale(gnrm, eps));
-                rdir = onb(v, inrm);
-                if (vdot(rdir, gnrm) < 0.0f) rdir = vneg(rdir);
-            }
-        }
-        float p = gl_max(gl_max(icol.x, icol.y), icol.z);
-        if (p < gpurnd4(&rndv)) {
-            cont = 0;
-        } else {
-            cont = 1;
-            icol = V3(icol.x / p, icol.y / p, icol.z / p);
-            flux = vmul(flux, icol);
-        }
+        cont = photon_bounce(s, isect_tri, V3(PP.x, PP.y, PP.z), pnrm, V3(PD.x, PD.y, PD.z), &rndv, &flux, &rorg,
+                             &rdir);
         emitted = 0;
     }
     if (cont) {
