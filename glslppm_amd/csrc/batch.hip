@@ -77,7 +77,8 @@ void queue_batch_pass(ppm_batch* b, hipStream_t st) {
     const int n = b->n_jobs;
     const Job* J = b->d_jobs;
     hipLaunchKernelGGL(k_pass_begin_batch, dim3((n + 63) / 64), dim3(64), 0, st, J, n);
-    hipLaunchKernelGGL(k_eye_batch, dim3(b->n_blocks[kMapTile]), dim3(256), 0, st, J, (const BlockRef*)b->d_map[kMapTile]);
+    hipLaunchKernelGGL(k_eye_batch, dim3(b->n_blocks[kMapTile]), dim3(256), walk_lds_bytes(kWalkStack, 256), st, J,
+                       (const BlockRef*)b->d_map[kMapTile]);
     hipLaunchKernelGGL(k_grid_batch, dim3(n), dim3(1024), 0, st, J);
     hipLaunchKernelGGL(k_clear_slots_batch, dim3(b->n_blocks[kMapClear]), dim3(256), 0, st, J,
                        (const BlockRef*)b->d_map[kMapClear]);
@@ -86,7 +87,8 @@ void queue_batch_pass(ppm_batch* b, hipStream_t st) {
     hipLaunchKernelGGL(k_bucket_scan_batch, dim3(n), dim3(1024), 0, st, J);
     hipLaunchKernelGGL(k_bucket_scatter_batch, dim3(b->n_blocks[kMapPhoton]), dim3(256), 0, st, J,
                        (const BlockRef*)b->d_map[kMapPhoton]);
-    hipLaunchKernelGGL(k_photon_trace_batch, dim3(b->n_blocks[kMapTrace]), dim3(b->trace_block), 0, st, J,
+    hipLaunchKernelGGL(k_photon_trace_batch, dim3(b->n_blocks[kMapTrace]), dim3(b->trace_block),
+                       walk_lds_bytes(kWalkStack, b->trace_block), st, J,
                        (const BlockRef*)b->d_map[kMapTrace]);
     hipLaunchKernelGGL(k_gather_batch, dim3(b->n_blocks[kMapTile]), dim3(256), 0, st, J, (const BlockRef*)b->d_map[kMapTile]);
 }
@@ -380,7 +382,7 @@ uint64_t ppm_scene_footprint(const ppm_scene* scene, const ppm_params* params) {
     const int tex = (scene->texcoords && scene->textures) ? scene->texture_size : 0;
     return ppm_internal::state_footprint(params) +
            ppm_internal::scene_footprint(scene->num_triangles, scene->num_nodes, scene->num_materials, scene->num_lights,
-                                         tex);
+                                         tex, ppm_internal::wide_records(scene));
 }
 
 int ppm_batch_get_tuning(ppm_batch* b, ppm_tuning* t) {

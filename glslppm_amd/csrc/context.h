@@ -49,6 +49,8 @@ private:
 // A scene in the form the kernels read (CGPURT::CreateTextures, gpurt.cpp:47-552), on the host.
 struct PackedScene {
     std::vector<uint4> nodes;
+    std::vector<uint4> wide, wide_pos;  // four-wide records [6][wide_stride][4] and their members' positions
+    int wide_stride = 1;
     std::vector<float4> isect, shade, mats, tex;
     std::vector<uchar4> volume;
     int tex_size = 0;
@@ -170,7 +172,9 @@ int create_context(ppm_ctx* c, std::shared_ptr<SceneDev> scene, const ppm_camera
 void release_context(ppm_ctx* c);
 // HBM bytes of a context's pass state, and of a scene, as the arena allocates them.
 size_t state_footprint(const ppm_params* p);
-size_t scene_footprint(int T, int N, int M, int NL, int tex_size);
+size_t scene_footprint(int T, int N, int M, int NL, int tex_size, int wide_stride);
+// Four-wide records per face of a scene's tree (the packing's wide_stride; T if the tree is invalid).
+int wide_records(const ppm_scene* s);
 int set_error(ppm_ctx* c, int code, const std::string& msg);
 const std::string& create_error();
 // Blocks of each kind for one job.

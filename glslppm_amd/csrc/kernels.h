@@ -7,6 +7,10 @@
 //                                  hi.y hi.z), then the miss position of an interior node or
 //                                  ~triangle of a leaf (whose miss position is the next one); the
 //                                  hit link is the next position, so it is not stored.
+//           wide[6][R][4] uint4    the same trees four-wide: a record is four member slots (the
+//                                  member's box as above, then its record, ~triangle of a leaf, or
+//                                  kWideEmpty), members in pre-order; record 0 is the top
+//           wide_pos[6][R] uint4   the members' pre-order positions (N: empty)
 //           tri_isect[T][5] float4 (v0, brdf), (v1 - v0, material), (v2 - v0, 0), exact leaf box
 //                                  (min.xyz, 0), (max.xyz, 0)
 //           tri_shade[T][6] float4 (v0, n0.x) (v1, n0.y) (v2, n0.z) (n1, n2.x)
@@ -22,8 +26,13 @@
 
 #include "fmath.h"
 
+// Unused slot of a four-wide record.
+constexpr unsigned int kWideEmpty = 0x80000000u;
+
 struct DevScene {
     const uint4* __restrict__ nodes;
+    const uint4* __restrict__ wide;      // [6][wide_stride][4] four-wide records
+    const uint4* __restrict__ wide_pos;  // [6][wide_stride] pre-order positions of their members
     const float4* __restrict__ tri_isect;
     const float4* __restrict__ tri_shade;
     const float4* __restrict__ mats;
@@ -31,6 +40,7 @@ struct DevScene {
     const float* __restrict__ light_cdf;
     int num_nodes;
     int num_lights;
+    int wide_stride;
     float light_area;
     float bsphere_x, bsphere_y, bsphere_z, bsphere_r;
     // textures (null when no triangle is textured): per triangle (u0, v0, u1, v1), (u2, v2, 0, 0);
@@ -156,6 +166,11 @@ struct Tiles {
 // Launch shapes that leave every result unchanged, chosen by the autotuner.
 constexpr int kTraceBlockMax = 512;
 
+// Stack entries per lane of the four-wide walks; deeper walks continue on the pre-order records.
+constexpr int kWalkStack = 16;
+// Dynamic LDS of a walk launch (k_eye, k_photon_trace): the lanes' stacks.
+inline size_t walk_lds_bytes(int walk_stack, int block) { return (size_t)walk_stack * (size_t)block * 4; }
+
 // Everything the kernels need about one job (one scene, camera and state set).  A plain pass loop
 // passes it by value; a batch keeps an array of them in HBM and every launch covers all jobs.
 struct Job {
@@ -174,6 +189,7 @@ struct Job {
     int photon_blocks;   // 256 photon slots per block (k_photon_gen, k_bucket_scatter)
     int clear_blocks;    // 256 hash slots or ray buckets per block (k_clear_slots)
     int trace_blocks;    // rays per block = the launch's block size (k_photon_trace)
+    int walk_stack;      // stack entries per lane of the walks (LDS: walk_stack x block x 4 bytes)
 };
 
 // Block map of a batched launch: entry b = (job, block of that job) for grid block b.

@@ -102,80 +102,131 @@ PPM_DEV bool node_test(uint4 r, f3 rcp, f3 t0, float ht, float& tmin_out) {
     return (tmin <= tmax) && (tmin <= ht) && (tmax >= 0.0f);
 }
 
+// The walk's stack: each lane's entries are walk_lds[k * blockDim.x + lane], k < Job::walk_stack.
+extern __shared__ int walk_lds[];
+
 // Closest hit over the threaded BVH of one cube face (eyeraytrace.fs:67-254,
-// photontrace.fs:123-218): box tests walk the pre-order (hit: next position, miss: the stored
-// link); a leaf whose box is hit gets its triangle tested.
+// photontrace.fs:123-218).  The reference walks the face's pre-order (hit: next position, miss:
+// the stored link) and tests the triangle of each leaf whose box is hit.
+//
+// This walk visits the same tree four-wide: a record holds up to four members (descendants of
+// one node, in pre-order), all four boxes are tested at once, the first member hit is visited
+// next and the others are pushed on the lane's stack in LDS, to be popped in pre-order.  A leaf
+// member is queued when it is reached.  A lane whose stack would overflow walks on over the
+// pre-order records from the first member hit (the pre-order from there covers everything still
+// on its stack).
 //
 // The walk runs in two alternating phases so that a wave's triangle tests (three correctly
 // rounded divisions each) run with many lanes at once instead of whenever any lane meets a leaf.
-// In the box phase a lane that meets a leaf with a hit box queues it, with the box's entry
-// distance tmin, and walks on; the wave leaves the box phase when no lane can step or when more
-// than a quarter of the lanes that were still walking when the phase began hold full queues.  The leaf phase then takes each lane's
-// queued leaves in the order they were met: a leaf whose tmin now exceeds the closest hit is
-// dropped, the others get the reference's slab test of their exact box with the current closest
-// hit, then their triangle test.
+// In the box phase a lane that reaches a leaf queues it and walks on; the wave leaves the box
+// phase when no lane can step or when more than a quarter of the lanes that were still walking
+// when the phase began hold full queues.  The leaf phase then takes each lane's queued leaves in
+// the order they were met: each gets the reference's slab test of its exact box with the current
+// closest hit, then its triangle test.
 //
-// Box-phase boxes are the node records' binary16 boxes, rounded outward.  A slab value
-// rcp * b + t0 is monotone in b, and rounding outward keeps the sign of each bound (which decides
-// the NaN axes of zero direction components), so a rounded box passes whenever the exact one
-// does, with a tmin no larger.  Boxes walked while a leaf was queued were also tested against an
-// older, larger closest-hit distance.  So the lane may enter boxes the reference skips, never
-// skips one the reference enters, and meets the reference's leaves in the reference's order.  A
-// leaf inside a box the reference skips either fails its exact test or has an exact tmin >= that
+// Box-phase boxes are binary16 boxes rounded outward.  A slab value rcp * b + t0 is monotone in
+// b, and rounding outward keeps the sign of each bound (which decides the NaN axes of zero
+// direction components), so a rounded box passes whenever the exact one does, with a tmin no
+// larger.  Boxes are tested against a closest-hit distance no smaller than the reference's at
+// the same point of the pre-order (members are tested before their turn, and leaves queued
+// earlier may still be untested).  So the lane may enter boxes the reference skips, never skips
+// one the reference enters, and reaches the reference's leaves in the reference's order.  A leaf
+// inside a box the reference skips either fails its exact test or has an exact tmin >= that
 // box's tmin (nested boxes, monotone slab test, NaN axes included) > the closest hit at that
 // point; every other leaf is tested exactly as the reference tests it, with the closest hit the
 // reference has at that point.  The triangle tests, their order and so every result, ties
-// included, are the reference's.  Nodes i and i + 1 are fetched together (55% of steps continue
-// at i + 1); each face array ends with a padding node, so the second fetch stays in bounds.
+// included, are the reference's.  On the pre-order records nodes i and i + 1 are fetched
+// together; each face array ends with a padding node, so the second fetch stays in bounds.
+constexpr int kDone = (int)0x80000000;  // no record or leaf left (never ~triangle: T < 2^31 - 1)
+
 template <bool PHOTON>
-PPM_DEV void walk(const DevScene& s, int face, f3 org, f3 dir, f3 rcp, f3 t0, bool cull, Hit& h) {
+PPM_DEV void walk(const DevScene& s, int face, f3 org, f3 dir, f3 rcp, f3 t0, bool cull, Hit& h, int depth) {
     const int n = s.num_nodes;
     const uint4* __restrict__ nd = s.nodes + (size_t)face * (size_t)(n + 1);
-    int i = 0;
-    // queued leaves, newest first: q0 (tmin m0), then q1 (m1) when two are queued
+    const uint4* __restrict__ wr = s.wide + (size_t)face * (size_t)s.wide_stride * 4;
+    int* __restrict__ stk = walk_lds + threadIdx.x;
+    const int stride = blockDim.x;
+    int cur = 0;  // >= 0: the record to visit; else ~triangle of a leaf to queue, or kDone
+    int sp = 0;   // stack entries
+    int i = -1;   // >= 0: the lane walks the pre-order records from position i
+    // queued leaves, newest first: q0, then q1 when two are queued
     int q0 = -1, q1 = -1;
-    float m0 = 0.0f, m1 = 0.0f;
     int nq = 0;
     for (;;) {
-        const int live = __popcll(__ballot(i < n));  // every queue is empty here
+        const int live = __popcll(__ballot(cur != kDone));  // every queue is empty here
         for (;;) {
-            const bool can = i < n && nq < kLeafQueue;
+            const bool can = cur != kDone && nq < kLeafQueue;
             if (__ballot(can) == 0ull) break;
             if (4 * __popcll(__ballot(nq == kLeafQueue)) > live) break;
             if (can) {
-                const uint4 r0 = nd[i], r1 = nd[i + 1];
-                float tm;
-                bool hit = node_test(r0, rcp, t0, h.t, tm);
-                int link = (int)r0.w;
-                bool cont = hit || link < 0;
-                if (hit && link < 0) {
-                    q1 = q0;
-                    m1 = m0;
-                    q0 = ~link;
-                    m0 = tm;
-                    nq++;
-                }
-                int next = cont ? i + 1 : link;
-                if (cont && i + 1 < n && nq < kLeafQueue) {
-                    hit = node_test(r1, rcp, t0, h.t, tm);
-                    link = (int)r1.w;
-                    cont = hit || link < 0;
+                if (i >= 0) {
+                    const uint4 r0 = nd[i], r1 = nd[i + 1];
+                    float tm;
+                    bool hit = node_test(r0, rcp, t0, h.t, tm);
+                    int link = (int)r0.w;
+                    bool cont = hit || link < 0;
                     if (hit && link < 0) {
                         q1 = q0;
-                        m1 = m0;
                         q0 = ~link;
-                        m0 = tm;
                         nq++;
                     }
-                    next = cont ? i + 2 : link;
+                    int next = cont ? i + 1 : link;
+                    if (cont && i + 1 < n && nq < kLeafQueue) {
+                        hit = node_test(r1, rcp, t0, h.t, tm);
+                        link = (int)r1.w;
+                        cont = hit || link < 0;
+                        if (hit && link < 0) {
+                            q1 = q0;
+                            q0 = ~link;
+                            nq++;
+                        }
+                        next = cont ? i + 2 : link;
+                    }
+                    i = next;
+                    if (i >= n) cur = kDone;
+                } else if (cur >= 0) {
+                    const uint4* __restrict__ R = wr + 4 * (size_t)cur;
+                    const uint4 a = R[0], b = R[1], c = R[2], d = R[3];
+                    float tm;
+                    const bool ha = node_test(a, rcp, t0, h.t, tm) && a.w != kWideEmpty;
+                    const bool hb = node_test(b, rcp, t0, h.t, tm) && b.w != kWideEmpty;
+                    const bool hc = node_test(c, rcp, t0, h.t, tm) && c.w != kWideEmpty;
+                    const bool hd = node_test(d, rcp, t0, h.t, tm) && d.w != kWideEmpty;
+                    const int nh = (int)ha + (int)hb + (int)hc + (int)hd;
+                    if (sp + nh - 1 > depth) {
+                        const uint4 ps = s.wide_pos[(size_t)face * (size_t)s.wide_stride + cur];
+                        i = (int)(ha ? ps.x : hb ? ps.y : hc ? ps.z : ps.w);
+                        sp = 0;
+                    } else {
+                        // members hit after the first are pushed last to first
+                        int first = kDone;
+                        if (hd) first = (int)d.w;
+                        if (hc) {
+                            if (first != kDone) stk[stride * sp++] = first;
+                            first = (int)c.w;
+                        }
+                        if (hb) {
+                            if (first != kDone) stk[stride * sp++] = first;
+                            first = (int)b.w;
+                        }
+                        if (ha) {
+                            if (first != kDone) stk[stride * sp++] = first;
+                            first = (int)a.w;
+                        }
+                        cur = first;
+                        if (cur == kDone && sp > 0) cur = stk[stride * --sp];
+                    }
+                } else {
+                    q1 = q0;
+                    q0 = ~cur;
+                    nq++;
+                    cur = sp > 0 ? stk[stride * --sp] : kDone;
                 }
-                i = next;
             }
         }
         if (__ballot(nq > 0) == 0ull) break;
         // the leaves in the order they were met: with two queued, q1 came first
         const int qa = nq == 2 ? q1 : q0, qb = q0;
-        const float ma = nq == 2 ? m1 : m0, mb = m0;
         // both queued leaves are fetched before the first test: the second leaf's tests need the
         // first test's result, its data does not
         float4 x0 = make_float4(0.0f, 0.0f, 0.0f, 0.0f), x1 = x0, x2 = x0, x3 = x0, x4 = x0;
@@ -197,23 +248,23 @@ PPM_DEV void walk(const DevScene& s, int face, f3 org, f3 dir, f3 rcp, f3 t0, bo
             y4 = T[4];
         }
         float tm;
-        if (nq > 0 && ma <= h.t && box_test(rcp, t0, xyz(x3), xyz(x4), h.t, tm))
+        if (nq > 0 && box_test(rcp, t0, xyz(x3), xyz(x4), h.t, tm))
             tri_test<PHOTON>(x0, x1, x2, org, dir, cull, qa, h);
-        if (nq > 1 && mb <= h.t && box_test(rcp, t0, xyz(y3), xyz(y4), h.t, tm))
+        if (nq > 1 && box_test(rcp, t0, xyz(y3), xyz(y4), h.t, tm))
             tri_test<PHOTON>(y0, y1, y2, org, dir, cull, qb, h);
         nq = 0;
     }
 }
 
 template <bool PHOTON>
-PPM_DEV Hit trace(const DevScene& s, f3 org, f3 dir, bool cull) {
+PPM_DEV Hit trace(const DevScene& s, f3 org, f3 dir, bool cull, int depth) {
     const f3 rcp = mk3(1.0f / dir.x, 1.0f / dir.y, 1.0f / dir.z);
     const f3 t0 = mk3(-rcp.x * org.x, -rcp.y * org.y, -rcp.z * org.z);
     Hit h;
     h.tri = -1;
     h.t = PHOTON ? 1.0e30f : 1.0e20f;
     h.bx = h.by = h.bz = 0.0f;
-    walk<PHOTON>(s, cube_face(dir), org, dir, rcp, t0, cull, h);
+    walk<PHOTON>(s, cube_face(dir), org, dir, rcp, t0, cull, h, depth);
     return h;
 }
 
@@ -460,7 +511,7 @@ PPM_DEV void eye(const Job& J, int blk) {
         int last_tri = -1, last_mat = -1;
         float last_t = 1.0e20f;
         for (int j = 0; j < ep.max_path; j++) {
-            const Hit h = trace<false>(s, org, dir, j == 0);
+            const Hit h = trace<false>(s, org, dir, j == 0, J.walk_stack);
             last_tri = h.tri;
             last_t = h.t;
             if (h.tri < 0) {
@@ -1012,7 +1063,7 @@ PPM_DEV void photon_trace(const Job& J, int blk) {
     W.h.tri = -1;
     W.h.t = 1.0e30f;
     W.h.bx = W.h.by = W.h.bz = 0.0f;
-    walk<true>(s, W.face, W.org, W.dir, W.rcp, W.t0, false, W.h);
+    walk<true>(s, W.face, W.org, W.dir, W.rcp, W.t0, false, W.h, J.walk_stack);
     photon_finish(s, J.pp, J.ps, J.h, g, x.z, W, mk3(o.w, d.w, __int_as_float(x.x)), x.y);
 }
 

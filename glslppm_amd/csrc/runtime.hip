@@ -151,10 +151,23 @@ uint16_t half_outward(float x, bool down) {
 // interior node or ~triangle for a leaf, whose miss position is always the next one.  The hit
 // link of position i is i + 1.  Each face array has N + 1 records: the last is padding that the
 // walk's look-ahead fetch may read but never visits.
-int pack_nodes(const ppm_scene* s, std::vector<uint4>& out) {
+//
+// The same trees, four-wide: each face's pre-order is read as a tree (the children of position i
+// are i + 1, then the miss position of each child in turn up to the miss position of i), and
+// every record holds a cut of up to four descendants of one interior node, found by replacing the
+// interior member of largest surface area by its children while they fit, in pre-order.  Record 0
+// holds the cut below the top of the tree.  A record is four slots in the node format above whose
+// w is the child's record (an interior member), ~triangle (a leaf) or kWideEmpty; records are
+// numbered in pre-order.  wide_pos holds each record's four members' pre-order positions (N for
+// an empty slot), where a walk that runs out of stack continues on the pre-order records.
+int pack_nodes(const ppm_scene* s, std::vector<uint4>& out, std::vector<uint4>& wide, std::vector<uint4>& wide_pos,
+               int* wide_stride) {
     const int N = s->num_nodes;
     out.assign((size_t)6 * (N + 1), make_uint4(0, 0, 0, 0));
-    std::vector<int> order(N), pos(N);
+    std::vector<int> order(N), pos(N), mpos(N), tri(N);
+    std::vector<uint4> box(N);
+    std::vector<std::vector<uint4>> fw(6), fp(6);
+    int stride = 1;
     for (int f = 0; f < 6; f++) {
         const int32_t* L = s->node_links + (size_t)f * N * 2;
         std::fill(pos.begin(), pos.end(), -1);
@@ -169,19 +182,93 @@ int pack_nodes(const ppm_scene* s, std::vector<uint4>& out) {
         for (int i = 0; i < N; i++) {
             const int nk = order[i];
             const int miss = L[2 * nk + 1];
-            const int mpos = miss < 0 ? N : pos[miss];
-            if (miss >= 0 && mpos <= i) return set_err(nullptr, PPM_E_INVAL, "BVH miss link points backwards");
+            mpos[i] = miss < 0 ? N : pos[miss];
+            if (miss >= 0 && mpos[i] <= i) return set_err(nullptr, PPM_E_INVAL, "BVH miss link points backwards");
             const float* b = s->node_bounds + (size_t)nk * 6;
-            const int tri = s->node_triangle[nk];
-            if (tri >= 0 && mpos != i + 1) return set_err(nullptr, PPM_E_INVAL, "BVH leaf miss link is not the next node");
+            tri[i] = s->node_triangle[nk];
+            if (tri[i] >= 0 && mpos[i] != i + 1) return set_err(nullptr, PPM_E_INVAL, "BVH leaf miss link is not the next node");
             uint16_t h[6];
             for (int k = 0; k < 6; k++) h[k] = half_outward(b[k], k < 3);
-            out[(size_t)f * (N + 1) + i] =
-                make_uint4(h[0] | (uint32_t)h[1] << 16, h[2] | (uint32_t)h[3] << 16, h[4] | (uint32_t)h[5] << 16,
-                           tri >= 0 ? (uint32_t)~tri : (uint32_t)mpos);
+            box[i] = make_uint4(h[0] | (uint32_t)h[1] << 16, h[2] | (uint32_t)h[3] << 16, h[4] | (uint32_t)h[5] << 16, 0);
+            out[(size_t)f * (N + 1) + i] = box[i];
+            out[(size_t)f * (N + 1) + i].w = tri[i] >= 0 ? (uint32_t)~tri[i] : (uint32_t)mpos[i];
         }
         out[(size_t)f * (N + 1) + N] = make_uint4(0, 0, 0, (uint32_t)(N + 1));
+
+        // children of position x (x = -1: the top of the tree), checked to tile x's range
+        auto children = [&](int x, std::vector<int>& ch) -> bool {
+            ch.clear();
+            const int end = x < 0 ? N : mpos[x];
+            int c = x + 1;
+            while (c < end) {
+                ch.push_back(c);
+                if (ch.size() > 4) return false;
+                c = mpos[c];
+            }
+            return c == end && (x < 0 || !ch.empty());
+        };
+        auto area = [&](int i) {
+            const float* b = s->node_bounds + (size_t)order[i] * 6;
+            const double dx = (double)b[3] - b[0], dy = (double)b[4] - b[1], dz = (double)b[5] - b[2];
+            return dx * dy + dy * dz + dz * dx;
+        };
+        std::vector<int> rec_of(N, -1), todo{-1}, ch, cut;
+        std::vector<std::array<int, 4>> cuts;
+        size_t members = 0;
+        while (!todo.empty()) {
+            const int x = todo.back();
+            todo.pop_back();
+            if (!children(x, cut)) return set_err(nullptr, PPM_E_INVAL, "BVH links do not form a binary tree");
+            for (;;) {
+                int best = -1;
+                double ba = -1.0;
+                for (size_t j = 0; j < cut.size(); j++)
+                    if (tri[cut[j]] < 0 && area(cut[j]) > ba) {
+                        ba = area(cut[j]);
+                        best = (int)j;
+                    }
+                if (best < 0) break;
+                if (!children(cut[best], ch)) return set_err(nullptr, PPM_E_INVAL, "BVH links do not form a binary tree");
+                if (cut.size() - 1 + ch.size() > 4) break;
+                cut.erase(cut.begin() + best);
+                cut.insert(cut.begin() + best, ch.begin(), ch.end());
+            }
+            if (x >= 0) rec_of[x] = (int)cuts.size();
+            std::array<int, 4> a = {-1, -1, -1, -1};
+            for (size_t j = 0; j < cut.size(); j++) a[j] = cut[j];
+            cuts.push_back(a);
+            members += cut.size();
+            if (members > (size_t)N) return set_err(nullptr, PPM_E_INVAL, "BVH links do not form a binary tree");
+            for (size_t j = cut.size(); j-- > 0;)
+                if (tri[cut[j]] < 0) todo.push_back(cut[j]);
+        }
+        std::vector<uint4>& W = fw[f];
+        std::vector<uint4>& WP = fp[f];
+        W.resize(cuts.size() * 4);
+        WP.resize(cuts.size());
+        for (size_t r = 0; r < cuts.size(); r++) {
+            uint32_t p4[4];
+            for (int j = 0; j < 4; j++) {
+                const int m = cuts[r][j];
+                uint4 slot = make_uint4(0, 0, 0, kWideEmpty);
+                if (m >= 0) {
+                    slot = box[m];
+                    slot.w = tri[m] >= 0 ? (uint32_t)~tri[m] : (uint32_t)rec_of[m];
+                }
+                W[r * 4 + j] = slot;
+                p4[j] = m >= 0 ? (uint32_t)m : (uint32_t)N;
+            }
+            WP[r] = make_uint4(p4[0], p4[1], p4[2], p4[3]);
+        }
+        stride = std::max(stride, (int)cuts.size());
     }
+    wide.assign((size_t)6 * stride * 4, make_uint4(0, 0, 0, kWideEmpty));
+    wide_pos.assign((size_t)6 * stride, make_uint4(N, N, N, N));
+    for (int f = 0; f < 6; f++) {
+        std::copy(fw[f].begin(), fw[f].end(), wide.begin() + (size_t)f * stride * 4);
+        std::copy(fp[f].begin(), fp[f].end(), wide_pos.begin() + (size_t)f * stride);
+    }
+    *wide_stride = stride;
     return PPM_OK;
 }
 
@@ -307,7 +394,7 @@ void queue_pass(ppm_ctx* c, hipStream_t st, bool eye) {
     hipLaunchKernelGGL(k_pass_begin, dim3(1), dim3(64), 0, st, J);
     if (eye) {
         hipEvent_t e0 = prof_begin(c);
-        hipLaunchKernelGGL(k_eye, dim3(J.tile_blocks), dim3(256), 0, st, J);
+        hipLaunchKernelGGL(k_eye, dim3(J.tile_blocks), dim3(256), walk_lds_bytes(J.walk_stack, 256), st, J);
         hipLaunchKernelGGL(k_grid, dim3(1), dim3(1024), 0, st, J);
         prof_end(c, PPM_KERNEL_EYE, e0);
     }
@@ -318,7 +405,8 @@ void queue_pass(ppm_ctx* c, hipStream_t st, bool eye) {
     hipLaunchKernelGGL(k_photon_gen, dim3(J.photon_blocks), dim3(256), 0, st, J);
     hipLaunchKernelGGL(k_bucket_scan, dim3(1), dim3(1024), 0, st, J);
     hipLaunchKernelGGL(k_bucket_scatter, dim3(J.photon_blocks), dim3(256), 0, st, J);
-    hipLaunchKernelGGL(k_photon_trace, dim3(J.trace_blocks), dim3(c->trace_block), 0, st, J);
+    hipLaunchKernelGGL(k_photon_trace, dim3(J.trace_blocks), dim3(c->trace_block),
+                       walk_lds_bytes(J.walk_stack, c->trace_block), st, J);
     prof_end(c, PPM_KERNEL_PHOTON, e2);
     hipEvent_t e3 = prof_begin(c);
     hipLaunchKernelGGL(k_gather, dim3(J.tile_blocks), dim3(256), 0, st, J);
@@ -496,16 +584,23 @@ size_t state_footprint(const ppm_params* p) {
     return tot;
 }
 
-size_t scene_footprint(int T, int N, int M, int NL, int tex_size) {
-    const size_t S = (size_t)tex_size;
-    const size_t sizes[] = {(size_t)6 * (N + 1) * 16, (size_t)T * 80, (size_t)T * 96, (size_t)M * 32, (size_t)NL * 4,
-                            (size_t)NL * 4, S ? (size_t)T * 32 : 0, (size_t)M * S * S * 4};
+size_t scene_footprint(int T, int N, int M, int NL, int tex_size, int wide_stride) {
+    const size_t S = (size_t)tex_size, R = (size_t)wide_stride;
+    const size_t sizes[] = {(size_t)6 * (N + 1) * 16, 6 * R * 64, 6 * R * 16, (size_t)T * 80, (size_t)T * 96,
+                            (size_t)M * 32, (size_t)NL * 4, (size_t)NL * 4, S ? (size_t)T * 32 : 0, (size_t)M * S * S * 4};
     size_t tot = 0;
     for (size_t b : sizes) tot += Arena::round_size(b ? b : 16);
     return tot;
 }
 
-size_t PackedScene::device_bytes() const { return scene_footprint(T, N, M, NL, tex_size); }
+size_t PackedScene::device_bytes() const { return scene_footprint(T, N, M, NL, tex_size, wide_stride); }
+
+int wide_records(const ppm_scene* s) {
+    std::vector<uint4> nodes, wide, wide_pos;
+    int stride = 0;
+    if (pack_nodes(s, nodes, wide, wide_pos, &stride)) return std::max(1, s->num_triangles);
+    return stride;
+}
 
 SceneDev::~SceneDev() {
     for (void* p : allocs) {
@@ -516,7 +611,7 @@ SceneDev::~SceneDev() {
 
 int pack_scene(const ppm_scene* s, PackedScene* out) {
     int rc;
-    if ((rc = pack_nodes(s, out->nodes))) return rc;
+    if ((rc = pack_nodes(s, out->nodes, out->wide, out->wide_pos, &out->wide_stride))) return rc;
     pack_triangles(s, out->isect, out->shade);
     out->T = s->num_triangles;
     out->N = s->num_nodes;
@@ -576,9 +671,11 @@ int upload_scene(const PackedScene& ps, Arena* arena, hipStream_t up, hipStream_
         }
         return PPM_OK;
     };
-    void *d_nodes, *d_isect, *d_shade, *d_mats, *d_ltri, *d_lcdf, *d_tex = nullptr, *d_vol = nullptr;
+    void *d_nodes, *d_wide, *d_wpos, *d_isect, *d_shade, *d_mats, *d_ltri, *d_lcdf, *d_tex = nullptr, *d_vol = nullptr;
     int rc;
     if ((rc = put(ps.nodes.data(), ps.nodes.size() * 16, &d_nodes)) ||
+        (rc = put(ps.wide.data(), ps.wide.size() * 16, &d_wide)) ||
+        (rc = put(ps.wide_pos.data(), ps.wide_pos.size() * 16, &d_wpos)) ||
         (rc = put(ps.isect.data(), ps.isect.size() * 16, &d_isect)) ||
         (rc = put(ps.shade.data(), ps.shade.size() * 16, &d_shade)) ||
         (rc = put(ps.mats.data(), ps.mats.size() * 16, &d_mats)) ||
@@ -591,6 +688,9 @@ int upload_scene(const PackedScene& ps, Arena* arena, hipStream_t up, hipStream_
     DevScene& sc = sd->s;
     std::memset(&sc, 0, sizeof(sc));
     sc.nodes = (const uint4*)d_nodes;
+    sc.wide = (const uint4*)d_wide;
+    sc.wide_pos = (const uint4*)d_wpos;
+    sc.wide_stride = ps.wide_stride;
     sc.tri_isect = (const float4*)d_isect;
     sc.tri_shade = (const float4*)d_shade;
     sc.mats = (const float4*)d_mats;
@@ -669,6 +769,7 @@ int create_context(ppm_ctx* c, std::shared_ptr<SceneDev> scene, const ppm_camera
     J.photon_blocks = photon_blocks(p);
     J.clear_blocks = clear_blocks(p);
     set_launch_shape(c, 256, 3);  // 8x8-pixel waves; the tuner may change both
+    J.walk_stack = kWalkStack;
     c->use_graph = (p->flags & PPM_FLAG_GRAPH) != 0;
 
     const size_t Q = c->Q, P = c->P, H = c->H;
