@@ -325,6 +325,8 @@ def main():
     ap.add_argument("--graph", action="store_true", help="plain workloads: replay passes as hipGraphs")
     ap.add_argument("--autotune", action="store_true",
                     help="plain workloads: pick launch shapes with ppm_autotune during the warm-up")
+    ap.add_argument("--tuning", default=None,
+                    help='plain workloads: launch shapes to set first, as JSON, e.g. \'{"walk_stack": 8}\'')
     ap.add_argument("--cpu-passes", type=int, default=50, help="passes of the CPU baseline sample (0: skip)")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="first run a plain device copy of MIB MiB (torch), the known-size reference for "
@@ -365,6 +367,8 @@ def main():
     L = params.max_path_length
 
     tuning = None
+    if args.tuning:
+        r.set_tuning(**json.loads(args.tuning))
     if args.autotune:
         tuning = r.autotune()
     r.run(args.warmup)

@@ -77,7 +77,8 @@ class BatchConfig(ctypes.Structure):
 
 class Tuning(ctypes.Structure):
     _fields_ = [("trace_block", ctypes.c_int32), ("tile_wave_width", ctypes.c_int32),
-                ("passes_per_graph", ctypes.c_uint32), ("ms_per_pass", ctypes.c_float)]
+                ("passes_per_graph", ctypes.c_uint32), ("walk_stack", ctypes.c_int32),
+                ("ms_per_pass", ctypes.c_float)]
 
     def update(self, **kw):
         for k, v in kw.items():
@@ -413,8 +414,8 @@ class Renderer:
         return t.as_dict()
 
     def set_tuning(self, **fields):
-        """Changes the named launch-shape fields (trace_block, tile_wave_width, passes_per_graph);
-        the others keep their values."""
+        """Changes the named launch-shape fields (trace_block, tile_wave_width, passes_per_graph,
+        walk_stack); the others keep their values."""
         t = Tuning()
         self._check(library().ppm_get_tuning(self._h, ctypes.byref(t)))
         self._check(library().ppm_set_tuning(self._h, ctypes.byref(t.update(**fields))))
@@ -566,8 +567,8 @@ class Batch:
         return t.as_dict()
 
     def set_tuning(self, **fields):
-        """Changes the named launch-shape fields (trace_block, tile_wave_width, passes_per_graph);
-        the others keep their values."""
+        """Changes the named launch-shape fields (trace_block, tile_wave_width, passes_per_graph,
+        walk_stack); the others keep their values."""
         t = Tuning()
         self._check(library().ppm_batch_get_tuning(self._h, ctypes.byref(t)))
         self._check(library().ppm_batch_set_tuning(self._h, ctypes.byref(t.update(**fields))))

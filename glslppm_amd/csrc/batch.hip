@@ -77,7 +77,7 @@ void queue_batch_pass(ppm_batch* b, hipStream_t st) {
     const int n = b->n_jobs;
     const Job* J = b->d_jobs;
     hipLaunchKernelGGL(k_pass_begin_batch, dim3((n + 63) / 64), dim3(64), 0, st, J, n);
-    hipLaunchKernelGGL(k_eye_batch, dim3(b->n_blocks[kMapTile]), dim3(256), walk_lds_bytes(kWalkStack, 256), st, J,
+    hipLaunchKernelGGL(k_eye_batch, dim3(b->n_blocks[kMapTile]), dim3(256), walk_lds_bytes(b->walk_stack, 256), st, J,
                        (const BlockRef*)b->d_map[kMapTile]);
     hipLaunchKernelGGL(k_grid_batch, dim3(n), dim3(1024), 0, st, J);
     hipLaunchKernelGGL(k_clear_slots_batch, dim3(b->n_blocks[kMapClear]), dim3(256), 0, st, J,
@@ -88,7 +88,7 @@ void queue_batch_pass(ppm_batch* b, hipStream_t st) {
     hipLaunchKernelGGL(k_bucket_scatter_batch, dim3(b->n_blocks[kMapPhoton]), dim3(256), 0, st, J,
                        (const BlockRef*)b->d_map[kMapPhoton]);
     hipLaunchKernelGGL(k_photon_trace_batch, dim3(b->n_blocks[kMapTrace]), dim3(b->trace_block),
-                       walk_lds_bytes(kWalkStack, b->trace_block), st, J,
+                       walk_lds_bytes(b->walk_stack, b->trace_block), st, J,
                        (const BlockRef*)b->d_map[kMapTrace]);
     hipLaunchKernelGGL(k_gather_batch, dim3(b->n_blocks[kMapTile]), dim3(256), 0, st, J, (const BlockRef*)b->d_map[kMapTile]);
 }
@@ -165,7 +165,7 @@ int batch_add_resident(ppm_batch* b, std::shared_ptr<SceneDev> scene, const ppm_
         delete c;
         return berr(b, rc, m);
     }
-    set_launch_shape(c, b->trace_block, b->wave_shift);
+    set_launch_shape(c, b->trace_block, b->wave_shift, b->walk_stack);
     b->slots.push_back(c);
     *job = (int32_t)(b->slots.size() - 1);
     b->dirty = true;
@@ -390,6 +390,7 @@ int ppm_batch_get_tuning(ppm_batch* b, ppm_tuning* t) {
     t->trace_block = b->trace_block;
     t->tile_wave_width = 1 << b->wave_shift;
     t->passes_per_graph = b->passes_per_graph;
+    t->walk_stack = b->walk_stack;
     t->ms_per_pass = 0.0f;
     return PPM_OK;
 }
@@ -401,8 +402,9 @@ int ppm_batch_set_tuning(ppm_batch* b, const ppm_tuning* t) {
     b->trace_block = t->trace_block;
     b->wave_shift = sh;
     b->passes_per_graph = t->passes_per_graph;
+    b->walk_stack = t->walk_stack;
     for (ppm_ctx* c : b->slots)
-        if (c) ppm_internal::set_launch_shape(c, b->trace_block, b->wave_shift);
+        if (c) ppm_internal::set_launch_shape(c, b->trace_block, b->wave_shift, b->walk_stack);
     b->dirty = true;
     return PPM_OK;
 }

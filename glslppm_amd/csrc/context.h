@@ -115,6 +115,7 @@ struct ppm_batch {
     uint32_t passes_per_graph = 1;  // 0: plain launches
     int trace_block = 256;          // launch shape shared by every job (ppm_batch_set_tuning)
     int wave_shift = 3;
+    int walk_stack = kWalkStack;
     std::vector<ppm_ctx*> slots;    // job id -> context (null once removed)
     // device tables of the current job set
     bool dirty = true;
@@ -185,7 +186,7 @@ int clear_blocks(const ppm_params* p);
 // Checks a tuning and converts it to (trace block, wave shift); PPM_E_INVAL if out of range.
 int check_tuning(const ppm_tuning* t, int* wave_shift);
 // Sets the launch shape of a context's job (invalidates its captured graphs).
-void set_launch_shape(ppm_ctx* c, int trace_block, int wave_shift);
+void set_launch_shape(ppm_ctx* c, int trace_block, int wave_shift, int walk_stack);
 // Coordinate search over the launch shapes: for each knob in turn, `trial(t)` applies t, runs
 // passes and returns their milliseconds; the fastest value of each knob is kept.  The winner is
 // applied last and written to *best.
@@ -200,17 +201,20 @@ int autotune(ppm_tuning start, bool batch, Trial trial, ppm_tuning* best) {
     const int widths[] = {4, 8, 16, 32, 64};
     const uint32_t graphs_batch[] = {0, 1, 4};
     const uint32_t graphs_plain[] = {0, 1};
-    for (int knob = 0; knob < 3; knob++) {
-        const int n = knob == 0 ? 4 : knob == 1 ? 5 : (batch ? 3 : 2);
+    const int stacks[] = {8, 16, 32};
+    for (int knob = 0; knob < 4; knob++) {
+        const int n = knob == 0 ? 4 : knob == 1 ? 5 : knob == 2 ? (batch ? 3 : 2) : 3;
         const ppm_tuning base = cur;  // the value this knob starts from is already timed
         for (int k = 0; k < n; k++) {
             ppm_tuning t = cur;
             if (knob == 0) t.trace_block = blocks[k];
             if (knob == 1) t.tile_wave_width = widths[k];
             if (knob == 2) t.passes_per_graph = batch ? graphs_batch[k] : graphs_plain[k];
+            if (knob == 3) t.walk_stack = stacks[k];
             if ((knob == 0 && t.trace_block == base.trace_block) ||
                 (knob == 1 && t.tile_wave_width == base.tile_wave_width) ||
-                (knob == 2 && t.passes_per_graph == base.passes_per_graph))
+                (knob == 2 && t.passes_per_graph == base.passes_per_graph) ||
+                (knob == 3 && t.walk_stack == base.walk_stack))
                 continue;
             if ((rc = trial(t, &ms))) return rc;
             if (ms < cur.ms_per_pass) {

@@ -166,8 +166,10 @@ struct Tiles {
 // Launch shapes that leave every result unchanged, chosen by the autotuner.
 constexpr int kTraceBlockMax = 512;
 
-// Stack entries per lane of the four-wide walks; deeper walks continue on the pre-order records.
+// Stack entries per lane of the four-wide walks (default, most: 32 x 512 lanes x 4 bytes = 64 KiB
+// of LDS); deeper walks continue on the pre-order records.
 constexpr int kWalkStack = 16;
+constexpr int kWalkStackMax = 32;
 // Dynamic LDS of a walk launch (k_eye, k_photon_trace): the lanes' stacks.
 inline size_t walk_lds_bytes(int walk_stack, int block) { return (size_t)walk_stack * (size_t)block * 4; }
 

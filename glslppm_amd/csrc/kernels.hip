@@ -215,6 +215,13 @@ PPM_DEV void walk(const DevScene& s, int face, f3 org, f3 dir, f3 rcp, f3 t0, bo
                         }
                         cur = first;
                         if (cur == kDone && sp > 0) cur = stk[stride * --sp];
+                        if (cur < 0 && cur != kDone) {
+                            // a leaf next: queued in the same step (the queue had room)
+                            q1 = q0;
+                            q0 = ~cur;
+                            nq++;
+                            cur = sp > 0 ? stk[stride * --sp] : kDone;
+                        }
                     }
                 } else {
                     q1 = q0;
@@ -1175,8 +1182,9 @@ PPM_DEV void gather(const Job& J, int blk) {
         const BlockRef m = map[blockIdx.x];                                                         \
         body(jobs[m.job], m.block);                                                                 \
     }
-// The eye kernel's bounce loop keeps ~100 VGPRs live; asking for 5 waves per SIMD fits it in 96
-// without spilling, one wave more than it would get, which shortens its dependent-load stalls.
+// A floor on waves per SIMD caps a kernel's VGPRs: the eye kernel's bounce loop fits 96 at 5 waves
+// (one wave more than it would get), the photon walk 72 at 7 (one more), which shortens their
+// dependent-load stalls; 8 waves would spill the walk.
 #define PPM_JOB_KERNEL_WAVES(name, body, threads, waves)                                          \
     __global__ __launch_bounds__(threads) __attribute__((amdgpu_waves_per_eu(waves))) void name(Job J) { \
         body(J, (int)blockIdx.x);                                                                   \
@@ -1198,7 +1206,7 @@ PPM_JOB_KERNEL1(k_grid, grid, 1024)
 PPM_JOB_KERNEL(k_photon_gen, photon_gen_blk, kBlock)
 PPM_JOB_KERNEL1(k_bucket_scan, bucket_scan, 1024)
 PPM_JOB_KERNEL(k_bucket_scatter, bucket_scatter, kBlock)
-PPM_JOB_KERNEL(k_photon_trace, photon_trace, kTraceBlockMax)
+PPM_JOB_KERNEL_WAVES(k_photon_trace, photon_trace, kTraceBlockMax, 7)
 PPM_JOB_KERNEL(k_gather, gather, kBlock)
 
 __global__ void k_pass_begin(Job J) {

@@ -543,12 +543,14 @@ int check_tuning(const ppm_tuning* t, int* wave_shift) {
         if (t->tile_wave_width == (1 << k)) sh = k;
     if (sh < 0) return PPM_E_INVAL;
     if (t->passes_per_graph > 64) return PPM_E_INVAL;
+    if (t->walk_stack < 0 || t->walk_stack > kWalkStackMax) return PPM_E_INVAL;
     if (wave_shift) *wave_shift = sh;
     return PPM_OK;
 }
 
-void set_launch_shape(ppm_ctx* c, int trace_block, int wave_shift) {
+void set_launch_shape(ppm_ctx* c, int trace_block, int wave_shift, int walk_stack) {
     Job& J = c->job;
+    J.walk_stack = walk_stack;
     J.tl.wave_shift = wave_shift;
     J.tl.tiles_x = (J.tl.width + (2 << wave_shift) - 1) / (2 << wave_shift);
     J.tile_blocks = tile_blocks(J.tl.width, J.tl.height, wave_shift);
@@ -768,8 +770,7 @@ int create_context(ppm_ctx* c, std::shared_ptr<SceneDev> scene, const ppm_camera
     J.tl.height = p->image_height;
     J.photon_blocks = photon_blocks(p);
     J.clear_blocks = clear_blocks(p);
-    set_launch_shape(c, 256, 3);  // 8x8-pixel waves; the tuner may change both
-    J.walk_stack = kWalkStack;
+    set_launch_shape(c, 256, 3, kWalkStack);  // 8x8-pixel waves; the tuner may change all three
     c->use_graph = (p->flags & PPM_FLAG_GRAPH) != 0;
 
     const size_t Q = c->Q, P = c->P, H = c->H;
@@ -1117,6 +1118,7 @@ int ppm_get_tuning(ppm_ctx* c, ppm_tuning* t) {
     t->trace_block = c->trace_block;
     t->tile_wave_width = 1 << c->job.tl.wave_shift;
     t->passes_per_graph = c->use_graph ? 1u : 0u;
+    t->walk_stack = c->job.walk_stack;
     t->ms_per_pass = 0.0f;
     return PPM_OK;
 }
@@ -1126,7 +1128,7 @@ int ppm_set_tuning(ppm_ctx* c, const ppm_tuning* t) {
     int sh = 0;
     if (ppm_internal::check_tuning(t, &sh) || t->passes_per_graph > 1)
         return set_err(c, PPM_E_INVAL, "tuning out of range");
-    ppm_internal::set_launch_shape(c, t->trace_block, sh);
+    ppm_internal::set_launch_shape(c, t->trace_block, sh, t->walk_stack);
     c->use_graph = t->passes_per_graph > 0;
     return PPM_OK;
 }

@@ -207,13 +207,15 @@ int ppm_profile_read(ppm_ctx* ctx, double* ms, uint64_t* launches);
 /* Launch shapes.  No counterpart in the reference (GL rasterises one fragment per texel).  Every
  * shape gives bit-identical results; only the speed differs.  ppm_autotune runs real passes of
  * the context (they count as passes) under each candidate value of one knob at a time -- photon
- * walk block size, pixel tile shape, graph replay -- and keeps the fastest; call it before the
- * passes whose speed matters. */
+ * walk block size, pixel tile shape, graph replay, walk stack depth -- and keeps the fastest;
+ * call it before the passes whose speed matters. */
 typedef struct ppm_tuning {
     int32_t trace_block;       /* threads per block of the photon walk: 64, 128, ..., 512 */
     int32_t tile_wave_width;   /* a wave's eye/gather tile is w x 64/w pixels: 1, 2, 4, ..., 64 */
     uint32_t passes_per_graph; /* plain loop: 0 launches, 1 hipGraph per pass (PPM_FLAG_GRAPH);
                                   batch: passes per hipGraph, 0 for plain launches */
+    int32_t walk_stack;        /* LDS stack entries per lane of the BVH walks, 0..32; deeper
+                                  walks go on over the pre-order node records */
     float ms_per_pass;         /* ppm_autotune / ppm_batch_autotune: time of the chosen shape */
 } ppm_tuning;
 

@@ -127,9 +127,12 @@ def test_plain_loop_graph_replay_equals_launches():
 
 @pytest.mark.parametrize("shape", [dict(trace_block=64), dict(trace_block=512), dict(tile_wave_width=1),
                                    dict(tile_wave_width=4), dict(tile_wave_width=64),
-                                   dict(trace_block=128, tile_wave_width=16, passes_per_graph=1)])
+                                   dict(trace_block=128, tile_wave_width=16, passes_per_graph=1),
+                                   dict(walk_stack=0), dict(walk_stack=1), dict(walk_stack=2),
+                                   dict(walk_stack=32, trace_block=512)])
 def test_launch_shapes_do_not_change_results(shape):
-    """Odd image sizes so that partial tiles and partial ray blocks occur under every shape."""
+    """Odd image sizes so that partial tiles and partial ray blocks occur under every shape; walk
+    stacks of 0-2 entries send most walks on over the pre-order records part way."""
     print("shape %s" % shape, flush=True)
     m = spheres(5, 4, 2, 50, 37)
     p = g.default_params(64, image_width=50, image_height=37, photon_width=40, photon_height=33, max_path_length=6)
@@ -148,10 +151,11 @@ def test_autotune_keeps_results_and_picks_a_valid_shape():
     r = g.Renderer(m, p)
     t = r.autotune(passes_per_trial=3)
     assert t["trace_block"] in (64, 128, 256, 512) and t["tile_wave_width"] in (1, 2, 4, 8, 16, 32, 64)
+    assert t["walk_stack"] in (8, 16, 32)
     assert t["ms_per_pass"] > 0
     done = r.counters()["frame_count"]
-    # 10 trials (the start, 3 + 4 + 1 alternatives, the winner), each 1 + 3 passes
-    assert done == 10 * 4
+    # 12 trials (the start, 3 + 4 + 1 + 2 alternatives, the winner), each 1 + 3 passes
+    assert done == 12 * 4
     a = g.Renderer(m, p)
     a.run(done)
     rep = compare_all(r, a, STATE_BUFFERS)
@@ -173,7 +177,7 @@ def test_tuning_out_of_range_is_refused():
     m = spheres(5, 4, 2, 32, 32)
     r = g.Renderer(m, g.default_params(32, max_path_length=4))
     for bad in (dict(trace_block=96), dict(trace_block=1024), dict(tile_wave_width=3), dict(passes_per_graph=2),
-                dict(ms_per_pass=1.0), dict(no_such_field=1)):
+                dict(walk_stack=-1), dict(walk_stack=33), dict(ms_per_pass=1.0), dict(no_such_field=1)):
         with pytest.raises(g.PPMError):
             r.set_tuning(**bad)
     assert r.tuning()["trace_block"] == 256
