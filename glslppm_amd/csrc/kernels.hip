@@ -36,41 +36,51 @@ struct Hit {
     float bx, by, bz;  // Barycentric = uvt.ywx
 };
 
-// Slab test of a box (eyeraytrace.fs:92-116, photontrace.fs:148-172) against the closest hit so
-// far; tmin is returned for the deferred re-check of leaves.
-PPM_DEV bool box_test(f3 rcp, f3 t0, f3 a, f3 b, float ht, float& tmin_out) {
-    const float lx = rcp.x * a.x + t0.x, ly = rcp.y * a.y + t0.y, lz = rcp.z * a.z + t0.z;
-    const float hx = rcp.x * b.x + t0.x, hy = rcp.y * b.y + t0.y, hz = rcp.z * b.z + t0.z;
-    const float tmin = max1(max1(min1(lx, hx), min1(ly, hy)), min1(lz, hz));
-    const float tmax = min1(min1(max1(lx, hx), max1(ly, hy)), max1(lz, hz));
-    tmin_out = tmin;
-    return (tmin <= tmax) && (tmin <= ht) && (tmax >= 0.0f);
-}
+// The closest hit while a walk runs; the barycentrics of the final one are computed at the end.
+struct Closest {
+    int tri;
+    float t;
+};
 
-// Triangle test of a leaf (eyeraytrace.fs:118-147, photontrace.fs:174-205).  Eye rays cull back
-// faces on their first segment; photon rays skip light triangles.
-template <bool PHOTON>
-PPM_DEV void tri_test(float4 q0, float4 q1, float4 q2, f3 org, f3 dir, bool cull, int tri, Hit& h) {
+// The Moller-Trumbore terms of a triangle record for a ray (eyeraytrace.fs:118-147,
+// photontrace.fs:174-205), in the shaders' operation order.
+PPM_DEV void tri_terms(float4 q0, float4 q1, float4 q2, f3 org, f3 dir, float& det, float& u, float& v, float& t) {
     const f3 e0 = xyz(q1), e1 = xyz(q2);
     const f3 pv = cross3(dir, e1);
-    const float det = dot3(e0, pv);
+    det = dot3(e0, pv);
+    const f3 tv = org - xyz(q0);
+    const f3 qv = cross3(tv, e0);
+    u = dot3(tv, pv) / det;
+    v = dot3(dir, qv) / det;
+    t = dot3(e1, qv) / det;
+}
+
+// Triangle test of a leaf.  Eye rays cull back faces on their first segment; photon rays skip
+// light triangles.
+template <bool PHOTON>
+PPM_DEV void tri_test(float4 q0, float4 q1, float4 q2, f3 org, f3 dir, bool cull, int tri, Closest& h) {
+    float det, u, v, t;
+    tri_terms(q0, q1, q2, org, dir, det, u, v, t);
     if (!cull || det > 1e-10f) {
-        const f3 tv = org - xyz(q0);
-        const f3 qv = cross3(tv, e0);
-        const float u = dot3(tv, pv) / det;
-        const float v = dot3(dir, qv) / det;
-        const float t = dot3(e1, qv) / det;
         const float w = (1.0f - u) - v;
         bool ok = (u >= 0.0f) && (v >= 0.0f) && (t >= 0.0f) && (w >= 0.0f) && (t < h.t);
         if (PHOTON) ok = ok && (__float_as_int(q0.w) != -1);
         if (ok) {
             h.tri = tri;
             h.t = t;
-            h.bx = v;
-            h.by = w;
-            h.bz = u;
         }
     }
+}
+
+// The barycentrics tri_test computed when it took the closest hit: the same terms, recomputed.
+PPM_DEV void hit_barycentrics(const DevScene& s, f3 org, f3 dir, Hit& h) {
+    if (h.tri < 0) return;
+    const float4* T = s.tri_isect + 5 * (size_t)h.tri;
+    float det, u, v, t;
+    tri_terms(T[0], T[1], T[2], org, dir, det, u, v, t);
+    h.bx = v;
+    h.by = (1.0f - u) - v;
+    h.bz = u;
 }
 
 // Leaves a lane may hold before its triangle tests run.
@@ -90,15 +100,81 @@ PPM_DEV float mul_half_hi(unsigned int w, float r) {
     return d;
 }
 
-// Slab test of a node record's outward-rounded binary16 box, in box_test's arithmetic.
-PPM_DEV bool node_test(uint4 r, f3 rcp, f3 t0, float ht, float& tmin_out) {
-    const float lx = mul_half_lo(r.x, rcp.x) + t0.x, ly = mul_half_hi(r.x, rcp.y) + t0.y;
-    const float lz = mul_half_lo(r.y, rcp.z) + t0.z;
-    const float hx = mul_half_hi(r.y, rcp.x) + t0.x, hy = mul_half_lo(r.z, rcp.y) + t0.y;
-    const float hz = mul_half_hi(r.z, rcp.z) + t0.z;
-    const float tmin = max1(max1(min1(lx, hx), min1(ly, hy)), min1(lz, hz));
-    const float tmax = min1(min1(max1(lx, hx), max1(ly, hy)), max1(lz, hz));
-    tmin_out = tmin;
+// A ray's slab terms (rcp, t0) in the axis order of its face's node records: the major axis m =
+// face / 2, then the other two in xyz order; sh_a / sh_b rotate a record word so that its low half
+// is the near bound on that axis (16 when the direction is negative there).
+struct WideRay {
+    float rm, tm, ra, ta, rb, tb;
+    unsigned int sh_a, sh_b;
+};
+
+PPM_DEV WideRay wide_ray(int face, f3 rcp, f3 t0) {
+    WideRay w;
+    const int m = face >> 1;
+    w.rm = m == 0 ? rcp.x : m == 1 ? rcp.y : rcp.z;
+    w.tm = m == 0 ? t0.x : m == 1 ? t0.y : t0.z;
+    w.ra = m == 0 ? rcp.y : rcp.x;
+    w.ta = m == 0 ? t0.y : t0.x;
+    w.rb = m == 2 ? rcp.y : rcp.z;
+    w.tb = m == 2 ? t0.y : t0.z;
+    w.sh_a = signbit(w.ra) ? 16u : 0u;
+    w.sh_b = signbit(w.rb) ? 16u : 0u;
+    return w;
+}
+
+// The reference's slab test of an exact fp32 box (eyeraytrace.fs:92-116, photontrace.fs:148-172:
+// per axis rcp * bound + t0, min/max, then tmin <= tmax, tmin <= closest hit, tmax >= 0) with the
+// slab terms in the face's axis order: the same products, sums and per-axis min/max; the axes
+// enter tmin and tmax in another order, which NaN-rejecting min and max do not see (up to the sign
+// of a zero, which no comparison sees).
+PPM_DEV bool box_test_face(const WideRay& w, int face, f3 a, f3 b, float ht) {
+    const int m = face >> 1;
+    const float am = m == 0 ? a.x : m == 1 ? a.y : a.z, bm = m == 0 ? b.x : m == 1 ? b.y : b.z;
+    const float aa = m == 0 ? a.y : a.x, ba = m == 0 ? b.y : b.x;
+    const float ab = m == 2 ? a.y : a.z, bb = m == 2 ? b.y : b.z;
+    const float lm = w.rm * am + w.tm, hm = w.rm * bm + w.tm;
+    const float la = w.ra * aa + w.ta, ha = w.ra * ba + w.ta;
+    const float lb = w.rb * ab + w.tb, hb = w.rb * bb + w.tb;
+    const float tmin = max1(max1(min1(lm, hm), min1(la, ha)), min1(lb, hb));
+    const float tmax = min1(min1(max1(lm, hm), max1(la, ha)), max1(lb, hb));
+    return (tmin <= tmax) && (tmin <= ht) && (tmax >= 0.0f);
+}
+
+// Slab test of a pre-order node record's outward-rounded binary16 box (axes in the face's order,
+// each as lo | hi << 16 or hi | lo << 16), in box_test_face's arithmetic: per-axis min/max whatever
+// the signs, so it passes exactly where the reference's test passes on the same bounds.
+PPM_DEV bool node_test(uint4 r, const WideRay& w, float ht) {
+    const float lm = mul_half_lo(r.x, w.rm) + w.tm, hm = mul_half_hi(r.x, w.rm) + w.tm;
+    const float la = mul_half_lo(r.y, w.ra) + w.ta, ha = mul_half_hi(r.y, w.ra) + w.ta;
+    const float lb = mul_half_lo(r.z, w.rb) + w.tb, hb = mul_half_hi(r.z, w.rb) + w.tb;
+    const float tmin = max1(max1(min1(lm, hm), min1(la, ha)), min1(lb, hb));
+    const float tmax = min1(min1(max1(lm, hm), max1(la, ha)), max1(lb, hb));
+    return (tmin <= tmax) && (tmin <= ht) && (tmax >= 0.0f);
+}
+
+// Slab test of one member slot of a four-wide record.  The near and far bounds are taken by the
+// direction's signs instead of by min/max.  The walk uses it only for rays whose six slab terms are
+// finite with nonzero reciprocals (sign_order_ok): then every slab value is a number (a binary16
+// bound may be infinite, never a product of zero and infinity), the two values of an axis are
+// ordered by the sign of its reciprocal, and this is exactly node_test's min/max.  (The order in
+// which the axes enter tmin and tmax does not matter: NaN-rejecting min and max of a set do not
+// depend on it, up to the sign of a zero, which no comparison here sees.)  Other rays, those
+// parallel to an axis above all, keep the reference's NaN behaviour on the pre-order records:
+// there, taking bounds by sign would leave a NaN slab out where min/max rejects the box, and such
+// a ray would enter almost every box.
+PPM_DEV bool sign_order_ok(const WideRay& w) {
+    const float m = fminf(fminf(fabsf(w.rm), fabsf(w.ra)), fabsf(w.rb));
+    return m != 0.0f && __builtin_isfinite(w.rm) && __builtin_isfinite(w.ra) && __builtin_isfinite(w.rb) &&
+           __builtin_isfinite(w.tm) && __builtin_isfinite(w.ta) && __builtin_isfinite(w.tb);
+}
+
+PPM_DEV bool wide_test(uint4 r, const WideRay& w, float ht) {
+    const unsigned int a = __builtin_rotateright32(r.y, w.sh_a), b = __builtin_rotateright32(r.z, w.sh_b);
+    const float nm = mul_half_lo(r.x, w.rm) + w.tm, fm = mul_half_hi(r.x, w.rm) + w.tm;
+    const float na = mul_half_lo(a, w.ra) + w.ta, fa = mul_half_hi(a, w.ra) + w.ta;
+    const float nb = mul_half_lo(b, w.rb) + w.tb, fb = mul_half_hi(b, w.rb) + w.tb;
+    const float tmin = max1(max1(nm, na), nb);
+    const float tmax = min1(min1(fm, fa), fb);
     return (tmin <= tmax) && (tmin <= ht) && (tmax >= 0.0f);
 }
 
@@ -140,15 +216,19 @@ extern __shared__ int walk_lds[];
 constexpr int kDone = (int)0x80000000;  // no record or leaf left (never ~triangle: T < 2^31 - 1)
 
 template <bool PHOTON>
-PPM_DEV void walk(const DevScene& s, int face, f3 org, f3 dir, f3 rcp, f3 t0, bool cull, Hit& h, int depth) {
+PPM_DEV void walk(const DevScene& s, int face, f3 org, f3 dir, f3 rcp_in, f3 t0_in, bool cull, Hit& hit, int depth) {
+    Closest h = {hit.tri, hit.t};
     const int n = s.num_nodes;
     const uint4* __restrict__ nd = s.nodes + (size_t)face * (size_t)(n + 1);
     const uint4* __restrict__ wr = s.wide + (size_t)face * (size_t)s.wide_stride * 4;
     int* __restrict__ stk = walk_lds + threadIdx.x;
     const int stride = blockDim.x;
+    const WideRay wr4 = wide_ray(face, rcp_in, t0_in);
     int cur = 0;  // >= 0: the record to visit; else ~triangle of a leaf to queue, or kDone
     int sp = 0;   // stack entries
-    int i = -1;   // >= 0: the lane walks the pre-order records from position i
+    // >= 0: the lane walks the pre-order records from position i (from the start for a ray the
+    // record tests do not cover)
+    int i = sign_order_ok(wr4) ? -1 : 0;
     // queued leaves, newest first: q0, then q1 when two are queued
     int q0 = -1, q1 = -1;
     int nq = 0;
@@ -161,8 +241,7 @@ PPM_DEV void walk(const DevScene& s, int face, f3 org, f3 dir, f3 rcp, f3 t0, bo
             if (can) {
                 if (i >= 0) {
                     const uint4 r0 = nd[i], r1 = nd[i + 1];
-                    float tm;
-                    bool hit = node_test(r0, rcp, t0, h.t, tm);
+                    bool hit = node_test(r0, wr4, h.t);
                     int link = (int)r0.w;
                     bool cont = hit || link < 0;
                     if (hit && link < 0) {
@@ -172,7 +251,7 @@ PPM_DEV void walk(const DevScene& s, int face, f3 org, f3 dir, f3 rcp, f3 t0, bo
                     }
                     int next = cont ? i + 1 : link;
                     if (cont && i + 1 < n && nq < kLeafQueue) {
-                        hit = node_test(r1, rcp, t0, h.t, tm);
+                        hit = node_test(r1, wr4, h.t);
                         link = (int)r1.w;
                         cont = hit || link < 0;
                         if (hit && link < 0) {
@@ -187,11 +266,10 @@ PPM_DEV void walk(const DevScene& s, int face, f3 org, f3 dir, f3 rcp, f3 t0, bo
                 } else if (cur >= 0) {
                     const uint4* __restrict__ R = wr + 4 * (size_t)cur;
                     const uint4 a = R[0], b = R[1], c = R[2], d = R[3];
-                    float tm;
-                    const bool ha = node_test(a, rcp, t0, h.t, tm) && a.w != kWideEmpty;
-                    const bool hb = node_test(b, rcp, t0, h.t, tm) && b.w != kWideEmpty;
-                    const bool hc = node_test(c, rcp, t0, h.t, tm) && c.w != kWideEmpty;
-                    const bool hd = node_test(d, rcp, t0, h.t, tm) && d.w != kWideEmpty;
+                    const bool ha = wide_test(a, wr4, h.t) && a.w != kWideEmpty;
+                    const bool hb = wide_test(b, wr4, h.t) && b.w != kWideEmpty;
+                    const bool hc = wide_test(c, wr4, h.t) && c.w != kWideEmpty;
+                    const bool hd = wide_test(d, wr4, h.t) && d.w != kWideEmpty;
                     const int nh = (int)ha + (int)hb + (int)hc + (int)hd;
                     if (sp + nh - 1 > depth) {
                         const uint4 ps = s.wide_pos[(size_t)face * (size_t)s.wide_stride + cur];
@@ -254,13 +332,15 @@ PPM_DEV void walk(const DevScene& s, int face, f3 org, f3 dir, f3 rcp, f3 t0, bo
             y3 = T[3];
             y4 = T[4];
         }
-        float tm;
-        if (nq > 0 && box_test(rcp, t0, xyz(x3), xyz(x4), h.t, tm))
+        if (nq > 0 && box_test_face(wr4, face, xyz(x3), xyz(x4), h.t))
             tri_test<PHOTON>(x0, x1, x2, org, dir, cull, qa, h);
-        if (nq > 1 && box_test(rcp, t0, xyz(y3), xyz(y4), h.t, tm))
+        if (nq > 1 && box_test_face(wr4, face, xyz(y3), xyz(y4), h.t))
             tri_test<PHOTON>(y0, y1, y2, org, dir, cull, qb, h);
         nq = 0;
     }
+    hit.tri = h.tri;
+    hit.t = h.t;
+    hit_barycentrics(s, org, dir, hit);
 }
 
 template <bool PHOTON>

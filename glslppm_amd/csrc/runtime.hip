@@ -9,6 +9,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <array>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -146,26 +147,35 @@ uint16_t half_outward(float x, bool down) {
     return b;
 }
 
-// Per-face pre-order node records, 16 bytes each: the box as binary16 (x = lo.x | lo.y << 16,
-// y = lo.z | hi.x << 16, z = hi.y | hi.z << 16) rounded outward, and w = the miss position of an
-// interior node or ~triangle for a leaf, whose miss position is always the next one.  The hit
-// link of position i is i + 1.  Each face array has N + 1 records: the last is padding that the
-// walk's look-ahead fetch may read but never visits.
+// Per-face pre-order node records, 16 bytes each: the box as binary16 bounds rounded outward and
+// ordered by the face (for a face whose rays go mainly along +axis M, or -M: x = near | far << 16
+// on M, i.e. lo | hi, or hi | lo; y = lo | hi << 16 on the lower other axis; z = the same on the
+// higher one), and w = the miss position of an interior node or ~triangle for a leaf, whose miss
+// position is always the next one.  The hit link of position i is i + 1.  Each face array has N + 1
+// records: the last is padding that the walk's look-ahead fetch may read but never visits.
 //
 // The same trees, four-wide: each face's pre-order is read as a tree (the children of position i
 // are i + 1, then the miss position of each child in turn up to the miss position of i), and
 // every record holds a cut of up to four descendants of one interior node, found by replacing the
 // interior member of largest surface area by its children while they fit, in pre-order.  Record 0
 // holds the cut below the top of the tree.  A record is four slots in the node format above whose
-// w is the child's record (an interior member), ~triangle (a leaf) or kWideEmpty; records are
-// numbered in pre-order.  wide_pos holds each record's four members' pre-order positions (N for
+// w is the member's record (interior), ~triangle (a leaf) or kWideEmpty; records are numbered in
+// pre-order.  wide_pos holds each record's four members' pre-order positions (N for
 // an empty slot), where a walk that runs out of stack continues on the pre-order records.
+// A node record of face f from the six outward-rounded bounds (lo xyz, hi xyz) and w.
+uint4 face_box(const uint16_t* h, int f, uint32_t w) {
+    const int M = f >> 1, A = M == 0 ? 1 : 0, B = M == 2 ? 1 : 2;
+    const uint32_t lo = h[M], hi = h[3 + M];
+    return make_uint4((f & 1) ? hi | lo << 16 : lo | hi << 16, h[A] | (uint32_t)h[3 + A] << 16,
+                      h[B] | (uint32_t)h[3 + B] << 16, w);
+}
+
 int pack_nodes(const ppm_scene* s, std::vector<uint4>& out, std::vector<uint4>& wide, std::vector<uint4>& wide_pos,
                int* wide_stride) {
     const int N = s->num_nodes;
     out.assign((size_t)6 * (N + 1), make_uint4(0, 0, 0, 0));
     std::vector<int> order(N), pos(N), mpos(N), tri(N);
-    std::vector<uint4> box(N);
+    std::vector<std::array<uint16_t, 6>> hb(N);
     std::vector<std::vector<uint4>> fw(6), fp(6);
     int stride = 1;
     for (int f = 0; f < 6; f++) {
@@ -187,11 +197,9 @@ int pack_nodes(const ppm_scene* s, std::vector<uint4>& out, std::vector<uint4>& 
             const float* b = s->node_bounds + (size_t)nk * 6;
             tri[i] = s->node_triangle[nk];
             if (tri[i] >= 0 && mpos[i] != i + 1) return set_err(nullptr, PPM_E_INVAL, "BVH leaf miss link is not the next node");
-            uint16_t h[6];
+            uint16_t* h = hb[i].data();
             for (int k = 0; k < 6; k++) h[k] = half_outward(b[k], k < 3);
-            box[i] = make_uint4(h[0] | (uint32_t)h[1] << 16, h[2] | (uint32_t)h[3] << 16, h[4] | (uint32_t)h[5] << 16, 0);
-            out[(size_t)f * (N + 1) + i] = box[i];
-            out[(size_t)f * (N + 1) + i].w = tri[i] >= 0 ? (uint32_t)~tri[i] : (uint32_t)mpos[i];
+            out[(size_t)f * (N + 1) + i] = face_box(h, f, tri[i] >= 0 ? (uint32_t)~tri[i] : (uint32_t)mpos[i]);
         }
         out[(size_t)f * (N + 1) + N] = make_uint4(0, 0, 0, (uint32_t)(N + 1));
 
@@ -251,10 +259,7 @@ int pack_nodes(const ppm_scene* s, std::vector<uint4>& out, std::vector<uint4>& 
             for (int j = 0; j < 4; j++) {
                 const int m = cuts[r][j];
                 uint4 slot = make_uint4(0, 0, 0, kWideEmpty);
-                if (m >= 0) {
-                    slot = box[m];
-                    slot.w = tri[m] >= 0 ? (uint32_t)~tri[m] : (uint32_t)rec_of[m];
-                }
+                if (m >= 0) slot = face_box(hb[m].data(), f, tri[m] >= 0 ? (uint32_t)~tri[m] : (uint32_t)rec_of[m]);
                 W[r * 4 + j] = slot;
                 p4[j] = m >= 0 ? (uint32_t)m : (uint32_t)N;
             }
