@@ -11,8 +11,9 @@
 //                                  member's box as above, then its record, ~triangle of a leaf, or
 //                                  kWideEmpty), members in pre-order; record 0 is the top
 //           wide_pos[6][R] uint4   the members' pre-order positions (N: empty)
-//           tri_isect[T][5] float4 (v0, brdf), (v1 - v0, material), (v2 - v0, 0), exact leaf box
-//                                  (min.xyz, 0), (max.xyz, 0)
+//           tri_isect[T][4] float4 (v0, brdf), (v1 - v0, leaf min.x), (v2 - v0, leaf min.y),
+//                                  (leaf min.z, leaf max.xyz): one 64-byte record per triangle
+//                                  with the exact box of its leaf
 //           tri_shade[T][6] float4 (v0, n0.x) (v1, n0.y) (v2, n0.z) (n1, n2.x)
 //                                  (n2.y, n2.z, gn.x, gn.y) (gn.z, material, 0, 0)
 //           mats[M][2] float4      (rgb, eta), (brdf, g, 0, 0)

@@ -75,7 +75,7 @@ PPM_DEV void tri_test(float4 q0, float4 q1, float4 q2, f3 org, f3 dir, bool cull
 // The barycentrics tri_test computed when it took the closest hit: the same terms, recomputed.
 PPM_DEV void hit_barycentrics(const DevScene& s, f3 org, f3 dir, Hit& h) {
     if (h.tri < 0) return;
-    const float4* T = s.tri_isect + 5 * (size_t)h.tri;
+    const float4* T = s.tri_isect + 4 * (size_t)h.tri;
     float det, u, v, t;
     tri_terms(T[0], T[1], T[2], org, dir, det, u, v, t);
     h.bx = v;
@@ -314,27 +314,26 @@ PPM_DEV void walk(const DevScene& s, int face, f3 org, f3 dir, f3 rcp_in, f3 t0_
         const int qa = nq == 2 ? q1 : q0, qb = q0;
         // both queued leaves are fetched before the first test: the second leaf's tests need the
         // first test's result, its data does not
-        float4 x0 = make_float4(0.0f, 0.0f, 0.0f, 0.0f), x1 = x0, x2 = x0, x3 = x0, x4 = x0;
-        float4 y0 = x0, y1 = x0, y2 = x0, y3 = x0, y4 = x0;
+        float4 x0 = make_float4(0.0f, 0.0f, 0.0f, 0.0f), x1 = x0, x2 = x0, x3 = x0;
+        float4 y0 = x0, y1 = x0, y2 = x0, y3 = x0;
         if (nq > 0) {
-            const float4* T = s.tri_isect + 5 * qa;
+            const float4* T = s.tri_isect + 4 * (size_t)qa;
             x0 = T[0];
             x1 = T[1];
             x2 = T[2];
             x3 = T[3];
-            x4 = T[4];
         }
         if (nq > 1) {
-            const float4* T = s.tri_isect + 5 * qb;
+            const float4* T = s.tri_isect + 4 * (size_t)qb;
             y0 = T[0];
             y1 = T[1];
             y2 = T[2];
             y3 = T[3];
-            y4 = T[4];
         }
-        if (nq > 0 && box_test_face(wr4, face, xyz(x3), xyz(x4), h.t))
+        // the leaf's exact box: (x1.w, x2.w, x3.x) to (x3.y, x3.z, x3.w)
+        if (nq > 0 && box_test_face(wr4, face, mk3(x1.w, x2.w, x3.x), mk3(x3.y, x3.z, x3.w), h.t))
             tri_test<PHOTON>(x0, x1, x2, org, dir, cull, qa, h);
-        if (nq > 1 && box_test_face(wr4, face, xyz(y3), xyz(y4), h.t))
+        if (nq > 1 && box_test_face(wr4, face, mk3(y1.w, y2.w, y3.x), mk3(y3.y, y3.z, y3.w), h.t))
             tri_test<PHOTON>(y0, y1, y2, org, dir, cull, qb, h);
         nq = 0;
     }
@@ -361,7 +360,6 @@ struct Surface {
     int brdf, mat;
 };
 
-PPM_DEV int tri_material(const DevScene& s, int tri) { return __float_as_int(s.tri_isect[5 * tri + 1].w); }
 
 // Shading fetch after the walk (eyeraytrace.fs:256-287).
 PPM_DEV Surface shade(const DevScene& s, const Hit& h) {
@@ -1139,11 +1137,12 @@ PPM_DEV void photon_trace(const Job& J, int blk) {
     const DevScene& s = J.s;
     const RayQueue& rq = J.rq;
     const int p = rq.perm[t];
-    const float4 o = rq.org[p], d = rq.dir[p];
-    const int4 x = rq.aux[p];
+    const float* po = reinterpret_cast<const float*>(rq.org + p);
+    const float* pd = reinterpret_cast<const float*>(rq.dir + p);
+    const f3 o = mk3(po[0], po[1], po[2]), d = mk3(pd[0], pd[1], pd[2]);
     Walk W;
-    W.org = xyz(o);
-    W.dir = xyz(d);
+    W.org = o;
+    W.dir = d;
     W.rcp = mk3(1.0f / d.x, 1.0f / d.y, 1.0f / d.z);
     W.t0 = mk3(-W.rcp.x * o.x, -W.rcp.y * o.y, -W.rcp.z * o.z);
     W.face = cube_face(W.dir);
@@ -1151,7 +1150,10 @@ PPM_DEV void photon_trace(const Job& J, int blk) {
     W.h.t = 1.0e30f;
     W.h.bx = W.h.by = W.h.bz = 0.0f;
     walk<true>(s, W.face, W.org, W.dir, W.rcp, W.t0, false, W.h, J.walk_stack);
-    photon_finish(s, J.pp, J.ps, J.h, g, x.z, W, mk3(o.w, d.w, __int_as_float(x.x)), x.y);
+    // the ray's flux and bookkeeping are read again rather than held in registers through the walk
+    const float fx = po[3], fy = pd[3];
+    const int4 y = rq.aux[p];
+    photon_finish(s, J.pp, J.ps, J.h, g, y.z, W, mk3(fx, fy, __int_as_float(y.x)), y.y);
 }
 
 // re.fs: tone map of one channel

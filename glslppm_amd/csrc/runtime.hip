@@ -283,7 +283,7 @@ int pack_nodes(const ppm_scene* s, std::vector<uint4>& out, std::vector<uint4>& 
 // intersection record also holds the exact box of the triangle's leaf.
 void pack_triangles(const ppm_scene* s, std::vector<float4>& isect, std::vector<float4>& shade) {
     const int T = s->num_triangles;
-    isect.resize((size_t)T * 5);
+    isect.resize((size_t)T * 4);
     shade.resize((size_t)T * 6);
     std::vector<int> leaf(T, -1);
     for (int k = 0; k < s->num_nodes; k++)
@@ -308,11 +308,10 @@ void pack_triangles(const ppm_scene* s, std::vector<float4>& isect, std::vector<
         const float inv = 1.0f / std::sqrt(c[0] * c[0] + c[1] * c[1] + c[2] * c[2]);
         const float gn[3] = {c[0] * inv, c[1] * inv, c[2] * inv};
         const float* lb = s->node_bounds + (size_t)leaf[t] * 6;
-        isect[(size_t)t * 5 + 0] = make_float4(p[0], p[1], p[2], as_float(brdf));
-        isect[(size_t)t * 5 + 1] = make_float4(e0[0], e0[1], e0[2], as_float(mat));
-        isect[(size_t)t * 5 + 2] = make_float4(e1[0], e1[1], e1[2], 0.0f);
-        isect[(size_t)t * 5 + 3] = make_float4(lb[0], lb[1], lb[2], 0.0f);
-        isect[(size_t)t * 5 + 4] = make_float4(lb[3], lb[4], lb[5], 0.0f);
+        isect[(size_t)t * 4 + 0] = make_float4(p[0], p[1], p[2], as_float(brdf));
+        isect[(size_t)t * 4 + 1] = make_float4(e0[0], e0[1], e0[2], lb[0]);
+        isect[(size_t)t * 4 + 2] = make_float4(e1[0], e1[1], e1[2], lb[1]);
+        isect[(size_t)t * 4 + 3] = make_float4(lb[2], lb[3], lb[4], lb[5]);
         float4* o = &shade[(size_t)t * 6];
         o[0] = make_float4(p[0], p[1], p[2], rn[0][0]);
         o[1] = make_float4(p[3], p[4], p[5], rn[0][1]);
@@ -593,7 +592,7 @@ size_t state_footprint(const ppm_params* p) {
 
 size_t scene_footprint(int T, int N, int M, int NL, int tex_size, int wide_stride) {
     const size_t S = (size_t)tex_size, R = (size_t)wide_stride;
-    const size_t sizes[] = {(size_t)6 * (N + 1) * 16, 6 * R * 64, 6 * R * 16, (size_t)T * 80, (size_t)T * 96,
+    const size_t sizes[] = {(size_t)6 * (N + 1) * 16, 6 * R * 64, 6 * R * 16, (size_t)T * 64, (size_t)T * 96,
                             (size_t)M * 32, (size_t)NL * 4, (size_t)NL * 4, S ? (size_t)T * 32 : 0, (size_t)M * S * S * 4};
     size_t tot = 0;
     for (size_t b : sizes) tot += Arena::round_size(b ? b : 16);

@@ -191,3 +191,22 @@ def test_smallest_scenes():
     m = g.Mesh.from_arrays(np.concatenate([tri_light, tri_floor]), [light, floor], [0, 1])
     m.set_camera((0, 0.2, 1.5), (0, -0.5, 0), 16, 16, 60.0).build()
     _check(m, g.default_params(16, max_path_length=4), 9, "two-triangles")
+
+
+def test_axis_parallel_rays():
+    """Every eye ray exactly (0, 0, -1): the Cornell room moved to x = y = 65536 and a pinhole
+    camera there with a focal distance of 1e-3, so that the focal point rounds onto the camera's
+    x and y.  Such rays have infinite reciprocals on two axes, where the reference's slab test
+    (eyeraytrace.fs:92-116) meets NaN; the walks must keep its outcome box by box, as they do for
+    every other ray."""
+    import oracle
+    a = g.Mesh.generate(g.SCENE_CORNELL).build().arrays()
+    off = np.array([65536.0, 65536.0, 0.0], np.float32)
+    pos = (a["positions"].reshape(-1, 3, 3) + off).astype(np.float32)
+    m = g.Mesh.from_arrays(pos, a["materials"], a["material"], a["normals"].reshape(-1, 3, 3))
+    m.set_camera(tuple(off + np.float32([0.0, 0.0, 1.667])), tuple(off), 32, 32, 45.0).build()
+    p = g.default_params(32, max_path_length=6, focal_length=1e-3, aperture=0.0)
+    for (i, j) in ((0, 0), (31, 0), (16, 16), (5, 27)):
+        _, d, _, _ = oracle.camera_ray(m.scene, p, i, j, 0.3, -0.4, (0.5, 0.5, 0.5, 0.5))
+        assert tuple(d) == (0.0, 0.0, -1.0)
+    _check(m, p, 13, "axis-parallel")
