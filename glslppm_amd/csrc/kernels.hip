@@ -221,7 +221,8 @@ PPM_DEV void walk(const DevScene& s, int face, f3 org, f3 dir, f3 rcp_in, f3 t0_
     const int n = s.num_nodes;
     const uint4* __restrict__ nd = s.nodes + (size_t)face * (size_t)(n + 1);
     const uint4* __restrict__ wr = s.wide + (size_t)face * (size_t)s.wide_stride * 4;
-    int* __restrict__ stk = walk_lds + threadIdx.x;
+    // the lane's stack: entry k at walk_lds[k * blockDim.x + lane]; top points past the last one
+    int* top = walk_lds + threadIdx.x;
     const int stride = blockDim.x;
     const WideRay wr4 = wide_ray(face, rcp_in, t0_in);
     int cur = 0;  // >= 0: the record to visit; else ~triangle of a leaf to queue, or kDone
@@ -266,46 +267,74 @@ PPM_DEV void walk(const DevScene& s, int face, f3 org, f3 dir, f3 rcp_in, f3 t0_
                 } else if (cur >= 0) {
                     const uint4* __restrict__ R = wr + 4 * (size_t)cur;
                     const uint4 a = R[0], b = R[1], c = R[2], d = R[3];
-                    const bool ha = wide_test(a, wr4, h.t) && a.w != kWideEmpty;
-                    const bool hb = wide_test(b, wr4, h.t) && b.w != kWideEmpty;
-                    const bool hc = wide_test(c, wr4, h.t) && c.w != kWideEmpty;
-                    const bool hd = wide_test(d, wr4, h.t) && d.w != kWideEmpty;
+                    // the members' links are held in registers: left to itself the compiler
+                    // fetches one again after the tests, a dependent load on the walk's chain
+                    unsigned int wa = a.w, wb = b.w, wc = c.w, wd = d.w;
+                    asm volatile("" : "+v"(wa), "+v"(wb), "+v"(wc), "+v"(wd));
+                    // an empty slot's infinite near bound fails it (see the record packing)
+                    const bool ha = wide_test(a, wr4, h.t), hb = wide_test(b, wr4, h.t);
+                    const bool hc = wide_test(c, wr4, h.t), hd = wide_test(d, wr4, h.t);
                     const int nh = (int)ha + (int)hb + (int)hc + (int)hd;
                     if (sp + nh - 1 > depth) {
                         const uint4 ps = s.wide_pos[(size_t)face * (size_t)s.wide_stride + cur];
-                        i = (int)(ha ? ps.x : hb ? ps.y : hc ? ps.z : ps.w);
-                        sp = 0;
+                        i = (int)(ha ? ps.x : hb ? ps.y : hc ? ps.z : ps.w);  // the stack is not used again
                     } else {
                         // members hit after the first are pushed last to first
                         int first = kDone;
-                        if (hd) first = (int)d.w;
+                        if (hd) first = (int)wd;
                         if (hc) {
-                            if (first != kDone) stk[stride * sp++] = first;
-                            first = (int)c.w;
+                            if (first != kDone) {
+                                *top = first;
+                                top += stride;
+                                sp++;
+                            }
+                            first = (int)wc;
                         }
                         if (hb) {
-                            if (first != kDone) stk[stride * sp++] = first;
-                            first = (int)b.w;
+                            if (first != kDone) {
+                                *top = first;
+                                top += stride;
+                                sp++;
+                            }
+                            first = (int)wb;
                         }
                         if (ha) {
-                            if (first != kDone) stk[stride * sp++] = first;
-                            first = (int)a.w;
+                            if (first != kDone) {
+                                *top = first;
+                                top += stride;
+                                sp++;
+                            }
+                            first = (int)wa;
                         }
                         cur = first;
-                        if (cur == kDone && sp > 0) cur = stk[stride * --sp];
+                        if (cur == kDone && sp > 0) {
+                            sp--;
+                            top -= stride;
+                            cur = *top;
+                        }
                         if (cur < 0 && cur != kDone) {
                             // a leaf next: queued in the same step (the queue had room)
                             q1 = q0;
                             q0 = ~cur;
                             nq++;
-                            cur = sp > 0 ? stk[stride * --sp] : kDone;
+                            cur = kDone;
+                            if (sp > 0) {
+                                sp--;
+                                top -= stride;
+                                cur = *top;
+                            }
                         }
                     }
                 } else {
                     q1 = q0;
                     q0 = ~cur;
                     nq++;
-                    cur = sp > 0 ? stk[stride * --sp] : kDone;
+                    cur = kDone;
+                    if (sp > 0) {
+                        sp--;
+                        top -= stride;
+                        cur = *top;
+                    }
                 }
             }
         }

@@ -160,7 +160,9 @@ uint16_t half_outward(float x, bool down) {
 // interior member of largest surface area by its children while they fit, in pre-order.  Record 0
 // holds the cut below the top of the tree.  A record is four slots in the node format above whose
 // w is the member's record (interior), ~triangle (a leaf) or kWideEmpty; records are numbered in
-// pre-order.  wide_pos holds each record's four members' pre-order positions (N for
+// pre-order.  An empty slot's near bound on the major axis is infinite (+inf for a + face, -inf
+// for a - face), so that its near slab value is +inf for every ray the record tests serve (finite
+// nonzero reciprocals of the face's sign, finite t0) and the slot never passes.  wide_pos holds each record's four members' pre-order positions (N for
 // an empty slot), where a walk that runs out of stack continues on the pre-order records.
 // A node record of face f from the six outward-rounded bounds (lo xyz, hi xyz) and w.
 uint4 face_box(const uint16_t* h, int f, uint32_t w) {
@@ -169,6 +171,9 @@ uint4 face_box(const uint16_t* h, int f, uint32_t w) {
     return make_uint4((f & 1) ? hi | lo << 16 : lo | hi << 16, h[A] | (uint32_t)h[3 + A] << 16,
                       h[B] | (uint32_t)h[3 + B] << 16, w);
 }
+
+// An unused member slot of a four-wide record of face f.
+uint4 empty_slot(int f) { return make_uint4((f & 1) ? 0xFC00FC00u : 0x7C007C00u, 0, 0, kWideEmpty); }
 
 int pack_nodes(const ppm_scene* s, std::vector<uint4>& out, std::vector<uint4>& wide, std::vector<uint4>& wide_pos,
                int* wide_stride) {
@@ -258,7 +263,7 @@ int pack_nodes(const ppm_scene* s, std::vector<uint4>& out, std::vector<uint4>& 
             uint32_t p4[4];
             for (int j = 0; j < 4; j++) {
                 const int m = cuts[r][j];
-                uint4 slot = make_uint4(0, 0, 0, kWideEmpty);
+                uint4 slot = empty_slot(f);
                 if (m >= 0) slot = face_box(hb[m].data(), f, tri[m] >= 0 ? (uint32_t)~tri[m] : (uint32_t)rec_of[m]);
                 W[r * 4 + j] = slot;
                 p4[j] = m >= 0 ? (uint32_t)m : (uint32_t)N;
@@ -267,9 +272,10 @@ int pack_nodes(const ppm_scene* s, std::vector<uint4>& out, std::vector<uint4>& 
         }
         stride = std::max(stride, (int)cuts.size());
     }
-    wide.assign((size_t)6 * stride * 4, make_uint4(0, 0, 0, kWideEmpty));
+    wide.resize((size_t)6 * stride * 4);
     wide_pos.assign((size_t)6 * stride, make_uint4(N, N, N, N));
     for (int f = 0; f < 6; f++) {
+        std::fill(wide.begin() + (size_t)f * stride * 4, wide.begin() + (size_t)(f + 1) * stride * 4, empty_slot(f));
         std::copy(fw[f].begin(), fw[f].end(), wide.begin() + (size_t)f * stride * 4);
         std::copy(fp[f].begin(), fp[f].end(), wide_pos.begin() + (size_t)f * stride);
     }
