@@ -1293,9 +1293,9 @@ PPM_DEV void gather(const Job& J, int blk) {
         const BlockRef m = map[blockIdx.x];                                                         \
         body(jobs[m.job], m.block);                                                                 \
     }
-// A floor on waves per SIMD caps a kernel's VGPRs: the eye kernel's bounce loop fits 96 at 5 waves
-// (one wave more than it would get), the photon walk 72 at 7 (one more), which shortens their
-// dependent-load stalls; 8 waves would spill the walk.
+// A floor on waves per SIMD caps a kernel's VGPRs: the eye kernel's bounce loop at 6 waves (80,
+// with a small spill; it would take 105 at 4), the photon walk at 7 (72; 8 waves would spill it),
+// which shortens their dependent-load stalls (measured: eye pass 1.04 -> 0.99 ms on a 1024^2 image).
 #define PPM_JOB_KERNEL_WAVES(name, body, threads, waves)                                          \
     __global__ __launch_bounds__(threads) __attribute__((amdgpu_waves_per_eu(waves))) void name(Job J) { \
         body(J, (int)blockIdx.x);                                                                   \
@@ -1312,7 +1312,7 @@ PPM_DEV void gather(const Job& J, int blk) {
     }
 
 PPM_JOB_KERNEL(k_clear_slots, clear_slots, kBlock)
-PPM_JOB_KERNEL_WAVES(k_eye, eye, kBlock, 5)
+PPM_JOB_KERNEL_WAVES(k_eye, eye, kBlock, 6)
 PPM_JOB_KERNEL1(k_grid, grid, 1024)
 PPM_JOB_KERNEL(k_photon_gen, photon_gen_blk, kBlock)
 PPM_JOB_KERNEL1(k_bucket_scan, bucket_scan, 1024)
