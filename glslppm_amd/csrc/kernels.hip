@@ -442,7 +442,9 @@ PPM_DEV f3 glossy_reflect(f3 d, f3 n, float g, float4& rndv) {
     const float temp1 = 6.283184f * rnd1;
     const float temp2_ = ppm_powf(rnd2, g);
     const float temp2 = sqrtf(1.0f - temp2_ * temp2_);
-    const f3 v = mk3(ppm_sinf(temp1) * temp2, temp2_, ppm_cosf(temp1) * temp2);
+    float st, ct;
+    ppm_sincosf(temp1, st, ct);
+    const f3 v = mk3(st * temp2, temp2_, ct * temp2);
     f3 result = normalize3(onb(v, r));
     if (dot3(result, n) < 0.0f) result = reflect3(result, n);
     return result;
@@ -615,7 +617,9 @@ PPM_DEV void eye(const Job& J, int blk) {
         const f3 fp = C + dir * ep.focal_length;
         const float radius = sqrtf(gpu_rnd(rndv)) * ep.aperture;
         const float theta = 6.283184f * gpu_rnd(rndv);
-        const f3 lens = C + radius * (U * ppm_cosf(theta) + V * ppm_sinf(theta));
+        float st, ct;
+        ppm_sincosf(theta, st, ct);
+        const f3 lens = C + radius * (U * ct + V * st);
         dir = normalize3(fp - lens);
         f3 org = lens;
         f3 col = mk3(filter, filter, filter);
@@ -829,7 +833,9 @@ PPM_DEV void light_sample(const DevScene& s, float4& rndv, f3& flux, f3& dir, f3
     float ry = gpu_rnd(rndv);
     rx = 6.283184f * rx;
     ry = sqrtf(ry);
-    dir = onb(mk3(ppm_sinf(rx) * ry, sqrtf(1.0f - ry * ry), ppm_cosf(rx) * ry), nrm);
+    float sx, cx;
+    ppm_sincosf(rx, sx, cx);
+    dir = onb(mk3(sx * ry, sqrtf(1.0f - ry * ry), cx * ry), nrm);
 }
 
 // GenerateIBLSample (photontrace.fs:283-341), used when the scene has no light triangle
@@ -837,11 +843,13 @@ PPM_DEV void ibl_sample(const DevScene& s, float4& rndv, f3& flux, f3& dir, f3& 
     const float area = 12.566368f * s.bsphere_r * s.bsphere_r;
     const float temp2 = 6.283184f * gpu_rnd(rndv);
     const float aaa = sqrtf(1.0f - gpu_rnd(rndv));
-    const float sin_a = ppm_sinf(aaa);
-    const float sin_t1 = (sin_a + sin_a) * ppm_cosf(aaa);
-    dir.x = sin_t1 * ppm_cosf(temp2);
+    float sin_a, cos_a, s2, c2;
+    ppm_sincosf(aaa, sin_a, cos_a);
+    ppm_sincosf(temp2, s2, c2);
+    const float sin_t1 = (sin_a + sin_a) * cos_a;
+    dir.x = sin_t1 * c2;
     dir.y = 1.0f - (sin_a + sin_a) * sin_a;
-    dir.z = sin_t1 * ppm_sinf(temp2);
+    dir.z = sin_t1 * s2;
     if (gpu_rnd(rndv) > 0.5f) {
         dir = mk3(1.0f, 1.0f, 1.0f);
         dir.y = dir.y + dir.y;
@@ -853,7 +861,9 @@ PPM_DEV void ibl_sample(const DevScene& s, float4& rndv, f3& flux, f3& dir, f3& 
     }
     const float radius = sqrtf(gpu_rnd(rndv));
     const float theta = 6.283184f * gpu_rnd(rndv);
-    const f3 temp = mk3(radius * ppm_cosf(theta), 0.0f, radius * ppm_sinf(theta));
+    float st, ct;
+    ppm_sincosf(theta, st, ct);
+    const f3 temp = mk3(radius * ct, 0.0f, radius * st);
     org = (onb(temp * s.bsphere_r, dir) + dir * s.bsphere_r) + mk3(s.bsphere_x, s.bsphere_y, s.bsphere_z);
     dir = -dir;
 }
@@ -924,7 +934,9 @@ PPM_DEV bool photon_gen(const DevScene& s, const PhotonParams& pp, const PhotonS
         if (brdf == 0) {
             const float r0 = 6.283184f * gpu_rnd(rndv);
             const float r1 = sqrtf(gpu_rnd(rndv));
-            const f3 v = mk3(ppm_sinf(r0) * r1, sqrtf(1.0f - r1 * r1), ppm_cosf(r0) * r1);
+            float s0, c0;
+            ppm_sincosf(r0, s0, c0);
+            const f3 v = mk3(s0 * r1, sqrtf(1.0f - r1 * r1), c0 * r1);
             org = ipos + eps * gnrm;
             dir = onb(v, inrm);
             if (dot3(dir, gnrm) < 0.0f) dir = -dir;
@@ -971,7 +983,9 @@ PPM_DEV bool photon_gen(const DevScene& s, const PhotonParams& pp, const PhotonS
             } else {
                 const float r0 = 6.283184f * gpu_rnd(rndv);
                 const float r1 = sqrtf(gpu_rnd(rndv));
-                const f3 v = mk3(ppm_sinf(r0) * r1, sqrtf(1.0f - r1 * r1), ppm_cosf(r0) * r1);
+                float s0, c0;
+                ppm_sincosf(r0, s0, c0);
+                const f3 v = mk3(s0 * r1, sqrtf(1.0f - r1 * r1), c0 * r1);
                 org = ipos + eps * gnrm;
                 dir = onb(v, inrm);
                 if (dot3(dir, gnrm) < 0.0f) dir = -dir;
