@@ -116,6 +116,8 @@ def run_batch_workload(args):
         r.close()
 
     b = g.Batch(passes_per_graph=args.passes_per_graph)
+    if args.tuning:
+        b.set_tuning(**json.loads(args.tuning))
     ids = [b.add(m, p, n) for (m, p, n) in jobs]
     most = max(n for (_, _, n) in jobs)
     b.run(0)  # builds the tables and the graph
@@ -326,7 +328,7 @@ def main():
     ap.add_argument("--autotune", action="store_true",
                     help="plain workloads: pick launch shapes with ppm_autotune during the warm-up")
     ap.add_argument("--tuning", default=None,
-                    help='plain workloads: launch shapes to set first, as JSON, e.g. \'{"walk_stack": 8}\'')
+                    help='launch shapes to set first (plain and batch workloads), as JSON, e.g. \'{"walk_stack": 8}\'')
     ap.add_argument("--cpu-passes", type=int, default=50, help="passes of the CPU baseline sample (0: skip)")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="first run a plain device copy of MIB MiB (torch), the known-size reference for "

@@ -78,7 +78,7 @@ class BatchConfig(ctypes.Structure):
 class Tuning(ctypes.Structure):
     _fields_ = [("trace_block", ctypes.c_int32), ("tile_wave_width", ctypes.c_int32),
                 ("passes_per_graph", ctypes.c_uint32), ("walk_stack", ctypes.c_int32),
-                ("ms_per_pass", ctypes.c_float)]
+                ("eye_overlap", ctypes.c_int32), ("ms_per_pass", ctypes.c_float)]
 
     def update(self, **kw):
         for k, v in kw.items():
@@ -415,7 +415,7 @@ class Renderer:
 
     def set_tuning(self, **fields):
         """Changes the named launch-shape fields (trace_block, tile_wave_width, passes_per_graph,
-        walk_stack); the others keep their values."""
+        walk_stack, eye_overlap); the others keep their values."""
         t = Tuning()
         self._check(library().ppm_get_tuning(self._h, ctypes.byref(t)))
         self._check(library().ppm_set_tuning(self._h, ctypes.byref(t.update(**fields))))
@@ -568,7 +568,7 @@ class Batch:
 
     def set_tuning(self, **fields):
         """Changes the named launch-shape fields (trace_block, tile_wave_width, passes_per_graph,
-        walk_stack); the others keep their values."""
+        walk_stack, eye_overlap); the others keep their values."""
         t = Tuning()
         self._check(library().ppm_batch_get_tuning(self._h, ctypes.byref(t)))
         self._check(library().ppm_batch_set_tuning(self._h, ctypes.byref(t.update(**fields))))

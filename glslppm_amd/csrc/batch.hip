@@ -72,14 +72,22 @@ void retire_graph(ppm_batch* b) {
     b->graph = nullptr;
 }
 
-// One batched pass over the current tables: m_display's stage order for every job.
+// One batched pass over the current tables: m_display's stage order for every job.  The deposits
+// of the jobs in an eye pass wait for k_hash; with eye_overlap the eye stage runs on the side
+// stream beside the photon stage, as in the plain loop.
 void queue_batch_pass(ppm_batch* b, hipStream_t st) {
     const int n = b->n_jobs;
     const Job* J = b->d_jobs;
     hipLaunchKernelGGL(k_pass_begin_batch, dim3((n + 63) / 64), dim3(64), 0, st, J, n);
-    hipLaunchKernelGGL(k_eye_batch, dim3(b->n_blocks[kMapTile]), dim3(256), walk_lds_bytes(b->walk_stack, 256), st, J,
+    hipStream_t es = b->eye_overlap ? b->side : st;
+    if (b->eye_overlap) {
+        (void)hipEventRecord(b->ev_fork, st);
+        (void)hipStreamWaitEvent(es, b->ev_fork, 0);
+    }
+    hipLaunchKernelGGL(k_eye_batch, dim3(b->n_blocks[kMapTile]), dim3(256), walk_lds_bytes(b->walk_stack, 256), es, J,
                        (const BlockRef*)b->d_map[kMapTile]);
-    hipLaunchKernelGGL(k_grid_batch, dim3(n), dim3(1024), 0, st, J);
+    hipLaunchKernelGGL(k_grid_batch, dim3(n), dim3(1024), 0, es, J);
+    if (b->eye_overlap) (void)hipEventRecord(b->ev_join, es);
     hipLaunchKernelGGL(k_clear_slots_batch, dim3(b->n_blocks[kMapClear]), dim3(256), 0, st, J,
                        (const BlockRef*)b->d_map[kMapClear]);
     hipLaunchKernelGGL(k_photon_gen_batch, dim3(b->n_blocks[kMapPhoton]), dim3(256), 0, st, J,
@@ -90,6 +98,9 @@ void queue_batch_pass(ppm_batch* b, hipStream_t st) {
     hipLaunchKernelGGL(k_photon_trace_batch, dim3(b->n_blocks[kMapTrace]), dim3(b->trace_block),
                        walk_lds_bytes(b->walk_stack, b->trace_block), st, J,
                        (const BlockRef*)b->d_map[kMapTrace]);
+    if (b->eye_overlap) (void)hipStreamWaitEvent(st, b->ev_join, 0);
+    hipLaunchKernelGGL(k_hash_batch, dim3(b->n_blocks[kMapPhoton]), dim3(256), 0, st, J,
+                       (const BlockRef*)b->d_map[kMapPhoton]);
     hipLaunchKernelGGL(k_gather_batch, dim3(b->n_blocks[kMapTile]), dim3(256), 0, st, J, (const BlockRef*)b->d_map[kMapTile]);
 }
 
@@ -249,6 +260,9 @@ int ppm_batch_create(const ppm_batch_config* cfg, ppm_batch** out) {
     hipError_t e = hipSetDevice(b->device);
     if (e == hipSuccess) e = hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking);
     if (e == hipSuccess) e = hipStreamCreateWithFlags(&b->upload, hipStreamNonBlocking);
+    if (e == hipSuccess) e = hipStreamCreateWithFlags(&b->side, hipStreamNonBlocking);
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&b->ev_fork, hipEventDisableTiming);
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&b->ev_join, hipEventDisableTiming);
     if (e == hipSuccess) e = hipEventCreate(&b->ev0);
     if (e == hipSuccess) e = hipEventCreate(&b->ev1);
     if (e != hipSuccess) {
@@ -391,6 +405,7 @@ int ppm_batch_get_tuning(ppm_batch* b, ppm_tuning* t) {
     t->tile_wave_width = 1 << b->wave_shift;
     t->passes_per_graph = b->passes_per_graph;
     t->walk_stack = b->walk_stack;
+    t->eye_overlap = b->eye_overlap ? 1 : 0;
     t->ms_per_pass = 0.0f;
     return PPM_OK;
 }
@@ -403,6 +418,7 @@ int ppm_batch_set_tuning(ppm_batch* b, const ppm_tuning* t) {
     b->wave_shift = sh;
     b->passes_per_graph = t->passes_per_graph;
     b->walk_stack = t->walk_stack;
+    b->eye_overlap = t->eye_overlap != 0;
     for (ppm_ctx* c : b->slots)
         if (c) ppm_internal::set_launch_shape(c, b->trace_block, b->wave_shift, b->walk_stack);
     b->dirty = true;
@@ -456,9 +472,13 @@ void ppm_batch_destroy(ppm_batch* b) {
             if (b->d_map[k]) b->arena->release(b->d_map[k], nullptr);
         delete b->arena;
     }
+    if (b->side) (void)hipStreamSynchronize(b->side);
     if (b->ev0) (void)hipEventDestroy(b->ev0);
     if (b->ev1) (void)hipEventDestroy(b->ev1);
+    if (b->ev_fork) (void)hipEventDestroy(b->ev_fork);
+    if (b->ev_join) (void)hipEventDestroy(b->ev_join);
     if (b->stream) (void)hipStreamDestroy(b->stream);
+    if (b->side) (void)hipStreamDestroy(b->side);
     if (b->upload) (void)hipStreamDestroy(b->upload);
     delete b;
 }

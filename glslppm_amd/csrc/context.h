@@ -81,6 +81,10 @@ struct ppm_ctx {
     int device = 0;
     hipStream_t stream = nullptr;   // the stream passes run on
     bool own_stream = true;         // false when the stream belongs to a batch
+    // an eye pass's eye stage runs on `side` while the photons fly on `stream` (joined before the
+    // deposits): only contexts that own their stream have one
+    hipStream_t side = nullptr;
+    hipEvent_t ev_fork = nullptr, ev_join = nullptr;
     ppm_internal::Arena* arena = nullptr;  // null: hipMalloc / hipFree
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     bool timed = false;
@@ -101,6 +105,7 @@ struct ppm_ctx {
     uint64_t frame = 0;             // passes queued by ppm_pass (the plain loop's eye cadence)
     int trace_block = 256;          // launch shape (ppm_set_tuning)
     bool use_graph = false;         // replay passes as hipGraphs (PPM_FLAG_GRAPH)
+    bool eye_overlap = false;       // eye stage on `side` in eye passes (ppm_set_tuning)
     // hipGraphs of one photon-only pass and one eye pass (PPM_FLAG_GRAPH)
     hipGraphExec_t graph_pass[2] = {nullptr, nullptr};
     std::string err;
@@ -110,12 +115,15 @@ struct ppm_ctx {
 struct ppm_batch {
     int device = 0;
     hipStream_t stream = nullptr;   // passes
+    hipStream_t side = nullptr;     // the eye stage of every pass, beside the photon stage
+    hipEvent_t ev_fork = nullptr, ev_join = nullptr;
     hipStream_t upload = nullptr;   // scene uploads and job initialisation
     ppm_internal::Arena* arena = nullptr;
     uint32_t passes_per_graph = 1;  // 0: plain launches
     int trace_block = 256;          // launch shape shared by every job (ppm_batch_set_tuning)
     int wave_shift = 3;
     int walk_stack = kWalkStack;
+    bool eye_overlap = true;        // eye stage on `side` every pass
     std::vector<ppm_ctx*> slots;    // job id -> context (null once removed)
     // device tables of the current job set
     bool dirty = true;
@@ -202,8 +210,8 @@ int autotune(ppm_tuning start, bool batch, Trial trial, ppm_tuning* best) {
     const uint32_t graphs_batch[] = {0, 1, 4};
     const uint32_t graphs_plain[] = {0, 1};
     const int stacks[] = {8, 16, 32};
-    for (int knob = 0; knob < 4; knob++) {
-        const int n = knob == 0 ? 4 : knob == 1 ? 5 : knob == 2 ? (batch ? 3 : 2) : 3;
+    for (int knob = 0; knob < 5; knob++) {
+        const int n = knob == 0 ? 4 : knob == 1 ? 5 : knob == 2 ? (batch ? 3 : 2) : knob == 3 ? 3 : 2;
         const ppm_tuning base = cur;  // the value this knob starts from is already timed
         for (int k = 0; k < n; k++) {
             ppm_tuning t = cur;
@@ -211,10 +219,11 @@ int autotune(ppm_tuning start, bool batch, Trial trial, ppm_tuning* best) {
             if (knob == 1) t.tile_wave_width = widths[k];
             if (knob == 2) t.passes_per_graph = batch ? graphs_batch[k] : graphs_plain[k];
             if (knob == 3) t.walk_stack = stacks[k];
+            if (knob == 4) t.eye_overlap = k;
             if ((knob == 0 && t.trace_block == base.trace_block) ||
                 (knob == 1 && t.tile_wave_width == base.tile_wave_width) ||
                 (knob == 2 && t.passes_per_graph == base.passes_per_graph) ||
-                (knob == 3 && t.walk_stack == base.walk_stack))
+                (knob == 3 && t.walk_stack == base.walk_stack) || (knob == 4 && t.eye_overlap == base.eye_overlap))
                 continue;
             if ((rc = trial(t, &ms))) return rc;
             if (ms < cur.ms_per_pass) {

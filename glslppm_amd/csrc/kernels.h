@@ -102,6 +102,9 @@ struct HitState {
     float* partial;  // [blocks][6] min xyz, max xyz of the eye kernel
 };
 
+// PhotonState::slot of a photon whose deposit waits for k_hash (eye passes).
+constexpr int kSlotPending = -2;
+
 struct PhotonState {
     float4* pos;
     float4* flux;
@@ -214,6 +217,7 @@ __global__ void k_photon_gen(Job J);
 __global__ void k_bucket_scan(Job J);
 __global__ void k_bucket_scatter(Job J);
 __global__ void k_photon_trace(Job J);
+__global__ void k_hash(Job J);
 __global__ void k_gather(Job J);
 __global__ void k_pass_begin_batch(const Job* jobs, int n_jobs);
 __global__ void k_clear_slots_batch(const Job* jobs, const BlockRef* map);
@@ -223,6 +227,7 @@ __global__ void k_photon_gen_batch(const Job* jobs, const BlockRef* map);
 __global__ void k_bucket_scan_batch(const Job* jobs);
 __global__ void k_bucket_scatter_batch(const Job* jobs, const BlockRef* map);
 __global__ void k_photon_trace_batch(const Job* jobs, const BlockRef* map);
+__global__ void k_hash_batch(const Job* jobs, const BlockRef* map);
 __global__ void k_gather_batch(const Job* jobs, const BlockRef* map);
 __global__ void k_batch_status(const Job* jobs, int n_jobs, unsigned long long* out);
 __global__ void k_resolve(int count, HitState hs, const DevGlobals* g, float4* linear, float4* mapped);

@@ -1025,6 +1025,19 @@ struct Walk {
 };
 
 // Deposit (photontrace.fs:690-710), hash.fs and the scatter/correction updates of slot p.
+// The slot of a photon that deposits at hpos (hash.fs), and the scatter and correction updates of
+// that slot; -1 for a slot past the last row, which lands outside the hash viewport and is clipped.
+PPM_DEV int deposit(const PhotonParams& pp, const HashState& h, const DevGlobals* g, int p, f3 hpos) {
+    const f3 cell = mk3(floorf((hpos.x - g->bbmin[0]) * g->grid_scale), floorf((hpos.y - g->bbmin[1]) * g->grid_scale),
+                        floorf((hpos.z - g->bbmin[2]) * g->grid_scale));
+    const float sf = cell_hash(cell, g->grid_scale, g->hash_scale, h.num_f);
+    if (!(sf < h.num_f)) return -1;
+    const int slot = (int)sf;
+    atomicAdd(h.slots + 2 * slot, 1u);
+    if (p < pp.rep_limit) atomicMin(h.slots + 2 * slot + 1, (unsigned int)p);
+    return slot;
+}
+
 PPM_DEV void photon_finish(const DevScene& s, const PhotonParams& pp, const PhotonState& ps, const HashState& h,
                            const DevGlobals* g, int p, const Walk& W, f3 flux, int out_bits) {
     const f3 dir = W.dir;
@@ -1051,21 +1064,20 @@ PPM_DEV void photon_finish(const DevScene& s, const PhotonParams& pp, const Phot
     ps.flux[p] = make_float4(flux.x, flux.y, flux.z, hnrm.y);
     ps.dir[p] = make_float4(dir.x, dir.y, dir.z, hnrm.z);
     ps.state[p] = make_int2(W.h.tri, out_bits);
+    // hash.fs: deposits with flux.x >= 0 (including -0) go to the grid.  In an eye pass the grid
+    // is still being set up while the photons fly, so the deposit waits for k_hash (kSlotPending).
     int slot = -1;
-    // hash.fs: deposits with flux.x >= 0 (including -0) go to the grid
-    if (!(flux.x < 0.0f)) {
-        const f3 cell = mk3(floorf((hpos.x - g->bbmin[0]) * g->grid_scale),
-                            floorf((hpos.y - g->bbmin[1]) * g->grid_scale),
-                            floorf((hpos.z - g->bbmin[2]) * g->grid_scale));
-        const float sf = cell_hash(cell, g->grid_scale, g->hash_scale, h.num_f);
-        // a slot past the last row lands outside the hash viewport and is clipped
-        if (sf < h.num_f) {
-            slot = (int)sf;
-            atomicAdd(h.slots + 2 * slot, 1u);
-            if (p < pp.rep_limit) atomicMin(h.slots + 2 * slot + 1, (unsigned int)p);
-        }
-    }
+    if (!(flux.x < 0.0f)) slot = g->eye_now ? kSlotPending : deposit(pp, h, g, p, hpos);
     ps.slot[p] = slot;
+}
+
+// The deposits an eye pass held back, once k_grid has set the grid up.
+PPM_DEV void hash_deferred(const Job& J, int blk) {
+    const DevGlobals* g = J.g;
+    if (!g->active || !g->eye_now) return;
+    const int p = blk * kBlock + threadIdx.x;
+    if (p >= J.pp.count || J.ps.slot[p] != kSlotPending) return;
+    J.ps.slot[p] = deposit(J.pp, J.h, g, p, xyz(J.ps.pos[p]));
 }
 
 PPM_DEV unsigned long long lanes_below(int lane) { return lane == 0 ? 0ull : (~0ull >> (64 - lane)); }
@@ -1332,6 +1344,7 @@ PPM_JOB_KERNEL(k_photon_gen, photon_gen_blk, kBlock)
 PPM_JOB_KERNEL1(k_bucket_scan, bucket_scan, 1024)
 PPM_JOB_KERNEL(k_bucket_scatter, bucket_scatter, kBlock)
 PPM_JOB_KERNEL_WAVES(k_photon_trace, photon_trace, kTraceBlockMax, 7)
+PPM_JOB_KERNEL(k_hash, hash_deferred, kBlock)
 PPM_JOB_KERNEL(k_gather, gather, kBlock)
 
 __global__ void k_pass_begin(Job J) {

@@ -207,7 +207,8 @@ int ppm_profile_read(ppm_ctx* ctx, double* ms, uint64_t* launches);
 /* Launch shapes.  No counterpart in the reference (GL rasterises one fragment per texel).  Every
  * shape gives bit-identical results; only the speed differs.  ppm_autotune runs real passes of
  * the context (they count as passes) under each candidate value of one knob at a time -- photon
- * walk block size, pixel tile shape, graph replay, walk stack depth -- and keeps the fastest;
+ * walk block size, pixel tile shape, graph replay, walk stack depth, eye-stage overlap -- and keeps
+ * the fastest;
  * call it before the passes whose speed matters. */
 typedef struct ppm_tuning {
     int32_t trace_block;       /* threads per block of the photon walk: 64, 128, ..., 512 */
@@ -216,6 +217,9 @@ typedef struct ppm_tuning {
                                   batch: passes per hipGraph, 0 for plain launches */
     int32_t walk_stack;        /* LDS stack entries per lane of the BVH walks, 0..32; deeper
                                   walks go on over the pre-order node records */
+    int32_t eye_overlap;       /* 1: an eye pass's eye stage runs on a second stream beside the
+                                  photon stage (default in a batch, and in the plain loop for
+                                  images of 2^18 pixels or more), 0: one stream */
     float ms_per_pass;         /* ppm_autotune / ppm_batch_autotune: time of the chosen shape */
 } ppm_tuning;
 

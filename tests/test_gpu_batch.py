@@ -67,10 +67,13 @@ def check_equal_alone(view, mesh, params, passes, label):
     assert view.counters()["frame_count"] == passes
 
 
-@pytest.mark.parametrize("passes_per_graph", [1, 4, 0])
-def test_batch_jobs_equal_alone(passes_per_graph):
-    print("passes_per_graph=%d" % passes_per_graph, flush=True)
+@pytest.mark.parametrize("passes_per_graph,eye_overlap", [(1, 0), (4, 0), (0, 0), (4, 1), (0, 1)])
+def test_batch_jobs_equal_alone(passes_per_graph, eye_overlap):
+    """eye_overlap 1: the eye stage on a second stream beside the photon stage, in graphs and in
+    plain launches."""
+    print("passes_per_graph=%d eye_overlap=%d" % (passes_per_graph, eye_overlap), flush=True)
     b = g.Batch(passes_per_graph=passes_per_graph)
+    b.set_tuning(eye_overlap=eye_overlap)
     specs = job_specs()
     ids = [b.add(m, p, n) for (_, m, p, n) in specs]
     b.run(5)
@@ -129,7 +132,8 @@ def test_plain_loop_graph_replay_equals_launches():
                                    dict(tile_wave_width=4), dict(tile_wave_width=64),
                                    dict(trace_block=128, tile_wave_width=16, passes_per_graph=1),
                                    dict(walk_stack=0), dict(walk_stack=1), dict(walk_stack=2),
-                                   dict(walk_stack=32, trace_block=512)])
+                                   dict(walk_stack=32, trace_block=512), dict(eye_overlap=1),
+                                   dict(eye_overlap=1, passes_per_graph=1)])
 def test_launch_shapes_do_not_change_results(shape):
     """Odd image sizes so that partial tiles and partial ray blocks occur under every shape; walk
     stacks of 0-2 entries send most walks on over the pre-order records part way."""
@@ -151,11 +155,11 @@ def test_autotune_keeps_results_and_picks_a_valid_shape():
     r = g.Renderer(m, p)
     t = r.autotune(passes_per_trial=3)
     assert t["trace_block"] in (64, 128, 256, 512) and t["tile_wave_width"] in (1, 2, 4, 8, 16, 32, 64)
-    assert t["walk_stack"] in (8, 16, 32)
+    assert t["walk_stack"] in (8, 16, 32) and t["eye_overlap"] in (0, 1)
     assert t["ms_per_pass"] > 0
     done = r.counters()["frame_count"]
-    # 12 trials (the start, 3 + 4 + 1 + 2 alternatives, the winner), each 1 + 3 passes
-    assert done == 12 * 4
+    # 13 trials (the start, 3 + 4 + 1 + 2 + 1 alternatives, the winner), each 1 + 3 passes
+    assert done == 13 * 4
     a = g.Renderer(m, p)
     a.run(done)
     rep = compare_all(r, a, STATE_BUFFERS)
@@ -177,7 +181,8 @@ def test_tuning_out_of_range_is_refused():
     m = spheres(5, 4, 2, 32, 32)
     r = g.Renderer(m, g.default_params(32, max_path_length=4))
     for bad in (dict(trace_block=96), dict(trace_block=1024), dict(tile_wave_width=3), dict(passes_per_graph=2),
-                dict(walk_stack=-1), dict(walk_stack=33), dict(ms_per_pass=1.0), dict(no_such_field=1)):
+                dict(walk_stack=-1), dict(walk_stack=33), dict(eye_overlap=2), dict(ms_per_pass=1.0),
+                dict(no_such_field=1)):
         with pytest.raises(g.PPMError):
             r.set_tuning(**bad)
     assert r.tuning()["trace_block"] == 256
