@@ -432,7 +432,7 @@ int ppm_batch_autotune(ppm_batch* b, uint32_t passes_per_trial, ppm_tuning* best
         int L = 1;
         for (ppm_ctx* c : b->slots)
             if (c) L = std::max(L, c->prm.max_path_length);
-        passes_per_trial = 2u * (uint32_t)L;
+        passes_per_trial = 4u * (uint32_t)L;
     }
     ppm_tuning start;
     ppm_batch_get_tuning(b, &start);

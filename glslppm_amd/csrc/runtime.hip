@@ -1173,7 +1173,8 @@ int ppm_set_tuning(ppm_ctx* c, const ppm_tuning* t) {
 int ppm_autotune(ppm_ctx* c, uint32_t passes_per_trial, ppm_tuning* best) {
     if (!c) return PPM_E_INVAL;
     if (!c->own_stream) return set_err(c, PPM_E_INVAL, "this context belongs to a batch; use ppm_batch_autotune");
-    if (passes_per_trial == 0) passes_per_trial = 2u * (uint32_t)c->prm.max_path_length;
+    // four eye cadences per trial: shorter trials leave the choice to noise of a few percent
+    if (passes_per_trial == 0) passes_per_trial = 4u * (uint32_t)c->prm.max_path_length;
     ppm_tuning start;
     ppm_get_tuning(c, &start);
     auto trial = [&](const ppm_tuning& t, float* ms) -> int {

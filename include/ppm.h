@@ -225,7 +225,7 @@ typedef struct ppm_tuning {
 
 int ppm_get_tuning(ppm_ctx* ctx, ppm_tuning* t);
 int ppm_set_tuning(ppm_ctx* ctx, const ppm_tuning* t);
-/* passes_per_trial 0: twice max_path_length. */
+/* passes_per_trial 0: four times max_path_length. */
 int ppm_autotune(ppm_ctx* ctx, uint32_t passes_per_trial, ppm_tuning* chosen);
 
 void ppm_destroy(ppm_ctx* ctx);
