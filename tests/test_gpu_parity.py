@@ -210,3 +210,27 @@ def test_axis_parallel_rays():
         _, d, _, _ = oracle.camera_ray(m.scene, p, i, j, 0.3, -0.4, (0.5, 0.5, 0.5, 0.5))
         assert tuple(d) == (0.0, 0.0, -1.0)
     _check(m, p, 13, "axis-parallel")
+
+
+def test_duplicated_triangles_tie_by_walk_order():
+    """Every non-light triangle of the Cornell room twice, the copy in another colour: every eye
+    ray and every photon meets two triangles at exactly the same distance, and the reference keeps
+    the one its walk tests first (strict t < closest, eyeraytrace.fs:140).  The four-wide walk with
+    its deferred leaf tests must keep that same one everywhere."""
+    a = g.Mesh.generate(g.SCENE_CORNELL).build().arrays()
+    pos = a["positions"].reshape(-1, 3, 3).astype(np.float32)
+    nrm = a["normals"].reshape(-1, 3, 3).astype(np.float32)
+    mat = np.asarray(a["material"], np.int32)
+    mats = list(a["materials"])
+    light = np.array([m[2] == -1 for m in mats])[mat]
+    extra = [((0.2, 0.3, 0.9), 1.7, 0, 0.0), ((0.9, 0.9, 0.2), 1.7, 1, 0.0)]  # diffuse blue, mirror yellow
+    base = len(mats)
+    dup_mat = np.where(np.array([mats[k][2] == 1 for k in mat]), base + 1, base)[~light]
+    m = g.Mesh.from_arrays(np.concatenate([pos, pos[~light]]), mats + extra,
+                           np.concatenate([mat, dup_mat]), np.concatenate([nrm, nrm[~light]]))
+    m.set_camera((0, 0, 1.667), (0, 0, 0), 48, 48, 45.0).build()
+    assert m.num_triangles == len(pos) + int((~light).sum())
+    gpu, cpu = _check(m, g.default_params(48, hash_width=64, hash_height=32, max_path_length=8), 17, "duplicated")
+    # both copies are hit somewhere: the tie goes to one copy or the other by walk order
+    tri = gpu.readback("hit_triangle")[:, 0]
+    assert (tri >= len(pos)).any() and ((tri >= 0) & (tri < len(pos))).any()
