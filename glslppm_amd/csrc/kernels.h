@@ -86,7 +86,7 @@ struct DevGlobals {
     int first_pass;                  // this pass is frame 0
     int num_eye_samples;             // the NumEyeSamples uniform of this eye pass (count + 1)
     float aa_x, aa_y;                // AAOffset of this eye pass
-    int pad0;
+    int order_tiles;                 // the tile count tile_order was built for (0: none yet)
 };
 
 struct HitState {
@@ -100,6 +100,8 @@ struct HitState {
     float2* mat_t;
     uchar4* rgba8;
     float* partial;  // [blocks][6] min xyz, max xyz of the eye kernel
+    int* tile_cost;  // [tiles] segments of the deepest eye path of each tile in the last eye pass
+    int* tile_order; // [tiles] the order k_eye takes the tiles in: deepest paths first
 };
 
 // PhotonState::slot of a photon whose deposit waits for k_hash (eye passes).

@@ -600,10 +600,15 @@ PPM_DEV void eye(const Job& J, int blk) {
     const EyeParams& ep = J.ep;
     const HitState& hs = J.hs;
     const float aa_x = g->aa_x, aa_y = g->aa_y;
+    // the tiles whose paths went deepest in the last eye pass first (k_grid's order): a path is a
+    // chain of dependent walks and a wave lasts as long as its deepest path, so the longest waves
+    // start first and the short ones fill in around them
+    const int tile = g->order_tiles == J.tile_blocks ? hs.tile_order[blk] : blk;
     int px_i, px_j;
-    tile_pixel(J.tl, blk, px_i, px_j);
+    tile_pixel(J.tl, tile, px_i, px_j);
     const int p = px_j * ep.width + px_i;
     f3 pos = mk3(0.0f, 0.0f, 0.0f);
+    int segs = 0;
     const bool active = px_i < ep.width && px_j < ep.height;
     if (active) {
         float4 rndv = hs.rng[p];
@@ -629,6 +634,7 @@ PPM_DEV void eye(const Job& J, int blk) {
         int last_tri = -1, last_mat = -1;
         float last_t = 1.0e20f;
         for (int j = 0; j < ep.max_path; j++) {
+            segs = j + 1;
             const Hit h = trace<false>(s, org, dir, j == 0, J.walk_stack);
             last_tri = h.tri;
             last_t = h.t;
@@ -723,13 +729,17 @@ PPM_DEV void eye(const Job& J, int blk) {
     // max.fs / min.fs over every pixel's position (misses contribute their last position): one
     // partial per block, reduced by k_grid
     __shared__ float red[kWaves][6];
+    __shared__ int deep[kWaves];
+    const int d = wave_max(active ? segs : 0);
     float v[6] = {active ? pos.x : kInf, active ? pos.y : kInf, active ? pos.z : kInf,
                   active ? pos.x : -kInf, active ? pos.y : -kInf, active ? pos.z : -kInf};
     for (int k = 0; k < 3; k++) v[k] = wave_min(v[k]);
     for (int k = 3; k < 6; k++) v[k] = wave_max(v[k]);
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    if (lane == 0)
+    if (lane == 0) {
         for (int k = 0; k < 6; k++) red[wave][k] = v[k];
+        deep[wave] = d;
+    }
     __syncthreads();
     if (threadIdx.x == 0) {
         float r[6];
@@ -739,6 +749,9 @@ PPM_DEV void eye(const Job& J, int blk) {
             for (int k = 3; k < 6; k++) r[k] = max1(r[k], red[w][k]);
         }
         for (int k = 0; k < 6; k++) hs.partial[blk * 6 + k] = r[k];
+        int dm = deep[0];
+        for (int w = 1; w < kWaves; w++) dm = dm > deep[w] ? dm : deep[w];
+        hs.tile_cost[tile] = dm;
     }
 }
 
@@ -786,6 +799,26 @@ PPM_DEV void grid(const Job& J) {
         g->grid_scale = gs;
         g->hash_scale = 4194304.0f / gs;
     }
+    // the next eye pass's tile order: a counting sort of the tiles by the depth of their deepest
+    // path, deepest first (any order of the tiles gives the same results)
+    __shared__ unsigned int hist[33];
+    const int n = J.tile_blocks;
+    if (threadIdx.x < 33) hist[threadIdx.x] = 0u;
+    __syncthreads();
+    const HitState& hs = J.hs;
+    for (int b = threadIdx.x; b < n; b += 1024) atomicAdd(&hist[min(max(hs.tile_cost[b], 0), 32)], 1u);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        unsigned int run = 0;
+        for (int c = 32; c >= 0; c--) {
+            const unsigned int k = hist[c];
+            hist[c] = run;
+            run += k;
+        }
+        g->order_tiles = n;
+    }
+    __syncthreads();
+    for (int b = threadIdx.x; b < n; b += 1024) hs.tile_order[atomicAdd(&hist[min(max(hs.tile_cost[b], 0), 32)], 1u)] = b;
 }
 
 }  // namespace

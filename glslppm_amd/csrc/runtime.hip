@@ -600,6 +600,7 @@ size_t state_footprint(const ppm_params* p) {
     const size_t sizes[] = {
         // hit points: pos, refl, nrm, rng, (tau, R), (E, N), tri, (mat, t), rgba8, eye partials
         Q * 16, Q * 16, Q * 16, Q * 16, Q * 16, Q * 16, Q * 4, Q * 8, Q * 4, (size_t)max_tile_blocks(p) * 24,
+        (size_t)max_tile_blocks(p) * 4, (size_t)max_tile_blocks(p) * 4,  // tile costs and order
         // photons: pos, flux, dir, rng, state, slot, emitted partials
         P * 16, P * 16, P * 16, P * 16, P * 8, P * 4, (size_t)photon_blocks(p) * 4,
         // rays: org, dir, aux, rank, perm, per-block counts
@@ -811,7 +812,9 @@ int create_context(ppm_ctx* c, std::shared_ptr<SceneDev> scene, const ppm_camera
     if ((rc = dev_alloc(c, &hs.pos, Q)) || (rc = dev_alloc(c, &hs.refl, Q)) || (rc = dev_alloc(c, &hs.nrm, Q)) ||
         (rc = dev_alloc(c, &hs.rng, Q)) || (rc = dev_alloc(c, &hs.tau_r, Q)) || (rc = dev_alloc(c, &hs.e_n, Q)) ||
         (rc = dev_alloc(c, &hs.tri, Q)) || (rc = dev_alloc(c, &hs.mat_t, Q)) || (rc = dev_alloc(c, &hs.rgba8, Q)) ||
-        (rc = dev_alloc(c, &hs.partial, (size_t)max_tile_blocks(p) * 6)) || (rc = dev_alloc(c, &ps.pos, P)) ||
+        (rc = dev_alloc(c, &hs.partial, (size_t)max_tile_blocks(p) * 6)) ||
+        (rc = dev_alloc(c, &hs.tile_cost, (size_t)max_tile_blocks(p))) ||
+        (rc = dev_alloc(c, &hs.tile_order, (size_t)max_tile_blocks(p))) || (rc = dev_alloc(c, &ps.pos, P)) ||
         (rc = dev_alloc(c, &ps.flux, P)) || (rc = dev_alloc(c, &ps.dir, P)) || (rc = dev_alloc(c, &ps.rng, P)) ||
         (rc = dev_alloc(c, &ps.state, P)) || (rc = dev_alloc(c, &ps.slot, P)) ||
         (rc = dev_alloc(c, &ps.partial, (size_t)J.photon_blocks)) || (rc = dev_alloc(c, &rq.org, P)) ||
