@@ -19,10 +19,12 @@
 //           mats[M][2] float4      (rgb, eta), (brdf, g, 0, 0)
 //           tri_tex[T][2] float4   (u0, v0, u1, v1), (u2, v2, 0, 0)     only with textures
 //           volume[M][S][S] uchar4 RGB8 + pad of every material (S = 512)  only with textures
-//   hit points (Q): pos, refl, nrm, rng, (tau, R), (E, N) float4; tri int; (mat, t) float2
+//   hit points (Q): pos, refl, nrm, rng, (tau, R), (E, N) float4; tri int; (mat, t) float2;
+//                   per pixel tile: depth of its deepest eye path, the next eye pass's tile order
 //   photons (P):    pos, flux, dir, rng float4 (.w of pos/flux/dir = hit normal);
 //                   state int2 (triangle, bounce | emitted << 16); slot int
-//   hash (H):       (count u32, representative u32) per slot (0xffffffff = none)
+//   hash (H):       (count u32, representative u32) per slot (0xffffffff = none); a photon's slot
+//                   is kSlotPending between an eye pass's photon trace and its k_hash
 #pragma once
 
 #include "fmath.h"
