@@ -103,7 +103,7 @@ struct ppm_ctx {
     double prof_ms[PPM_KERNEL_KINDS] = {0, 0, 0, 0};
     uint64_t prof_n[PPM_KERNEL_KINDS] = {0, 0, 0, 0};
     uint64_t frame = 0;             // passes queued by ppm_pass (the plain loop's eye cadence)
-    int trace_block = 256;          // launch shape (ppm_set_tuning)
+    int trace_block = kTraceBlockDefault;  // launch shape (ppm_set_tuning)
     bool use_graph = false;         // replay passes as hipGraphs (PPM_FLAG_GRAPH)
     bool eye_overlap = false;       // eye stage on `side` in eye passes (ppm_set_tuning)
     // hipGraphs of one photon-only pass and one eye pass (PPM_FLAG_GRAPH)
@@ -120,7 +120,7 @@ struct ppm_batch {
     hipStream_t upload = nullptr;   // scene uploads and job initialisation
     ppm_internal::Arena* arena = nullptr;
     uint32_t passes_per_graph = 1;  // 0: plain launches
-    int trace_block = 256;          // launch shape shared by every job (ppm_batch_set_tuning)
+    int trace_block = kTraceBlockDefault;  // launch shape shared by every job (ppm_batch_set_tuning)
     int wave_shift = 3;
     int walk_stack = kWalkStack;
     bool eye_overlap = true;        // eye stage on `side` every pass

@@ -173,6 +173,7 @@ struct Tiles {
 
 // Launch shapes that leave every result unchanged, chosen by the autotuner.
 constexpr int kTraceBlockMax = 512;
+constexpr int kTraceBlockDefault = 64;
 
 // Stack entries per lane of the four-wide walks (default, most: 32 x 512 lanes x 4 bytes = 64 KiB
 // of LDS); deeper walks continue on the pre-order records.

@@ -185,4 +185,4 @@ def test_tuning_out_of_range_is_refused():
                 dict(no_such_field=1)):
         with pytest.raises(g.PPMError):
             r.set_tuning(**bad)
-    assert r.tuning()["trace_block"] == 256
+    assert r.tuning()["trace_block"] == 64
