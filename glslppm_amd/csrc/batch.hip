@@ -84,6 +84,7 @@ void queue_batch_pass(ppm_batch* b, hipStream_t st) {
         (void)hipEventRecord(b->ev_fork, st);
         (void)hipStreamWaitEvent(es, b->ev_fork, 0);
     }
+    hipLaunchKernelGGL(k_eye_begin_batch, dim3((n + 63) / 64), dim3(64), 0, es, J, n);
     hipLaunchKernelGGL(k_eye_batch, dim3(b->n_blocks[kMapTile]), dim3(256), walk_lds_bytes(b->walk_stack, 256), es, J,
                        (const BlockRef*)b->d_map[kMapTile]);
     hipLaunchKernelGGL(k_grid_batch, dim3(n), dim3(1024), 0, es, J);

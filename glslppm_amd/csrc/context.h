@@ -77,6 +77,10 @@ struct SceneDev {
 
 }  // namespace ppm_internal
 
+// What one launch sequence of the plain loop does (queue_pass): a photon pass, an eye pass that
+// runs its own eye stage, an eye pass whose eye stage ran ahead, and that ahead stage alone.
+enum PassKind { kPhotonPass = 0, kEyePass = 1, kEyePassReady = 2, kEyeAhead = 3, kPassKinds = 4 };
+
 struct ppm_ctx {
     int device = 0;
     hipStream_t stream = nullptr;   // the stream passes run on
@@ -85,6 +89,9 @@ struct ppm_ctx {
     // deposits): only contexts that own their stream have one
     hipStream_t side = nullptr;
     hipEvent_t ev_fork = nullptr, ev_join = nullptr;
+    // the eye stage of the next eye pass, started on `side` after an eye pass (ppm_pass)
+    hipEvent_t ev_ahead = nullptr;
+    bool ahead_pending = false;
     ppm_internal::Arena* arena = nullptr;  // null: hipMalloc / hipFree
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     bool timed = false;
@@ -105,11 +112,18 @@ struct ppm_ctx {
     uint64_t frame = 0;             // passes queued by ppm_pass (the plain loop's eye cadence)
     int trace_block = kTraceBlockDefault;  // launch shape (ppm_set_tuning)
     bool use_graph = false;         // replay passes as hipGraphs (PPM_FLAG_GRAPH)
-    bool eye_overlap = false;       // eye stage on `side` in eye passes (ppm_set_tuning)
-    // hipGraphs of one photon-only pass and one eye pass (PPM_FLAG_GRAPH)
-    hipGraphExec_t graph_pass[2] = {nullptr, nullptr};
+    bool eye_overlap = false;       // eye stages on `side`, ahead of their eye passes (ppm_set_tuning)
+    // hipGraphs of one pass of each kind (PPM_FLAG_GRAPH)
+    hipGraphExec_t graph_pass[kPassKinds] = {nullptr, nullptr, nullptr, nullptr};
     std::string err;
 };
+
+// Waits for everything queued on a context's streams.
+inline hipError_t sync_streams(ppm_ctx* c) {
+    hipError_t e = hipStreamSynchronize(c->stream);
+    if (e == hipSuccess && c->side) e = hipStreamSynchronize(c->side);
+    return e;
+}
 
 // The batch (ppm_batch_*): resident jobs advanced by one launch per stage per pass.
 struct ppm_batch {

@@ -69,16 +69,24 @@ struct DevCamera {
 // needs a host round trip and the pass sequence is the same launch sequence every pass (which a
 // hipGraph can replay, and which many jobs can share).  k_pass_begin writes the per-pass fields
 // from the job's frame count and its own XORShift stream (glslppm.cpp:214-287).
-struct DevGlobals {
+// The grid scalars an eye pass sets up (glslppm.cpp:294-311).
+struct GridScalars {
     float bbmin[3];
     float bbmax[3];
     float initial_radius;
     float grid_scale;
     float hash_scale;
-    float num_photons_f;
+    float pad;
+};
+
+struct DevGlobals {
+    // of the latest eye pass of each parity: the passes after eye pass e read grid[e & 1], as they
+    // read that eye pass's hit points (HitState buffers)
+    GridScalars grid[2];
     unsigned long long photons_emitted;
     unsigned long long frame;        // passes run (FrameCount)
-    unsigned long long eye_samples;  // eye passes run
+    unsigned long long eye_samples;  // eye passes begun; the latest one's buffers are (eye_samples - 1) & 1
+    float num_photons_f;
     unsigned int emitted_this_pass;
     unsigned int queue_len;          // continuing photons of this pass (written by k_bucket_scan)
     unsigned int xs[4];              // XORShift state after the seed grids (AAOffset draws)
@@ -86,20 +94,32 @@ struct DevGlobals {
     int active;                      // this pass advances the job
     int eye_now;                     // this pass traces eye rays (FrameCount % MaxPathLength == 0)
     int first_pass;                  // this pass is frame 0
-    int num_eye_samples;             // the NumEyeSamples uniform of this eye pass (count + 1)
-    float aa_x, aa_y;                // AAOffset of this eye pass
+    int num_eye_samples;             // the NumEyeSamples uniform of the eye stage in progress
+    float aa_x, aa_y;                // AAOffset of the eye stage in progress
     int order_tiles;                 // the tile count tile_order was built for (0: none yet)
+    unsigned int eye_stages;         // eye stages begun (k_eye_begin)
+    int eye_stage;                   // the eye stage in progress; it writes buffers eye_stage & 1
+    int eye_go;                      // an eye stage runs in this launch sequence (k_eye_begin)
+    int defer;                       // this pass's deposits wait for k_hash (its grid is not set up)
+    unsigned int eye_stages_done;    // eye stages whose grid is set up (k_grid)
+    int pad1;
 };
 
+// The eye pass's outputs are double-buffered by eye-pass parity: eye pass e writes buffer e & 1
+// (pos, refl, nrm, tri, mat_t, rng, em at [b * q + pixel]) from buffer (e - 1) & 1 (rng, em), and
+// the passes until the next eye pass read buffer e & 1, so an eye stage can run while the passes
+// before it still read the previous one.  tau_r and e_n (progressive statistics) are single.
 struct HitState {
-    float4* pos;
-    float4* refl;
-    float4* nrm;
-    float4* rng;
+    int q;         // pixels (Q)
+    float4* pos;   // [2][Q]
+    float4* refl;  // [2][Q]
+    float4* nrm;   // [2][Q]
+    float4* rng;   // [2][Q]
+    float4* em;    // [2][Q] emission average (E of (E, N)); the gather copies it into e_n.xyz
     float4* tau_r;
     float4* e_n;
-    int* tri;
-    float2* mat_t;
+    int* tri;      // [2][Q]
+    float2* mat_t; // [2][Q]
     uchar4* rgba8;
     float* partial;  // [blocks][6] min xyz, max xyz of the eye kernel
     int* tile_cost;  // [tiles] segments of the deepest eye path of each tile in the last eye pass
@@ -216,6 +236,7 @@ struct BlockRef {
 // for k_grid and k_bucket_scan).
 __global__ void k_pass_begin(Job J);
 __global__ void k_clear_slots(Job J);
+__global__ void k_eye_begin(Job J, int ahead);
 __global__ void k_eye(Job J);
 __global__ void k_grid(Job J);
 __global__ void k_photon_gen(Job J);
@@ -226,6 +247,7 @@ __global__ void k_hash(Job J);
 __global__ void k_gather(Job J);
 __global__ void k_pass_begin_batch(const Job* jobs, int n_jobs);
 __global__ void k_clear_slots_batch(const Job* jobs, const BlockRef* map);
+__global__ void k_eye_begin_batch(const Job* jobs, int n_jobs);
 __global__ void k_eye_batch(const Job* jobs, const BlockRef* map);
 __global__ void k_grid_batch(const Job* jobs);
 __global__ void k_photon_gen_batch(const Job* jobs, const BlockRef* map);

@@ -527,6 +527,8 @@ __global__ void k_init_state(Job J) {
     if (i < J.gp.count) {
         J.hs.e_n[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
         J.hs.tau_r[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        J.hs.em[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        J.hs.em[J.hs.q + i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
     }
     if (i < J.pp.count) {
         J.ps.pos[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
@@ -568,16 +570,35 @@ PPM_DEV void pass_begin(const Job& J) {
     g->active = 1;
     g->eye_now = eye;
     g->first_pass = frame == 0ull;
-    if (eye) {
-        unsigned int xs[4] = {g->xs[0], g->xs[1], g->xs[2], g->xs[3]};
-        g->aa_x = (xs_frand(xs) - 0.5f) * 1.25f;
-        g->aa_y = (xs_frand(xs) - 0.5f) * 1.25f;
-        for (int k = 0; k < 4; k++) g->xs[k] = xs[k];
-        g->num_eye_samples = (int)(g->eye_samples + 1ull);
-        g->eye_samples = g->eye_samples + 1ull;
-    }
+    if (eye) g->eye_samples = g->eye_samples + 1ull;
+    // the deposits of an eye pass wait for k_hash unless its eye stage has already set the grid up
+    g->defer = eye && (unsigned long long)g->eye_stages_done < g->eye_samples;
     g->frame = frame + 1ull;
 }
+
+// The start of an eye stage (glslppm.cpp:238-287): the AAOffset draws of the job's XORShift
+// stream, NumEyeSamples, and the buffers the stage writes.  In an eye pass this is the stage of
+// that pass unless it has already begun; `ahead` begins the stage of the next eye pass instead
+// (after an eye pass, beside the photon passes that follow it).  Either way stages begin in eye
+// pass order and draw the same numbers, so the results do not depend on where a stage ran.
+PPM_DEV void eye_begin(const Job& J, int ahead) {
+    DevGlobals* g = J.g;
+    const unsigned long long begun = g->eye_stages;
+    const int go = g->active && (ahead ? begun == g->eye_samples : g->eye_now && begun < g->eye_samples);
+    g->eye_go = go;
+    if (!go) return;
+    unsigned int xs[4] = {g->xs[0], g->xs[1], g->xs[2], g->xs[3]};
+    g->aa_x = (xs_frand(xs) - 0.5f) * 1.25f;
+    g->aa_y = (xs_frand(xs) - 0.5f) * 1.25f;
+    for (int k = 0; k < 4; k++) g->xs[k] = xs[k];
+    const unsigned int e = g->eye_stages;
+    g->eye_stage = (int)e;
+    g->num_eye_samples = (int)(e + 1u);
+    g->eye_stages = e + 1u;
+}
+
+// The buffer of the latest eye pass, which the passes after it read.
+PPM_DEV int cur_buffer(const DevGlobals* g) { return (int)((g->eye_samples + 1ull) & 1ull); }
 
 PPM_DEV void clear_slots(const Job& J, int blk) {
     if (!J.g->active) return;
@@ -594,7 +615,7 @@ namespace {
 // eyeraytrace.fs:396-554 for one pixel, then the min/max of hit positions.
 PPM_DEV void eye(const Job& J, int blk) {
     const DevGlobals* g = J.g;
-    if (!g->active || !g->eye_now) return;
+    if (!g->eye_go) return;
     const DevScene& s = J.s;
     const DevCamera& cam = J.cam;
     const EyeParams& ep = J.ep;
@@ -610,8 +631,11 @@ PPM_DEV void eye(const Job& J, int blk) {
     f3 pos = mk3(0.0f, 0.0f, 0.0f);
     int segs = 0;
     const bool active = px_i < ep.width && px_j < ep.height;
+    // this stage writes buffer b from the previous stage's buffer (generator states, emission)
+    const size_t wb = (size_t)(g->eye_stage & 1) * (size_t)hs.q + (size_t)p;
+    const size_t rb = (size_t)((g->eye_stage & 1) ^ 1) * (size_t)hs.q + (size_t)p;
     if (active) {
-        float4 rndv = hs.rng[p];
+        float4 rndv = hs.rng[rb];
         const f3 C = mk3(cam.ox, cam.oy, cam.oz);
         const f3 U = mk3(cam.ux, cam.uy, cam.uz), V = mk3(cam.vx, cam.vy, cam.vz), W = mk3(cam.wx, cam.wy, cam.wz);
         const float ppx = ((float)px_i + 0.5f + aa_x) - cam.half_w;
@@ -712,18 +736,18 @@ PPM_DEV void eye(const Job& J, int blk) {
                 }
             }
         }
-        float4 en = hs.e_n[p];
+        float4 en = hs.em[rb];
         const float a = 1.0f / (float)g->num_eye_samples;
         en.x = en.x * (1.0f - a) + emi.x * a;
         en.y = en.y * (1.0f - a) + emi.y * a;
         en.z = en.z * (1.0f - a) + emi.z * a;
-        hs.pos[p] = make_float4(pos.x, pos.y, pos.z, dir.x);
-        hs.refl[p] = make_float4(col.x, col.y, col.z, dir.y);
-        hs.nrm[p] = make_float4(nrm.x, nrm.y, nrm.z, dir.z);
-        hs.rng[p] = rndv;
-        hs.tri[p] = last_tri;
-        hs.mat_t[p] = make_float2(last_mat < 0 ? -1.0f : (float)last_mat, last_t);
-        hs.e_n[p] = en;
+        hs.pos[wb] = make_float4(pos.x, pos.y, pos.z, dir.x);
+        hs.refl[wb] = make_float4(col.x, col.y, col.z, dir.y);
+        hs.nrm[wb] = make_float4(nrm.x, nrm.y, nrm.z, dir.z);
+        hs.rng[wb] = rndv;
+        hs.tri[wb] = last_tri;
+        hs.mat_t[wb] = make_float2(last_mat < 0 ? -1.0f : (float)last_mat, last_t);
+        hs.em[wb] = en;
     }
 
     // max.fs / min.fs over every pixel's position (misses contribute their last position): one
@@ -759,7 +783,8 @@ PPM_DEV void eye(const Job& J, int blk) {
 // glslppm.cpp:294-311 (one block of 1024 threads).
 PPM_DEV void grid(const Job& J) {
     DevGlobals* g = J.g;
-    if (!g->active || !g->eye_now) return;
+    if (!g->eye_go) return;
+    GridScalars& gr = g->grid[g->eye_stage & 1];
     const EyeParams& ep = J.ep;
     const float* partial = J.hs.partial;
     const int n_partial = J.tile_blocks;
@@ -791,13 +816,13 @@ PPM_DEV void grid(const Job& J) {
         float ir = (bbsize / 3.0f) / ep.footprint_width * ep.footprint;
         if (ep.fixed_radius > 0.0f) ir = ep.fixed_radius;
         for (int k = 0; k < 3; k++) {
-            g->bbmin[k] = bmin[k] - ir;
-            g->bbmax[k] = bmax[k] + ir;
+            gr.bbmin[k] = bmin[k] - ir;
+            gr.bbmax[k] = bmax[k] + ir;
         }
         const float gs = 0.5f / ir;
-        g->initial_radius = ir;
-        g->grid_scale = gs;
-        g->hash_scale = 4194304.0f / gs;
+        gr.initial_radius = ir;
+        gr.grid_scale = gs;
+        gr.hash_scale = 4194304.0f / gs;
     }
     // the next eye pass's tile order: a counting sort of the tiles by the depth of their deepest
     // path, deepest first (any order of the tiles gives the same results)
@@ -816,6 +841,8 @@ PPM_DEV void grid(const Job& J) {
             run += k;
         }
         g->order_tiles = n;
+        g->eye_stages_done = (unsigned int)g->eye_stage + 1u;
+        g->eye_go = 0;  // every thread read it before the first barrier
     }
     __syncthreads();
     for (int b = threadIdx.x; b < n; b += 1024) hs.tile_order[atomicAdd(&hist[min(max(hs.tile_cost[b], 0), 32)], 1u)] = b;
@@ -1061,9 +1088,10 @@ struct Walk {
 // The slot of a photon that deposits at hpos (hash.fs), and the scatter and correction updates of
 // that slot; -1 for a slot past the last row, which lands outside the hash viewport and is clipped.
 PPM_DEV int deposit(const PhotonParams& pp, const HashState& h, const DevGlobals* g, int p, f3 hpos) {
-    const f3 cell = mk3(floorf((hpos.x - g->bbmin[0]) * g->grid_scale), floorf((hpos.y - g->bbmin[1]) * g->grid_scale),
-                        floorf((hpos.z - g->bbmin[2]) * g->grid_scale));
-    const float sf = cell_hash(cell, g->grid_scale, g->hash_scale, h.num_f);
+    const GridScalars& gr = g->grid[cur_buffer(g)];
+    const f3 cell = mk3(floorf((hpos.x - gr.bbmin[0]) * gr.grid_scale), floorf((hpos.y - gr.bbmin[1]) * gr.grid_scale),
+                        floorf((hpos.z - gr.bbmin[2]) * gr.grid_scale));
+    const float sf = cell_hash(cell, gr.grid_scale, gr.hash_scale, h.num_f);
     if (!(sf < h.num_f)) return -1;
     const int slot = (int)sf;
     atomicAdd(h.slots + 2 * slot, 1u);
@@ -1100,14 +1128,14 @@ PPM_DEV void photon_finish(const DevScene& s, const PhotonParams& pp, const Phot
     // hash.fs: deposits with flux.x >= 0 (including -0) go to the grid.  In an eye pass the grid
     // is still being set up while the photons fly, so the deposit waits for k_hash (kSlotPending).
     int slot = -1;
-    if (!(flux.x < 0.0f)) slot = g->eye_now ? kSlotPending : deposit(pp, h, g, p, hpos);
+    if (!(flux.x < 0.0f)) slot = g->defer ? kSlotPending : deposit(pp, h, g, p, hpos);
     ps.slot[p] = slot;
 }
 
 // The deposits an eye pass held back, once k_grid has set the grid up.
 PPM_DEV void hash_deferred(const Job& J, int blk) {
     const DevGlobals* g = J.g;
-    if (!g->active || !g->eye_now) return;
+    if (!g->active || !g->defer) return;
     const int p = blk * kBlock + threadIdx.x;
     if (p >= J.pp.count || J.ps.slot[p] != kSlotPending) return;
     J.ps.slot[p] = deposit(J.pp, J.h, g, p, xyz(J.ps.pos[p]));
@@ -1280,16 +1308,27 @@ PPM_DEV void gather(const Job& J, int blk) {
     tile_pixel(tl, blk, px_i, px_j);
     if (px_i >= tl.width || px_j >= tl.height) return;
     const int p = px_j * tl.width + px_i;
-    float4 tr = g->first_pass ? make_float4(0.0f, 0.0f, 0.0f, g->initial_radius) : hs.tau_r[p];
+    // the hit points, emission and grid of the latest eye pass
+    const int cb = cur_buffer(g);
+    const GridScalars& gr = g->grid[cb];
+    const size_t hp = (size_t)cb * (size_t)hs.q + (size_t)p;
+    float4 tr = g->first_pass ? make_float4(0.0f, 0.0f, 0.0f, gr.initial_radius) : hs.tau_r[p];
     float4 en = hs.e_n[p];
-    if (hs.tri[p] >= 0) {
-        const f3 qpos = xyz(hs.pos[p]);
-        const f3 qnrm = xyz(hs.nrm[p]);
-        const f3 refl = xyz(hs.refl[p]);
+    if (g->eye_now) {
+        // the emission changes with the eye stage only: between eye passes e_n holds it already
+        const float4 em = hs.em[hp];
+        en.x = em.x;
+        en.y = em.y;
+        en.z = em.z;
+    }
+    if (hs.tri[hp] >= 0) {
+        const f3 qpos = xyz(hs.pos[hp]);
+        const f3 qnrm = xyz(hs.nrm[hp]);
+        const f3 refl = xyz(hs.refl[hp]);
         const float R = tr.w;
         const float N = en.w;
-        const f3 bbmin = mk3(g->bbmin[0], g->bbmin[1], g->bbmin[2]);
-        const float gs = g->grid_scale, hsc = g->hash_scale;
+        const f3 bbmin = mk3(gr.bbmin[0], gr.bbmin[1], gr.bbmin[2]);
+        const float gs = gr.grid_scale, hsc = gr.hash_scale;
         const f3 d = qpos - bbmin;
         const f3 ii = mk3(floorf(fabsf(d.x) * gs - 0.5f), floorf(fabsf(d.y) * gs - 0.5f),
                           floorf(fabsf(d.z) * gs - 0.5f));
@@ -1372,6 +1411,15 @@ PPM_DEV void gather(const Job& J, int blk) {
 
 PPM_JOB_KERNEL(k_clear_slots, clear_slots, kBlock)
 PPM_JOB_KERNEL_WAVES(k_eye, eye, kBlock, 6)
+
+__global__ void k_eye_begin(Job J, int ahead) {
+    if (threadIdx.x == 0) eye_begin(J, ahead);
+}
+
+__global__ void k_eye_begin_batch(const Job* __restrict__ jobs, int n_jobs) {
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j < n_jobs) eye_begin(jobs[j], 0);
+}
 PPM_JOB_KERNEL1(k_grid, grid, 1024)
 PPM_JOB_KERNEL(k_photon_gen, photon_gen_blk, kBlock)
 PPM_JOB_KERNEL1(k_bucket_scan, bucket_scan, 1024)

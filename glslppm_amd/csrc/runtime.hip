@@ -397,28 +397,40 @@ int rep_limit(const ppm_params& p) {
     return (int)(P - std::max(0ll, ex));
 }
 
-// Launches one pass of job J on stream st: m_display's stage order (glslppm.cpp:214-536).  The
-// eye stage is queued only when `eye` is set; its kernels also check the device-side cadence.
+// An eye stage of job J on stream es: AAOffset and NumEyeSamples (k_eye_begin), the eye trace
+// and the grid set-up.  `ahead` begins the next eye pass's stage instead of this pass's.
+void queue_eye_stage(ppm_ctx* c, hipStream_t es, int ahead) {
+    const Job& J = c->job;
+    hipEvent_t e0 = prof_begin(c, es);
+    hipLaunchKernelGGL(k_eye_begin, dim3(1), dim3(64), 0, es, J, ahead);
+    hipLaunchKernelGGL(k_eye, dim3(J.tile_blocks), dim3(256), walk_lds_bytes(J.walk_stack, 256), es, J);
+    hipLaunchKernelGGL(k_grid, dim3(1), dim3(1024), 0, es, J);
+    prof_end(c, PPM_KERNEL_EYE, e0, es);
+}
+
+// Launches one pass of job J on stream st: m_display's stage order (glslppm.cpp:214-536).
 // The eye stage (eye trace and grid set-up) and the photon trace do not read each other's
 // results: only the deposits need the grid.  So in an eye pass the eye stage runs on the side
 // stream beside the photon stage, whose deposits wait (kSlotPending) for k_hash after the join;
 // the deep eye paths, chains of dependent walks that keep few lanes busy, then share the GPU with
-// the photon walk.
-void queue_pass(ppm_ctx* c, hipStream_t st, bool eye) {
+// the photon walk.  An eye pass whose stage ran ahead (kEyePassReady) deposits at once.
+// kEyeAhead is the stage of the next eye pass alone, for the side stream after an eye pass: the
+// passes in between read the other half of the double-buffered hit points.
+void queue_pass(ppm_ctx* c, hipStream_t st, int kind) {
+    if (kind == kEyeAhead) {
+        queue_eye_stage(c, st, 1);
+        return;
+    }
     const Job& J = c->job;
+    const bool stage = kind == kEyePass;
     hipLaunchKernelGGL(k_pass_begin, dim3(1), dim3(64), 0, st, J);
-    const bool fork = eye && c->side && c->eye_overlap;
+    const bool fork = stage && c->side && c->eye_overlap;
     hipStream_t es = fork ? c->side : st;
     if (fork) {
         (void)hipEventRecord(c->ev_fork, st);
         (void)hipStreamWaitEvent(es, c->ev_fork, 0);
     }
-    if (eye) {
-        hipEvent_t e0 = prof_begin(c, es);
-        hipLaunchKernelGGL(k_eye, dim3(J.tile_blocks), dim3(256), walk_lds_bytes(J.walk_stack, 256), es, J);
-        hipLaunchKernelGGL(k_grid, dim3(1), dim3(1024), 0, es, J);
-        prof_end(c, PPM_KERNEL_EYE, e0, es);
-    }
+    if (stage) queue_eye_stage(c, es, 0);
     if (fork) (void)hipEventRecord(c->ev_join, es);
     hipEvent_t e1 = prof_begin(c, st);
     hipLaunchKernelGGL(k_clear_slots, dim3(J.clear_blocks), dim3(256), 0, st, J);
@@ -432,22 +444,23 @@ void queue_pass(ppm_ctx* c, hipStream_t st, bool eye) {
     prof_end(c, PPM_KERNEL_PHOTON, e2, st);
     if (fork) (void)hipStreamWaitEvent(st, c->ev_join, 0);
     hipEvent_t e3 = prof_begin(c, st);
-    if (eye) hipLaunchKernelGGL(k_hash, dim3(J.photon_blocks), dim3(256), 0, st, J);
+    if (stage) hipLaunchKernelGGL(k_hash, dim3(J.photon_blocks), dim3(256), 0, st, J);
     hipLaunchKernelGGL(k_gather, dim3(J.tile_blocks), dim3(256), 0, st, J);
     prof_end(c, PPM_KERNEL_GATHER, e3, st);
 }
 
-// One captured pass (photon-only or eye pass) for PPM_FLAG_GRAPH.
-int capture_pass(ppm_ctx* c, int eye) {
-    if (c->graph_pass[eye]) return PPM_OK;
+// One captured pass of a kind (queue_pass) for PPM_FLAG_GRAPH.
+int capture_pass(ppm_ctx* c, int kind) {
+    if (c->graph_pass[kind]) return PPM_OK;
     hipGraph_t g = nullptr;
-    HIP_TRY(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+    hipStream_t st = kind == kEyeAhead ? c->side : c->stream;
+    HIP_TRY(c, hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
     const bool prof = c->profiling;
     c->profiling = false;
-    queue_pass(c, c->stream, eye != 0);
+    queue_pass(c, st, kind);
     c->profiling = prof;
-    HIP_TRY(c, hipStreamEndCapture(c->stream, &g));
-    const hipError_t e = hipGraphInstantiate(&c->graph_pass[eye], g, nullptr, nullptr, 0);
+    HIP_TRY(c, hipStreamEndCapture(st, &g));
+    const hipError_t e = hipGraphInstantiate(&c->graph_pass[kind], g, nullptr, nullptr, 0);
     (void)hipGraphDestroy(g);
     HIP_TRY(c, e);
     return PPM_OK;
@@ -581,10 +594,10 @@ void set_launch_shape(ppm_ctx* c, int trace_block, int wave_shift, int walk_stac
     J.tile_blocks = tile_blocks(J.tl.width, J.tl.height, wave_shift);
     c->trace_block = trace_block;
     J.trace_blocks = (c->P + trace_block - 1) / trace_block;
-    for (int k = 0; k < 2; k++)
+    for (int k = 0; k < kPassKinds; k++)
         if (c->graph_pass[k]) {
-            // a graph in flight keeps running; destroy it once the stream is idle
-            (void)hipStreamSynchronize(c->stream);
+            // a graph in flight keeps running; destroy it once the streams are idle
+            (void)sync_streams(c);
             (void)hipGraphExecDestroy(c->graph_pass[k]);
             c->graph_pass[k] = nullptr;
         }
@@ -598,8 +611,9 @@ size_t state_footprint(const ppm_params* p) {
     const size_t Q = (size_t)p->image_width * p->image_height, P = (size_t)p->photon_width * p->photon_height;
     const size_t H = (size_t)p->hash_width * p->hash_height;
     const size_t sizes[] = {
-        // hit points: pos, refl, nrm, rng, (tau, R), (E, N), tri, (mat, t), rgba8, eye partials
-        Q * 16, Q * 16, Q * 16, Q * 16, Q * 16, Q * 16, Q * 4, Q * 8, Q * 4, (size_t)max_tile_blocks(p) * 24,
+        // hit points: pos, refl, nrm, rng, emission (two buffers each), (tau, R), (E, N), tri, (mat, t)
+        // (two buffers each), rgba8, eye partials
+        Q * 32, Q * 32, Q * 32, Q * 32, Q * 32, Q * 16, Q * 16, Q * 8, Q * 16, Q * 4, (size_t)max_tile_blocks(p) * 24,
         (size_t)max_tile_blocks(p) * 4, (size_t)max_tile_blocks(p) * 4,  // tile costs and order
         // photons: pos, flux, dir, rng, state, slot, emitted partials
         P * 16, P * 16, P * 16, P * 16, P * 8, P * 4, (size_t)photon_blocks(p) * 4,
@@ -769,6 +783,7 @@ int create_context(ppm_ctx* c, std::shared_ptr<SceneDev> scene, const ppm_camera
         HIP_TRY(c, hipStreamCreateWithFlags(&c->side, hipStreamNonBlocking));
         HIP_TRY(c, hipEventCreateWithFlags(&c->ev_fork, hipEventDisableTiming));
         HIP_TRY(c, hipEventCreateWithFlags(&c->ev_join, hipEventDisableTiming));
+        HIP_TRY(c, hipEventCreateWithFlags(&c->ev_ahead, hipEventDisableTiming));
     }
     hipStream_t up = upload_st ? upload_st : c->stream;
     HIP_TRY(c, hipEventCreate(&c->ev0));
@@ -811,9 +826,11 @@ int create_context(ppm_ctx* c, std::shared_ptr<SceneDev> scene, const ppm_camera
     HitState& hs = J.hs;
     PhotonState& ps = J.ps;
     RayQueue& rq = J.rq;
-    if ((rc = dev_alloc(c, &hs.pos, Q)) || (rc = dev_alloc(c, &hs.refl, Q)) || (rc = dev_alloc(c, &hs.nrm, Q)) ||
-        (rc = dev_alloc(c, &hs.rng, Q)) || (rc = dev_alloc(c, &hs.tau_r, Q)) || (rc = dev_alloc(c, &hs.e_n, Q)) ||
-        (rc = dev_alloc(c, &hs.tri, Q)) || (rc = dev_alloc(c, &hs.mat_t, Q)) || (rc = dev_alloc(c, &hs.rgba8, Q)) ||
+    hs.q = (int)Q;
+    if ((rc = dev_alloc(c, &hs.pos, 2 * Q)) || (rc = dev_alloc(c, &hs.refl, 2 * Q)) ||
+        (rc = dev_alloc(c, &hs.nrm, 2 * Q)) || (rc = dev_alloc(c, &hs.rng, 2 * Q)) || (rc = dev_alloc(c, &hs.em, 2 * Q)) ||
+        (rc = dev_alloc(c, &hs.tau_r, Q)) || (rc = dev_alloc(c, &hs.e_n, Q)) || (rc = dev_alloc(c, &hs.tri, 2 * Q)) ||
+        (rc = dev_alloc(c, &hs.mat_t, 2 * Q)) || (rc = dev_alloc(c, &hs.rgba8, Q)) ||
         (rc = dev_alloc(c, &hs.partial, (size_t)max_tile_blocks(p) * 6)) ||
         (rc = dev_alloc(c, &hs.tile_cost, (size_t)max_tile_blocks(p))) ||
         (rc = dev_alloc(c, &hs.tile_order, (size_t)max_tile_blocks(p))) || (rc = dev_alloc(c, &ps.pos, P)) ||
@@ -834,7 +851,8 @@ int create_context(ppm_ctx* c, std::shared_ptr<SceneDev> scene, const ppm_camera
     std::vector<float4> eye_seeds, photon_seeds;
     seed_grid(xs, eye_seeds, Q);
     seed_grid(xs, photon_seeds, P);
-    HIP_TRY(c, hipMemcpyAsync(hs.rng, eye_seeds.data(), Q * sizeof(float4), hipMemcpyHostToDevice, up));
+    // the first eye stage (buffer 0) reads its generator states from buffer 1
+    HIP_TRY(c, hipMemcpyAsync(hs.rng + Q, eye_seeds.data(), Q * sizeof(float4), hipMemcpyHostToDevice, up));
     HIP_TRY(c, hipMemcpyAsync(ps.rng, photon_seeds.data(), P * sizeof(float4), hipMemcpyHostToDevice, up));
     DevGlobals g0;
     std::memset(&g0, 0, sizeof(g0));
@@ -857,7 +875,7 @@ void release_context(ppm_ctx* c) {
     c->ev_pending.clear();
     for (hipEvent_t e : c->ev_pool) (void)hipEventDestroy(e);
     c->ev_pool.clear();
-    for (int k = 0; k < 2; k++)
+    for (int k = 0; k < kPassKinds; k++)
         if (c->graph_pass[k]) {
             (void)hipGraphExecDestroy(c->graph_pass[k]);
             c->graph_pass[k] = nullptr;
@@ -873,9 +891,10 @@ void release_context(ppm_ctx* c) {
     if (c->ev1) (void)hipEventDestroy(c->ev1);
     if (c->ev_fork) (void)hipEventDestroy(c->ev_fork);
     if (c->ev_join) (void)hipEventDestroy(c->ev_join);
+    if (c->ev_ahead) (void)hipEventDestroy(c->ev_ahead);
     if (c->side) (void)hipStreamDestroy(c->side);
     if (c->stream && c->own_stream) (void)hipStreamDestroy(c->stream);
-    c->ev0 = c->ev1 = c->ev_fork = c->ev_join = nullptr;
+    c->ev0 = c->ev1 = c->ev_fork = c->ev_join = c->ev_ahead = nullptr;
     c->side = nullptr;
     c->stream = nullptr;
 }
@@ -954,15 +973,35 @@ int ppm_pass(ppm_ctx* c, uint32_t npasses) {
     if (npasses == 0) return PPM_OK;
     const uint64_t L = (uint64_t)c->prm.max_path_length;
     const bool graph = c->use_graph && !c->profiling;
+    // after each eye pass the next eye pass's stage starts on `side`, beside the photon passes in
+    // between, and that eye pass waits for it; with one pass per eye pass there is nothing between
+    const bool ahead = c->side && c->eye_overlap && L >= 2;
     if (graph) {
         int rc;
-        if ((rc = capture_pass(c, 0)) || (rc = capture_pass(c, 1))) return rc;
+        for (int k = 0; k < kPassKinds; k++)
+            if ((k != kEyeAhead || c->side) && (rc = capture_pass(c, k))) return rc;
     }
     HIP_TRY(c, hipEventRecord(c->ev0, c->stream));
     for (uint32_t it = 0; it < npasses; it++) {
         const bool eye = c->frame % L == 0;
-        if (graph) HIP_TRY(c, hipGraphLaunch(c->graph_pass[eye ? 1 : 0], c->stream));
-        else queue_pass(c, c->stream, eye);
+        int kind = kPhotonPass;
+        if (eye && c->ahead_pending) {
+            HIP_TRY(c, hipStreamWaitEvent(c->stream, c->ev_ahead, 0));
+            c->ahead_pending = false;
+            kind = kEyePassReady;
+        } else if (eye) {
+            kind = kEyePass;
+        }
+        if (graph) HIP_TRY(c, hipGraphLaunch(c->graph_pass[kind], c->stream));
+        else queue_pass(c, c->stream, kind);
+        if (eye && ahead) {
+            HIP_TRY(c, hipEventRecord(c->ev_fork, c->stream));
+            HIP_TRY(c, hipStreamWaitEvent(c->side, c->ev_fork, 0));
+            if (graph) HIP_TRY(c, hipGraphLaunch(c->graph_pass[kEyeAhead], c->side));
+            else queue_pass(c, c->side, kEyeAhead);
+            HIP_TRY(c, hipEventRecord(c->ev_ahead, c->side));
+            c->ahead_pending = true;
+        }
         c->frame++;
     }
     HIP_TRY(c, hipGetLastError());
@@ -973,7 +1012,7 @@ int ppm_pass(ppm_ctx* c, uint32_t npasses) {
 
 int ppm_sync(ppm_ctx* c) {
     if (!c) return PPM_E_INVAL;
-    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    HIP_TRY(c, sync_streams(c));
     return PPM_OK;
 }
 
@@ -1020,7 +1059,7 @@ int ppm_readback(ppm_ctx* c, ppm_buf which, void* dst, size_t bytes) {
     if (need == 0) return set_err(c, PPM_E_INVAL, "unknown buffer");
     if (bytes != need) return set_err(c, PPM_E_INVAL, "byte count does not match the buffer size");
     HIP_TRY(c, hipSetDevice(c->device));
-    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    HIP_TRY(c, sync_streams(c));
     const size_t Q = c->Q, P = c->P, H = c->H;
     const Job& J = c->job;
     auto d2h = [&](const void* src, size_t n) -> int {
@@ -1029,6 +1068,8 @@ int ppm_readback(ppm_ctx* c, ppm_buf which, void* dst, size_t bytes) {
     };
     DevGlobals g;
     HIP_TRY(c, hipMemcpy(&g, J.g, sizeof(g), hipMemcpyDeviceToHost));
+    // the hit points of the latest eye pass (before any: buffer 1, which holds the seeds)
+    const size_t cb = (size_t)((g.eye_samples + 1) & 1) * Q;
     switch (which) {
         case PPM_BUF_RADIANCE:
         case PPM_BUF_TONEMAPPED:
@@ -1037,23 +1078,23 @@ int ppm_readback(ppm_ctx* c, ppm_buf which, void* dst, size_t bytes) {
             hipLaunchKernelGGL(k_resolve, dim3((c->Q + 255) / 256), dim3(256), 0, c->stream, c->Q, J.hs, J.g,
                                c->resolve_lin, c->resolve_map);
             HIP_TRY(c, hipGetLastError());
-            HIP_TRY(c, hipStreamSynchronize(c->stream));
+            HIP_TRY(c, sync_streams(c));
             if (which == PPM_BUF_RADIANCE) return d2h(c->resolve_lin, Q * 16);
             if (which == PPM_BUF_TONEMAPPED) return d2h(c->resolve_map, Q * 16);
             return d2h(J.hs.rgba8, Q * 4);
         }
-        case PPM_BUF_HIT_POSITION: return d2h(J.hs.pos, Q * 16);
-        case PPM_BUF_HIT_REFLECTANCE: return d2h(J.hs.refl, Q * 16);
-        case PPM_BUF_HIT_NORMAL: return d2h(J.hs.nrm, Q * 16);
-        case PPM_BUF_EYE_RNG: return d2h(J.hs.rng, Q * 16);
+        case PPM_BUF_HIT_POSITION: return d2h(J.hs.pos + cb, Q * 16);
+        case PPM_BUF_HIT_REFLECTANCE: return d2h(J.hs.refl + cb, Q * 16);
+        case PPM_BUF_HIT_NORMAL: return d2h(J.hs.nrm + cb, Q * 16);
+        case PPM_BUF_EYE_RNG: return d2h(J.hs.rng + cb, Q * 16);
         case PPM_BUF_FLUX_RADIUS: return d2h(J.hs.tau_r, Q * 16);
         case PPM_BUF_EMISSION_COUNT: return d2h(J.hs.e_n, Q * 16);
         case PPM_BUF_HIT_TRIANGLE:
         case PPM_BUF_HIT_T: {
             std::vector<int> tri(Q);
             std::vector<float2> mt(Q);
-            HIP_TRY(c, hipMemcpy(tri.data(), J.hs.tri, Q * 4, hipMemcpyDeviceToHost));
-            HIP_TRY(c, hipMemcpy(mt.data(), J.hs.mat_t, Q * 8, hipMemcpyDeviceToHost));
+            HIP_TRY(c, hipMemcpy(tri.data(), J.hs.tri + cb, Q * 4, hipMemcpyDeviceToHost));
+            HIP_TRY(c, hipMemcpy(mt.data(), J.hs.mat_t + cb, Q * 8, hipMemcpyDeviceToHost));
             if (which == PPM_BUF_HIT_T) {
                 float* o = (float*)dst;
                 for (size_t i = 0; i < Q; i++) o[i] = mt[i].y;
@@ -1104,13 +1145,14 @@ int ppm_readback(ppm_ctx* c, ppm_buf which, void* dst, size_t bytes) {
         }
         case PPM_BUF_GRID: {
             ppm_grid k;
+            const GridScalars& gr = g.grid[(g.eye_samples + 1) & 1];
             for (int i = 0; i < 3; i++) {
-                k.bbmin[i] = g.bbmin[i];
-                k.bbmax[i] = g.bbmax[i];
+                k.bbmin[i] = gr.bbmin[i];
+                k.bbmax[i] = gr.bbmax[i];
             }
-            k.initial_radius = g.initial_radius;
-            k.grid_scale = g.grid_scale;
-            k.hash_scale = g.hash_scale;
+            k.initial_radius = gr.initial_radius;
+            k.grid_scale = gr.grid_scale;
+            k.hash_scale = gr.hash_scale;
             std::memcpy(dst, &k, sizeof(k));
             return PPM_OK;
         }
@@ -1120,7 +1162,7 @@ int ppm_readback(ppm_ctx* c, ppm_buf which, void* dst, size_t bytes) {
 
 int ppm_profile_enable(ppm_ctx* c, int enable) {
     if (!c) return PPM_E_INVAL;
-    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    HIP_TRY(c, sync_streams(c));
     for (auto& pe : c->ev_pending) {
         c->ev_pool.push_back(pe.second.first);
         c->ev_pool.push_back(pe.second.second);
@@ -1136,7 +1178,7 @@ int ppm_profile_enable(ppm_ctx* c, int enable) {
 
 int ppm_profile_read(ppm_ctx* c, double* ms, uint64_t* launches) {
     if (!c) return PPM_E_INVAL;
-    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    HIP_TRY(c, sync_streams(c));
     for (auto& pe : c->ev_pending) {
         float t = 0.0f;
         HIP_TRY(c, hipEventElapsedTime(&t, pe.second.first, pe.second.second));
@@ -1198,7 +1240,7 @@ int ppm_autotune(ppm_ctx* c, uint32_t passes_per_trial, ppm_tuning* best) {
 void ppm_destroy(ppm_ctx* c) {
     if (!c) return;
     (void)hipSetDevice(c->device);
-    if (c->stream) (void)hipStreamSynchronize(c->stream);
+    if (c->stream) (void)sync_streams(c);
     ppm_internal::release_context(c);
     delete c;
 }

@@ -217,9 +217,10 @@ typedef struct ppm_tuning {
                                   batch: passes per hipGraph, 0 for plain launches */
     int32_t walk_stack;        /* LDS stack entries per lane of the BVH walks, 0..32; deeper
                                   walks go on over the pre-order node records */
-    int32_t eye_overlap;       /* 1: an eye pass's eye stage runs on a second stream beside the
-                                  photon stage (default in a batch, and in the plain loop for
-                                  images of 2^18 pixels or more), 0: one stream */
+    int32_t eye_overlap;       /* 1: eye stages run on a second stream beside the photon stage
+                                  (default in a batch, and in the plain loop for images of 2^18
+                                  pixels or more; the plain loop starts each eye pass's stage
+                                  right after the eye pass before it), 0: one stream */
     float ms_per_pass;         /* ppm_autotune / ppm_batch_autotune: time of the chosen shape */
 } ppm_tuning;
 

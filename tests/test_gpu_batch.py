@@ -149,6 +149,29 @@ def test_launch_shapes_do_not_change_results(shape):
     assert not format_report(rep), format_report(rep)
 
 
+def test_eye_stage_ahead_across_calls_and_tuning_changes():
+    """With eye_overlap the plain loop starts each eye pass's eye stage right after the eye pass
+    before it: the stage stays pending across ppm_pass calls, readbacks and tuning changes (the
+    overlap switched off, graphs switched on, the tiles reshaped) and every state matches one
+    stream at every stop."""
+    m = spheres(5, 4, 2, 50, 37)
+    p = g.default_params(64, image_width=50, image_height=37, photon_width=40, photon_height=33, max_path_length=3)
+    a = g.Renderer(m, p)
+    b = g.Renderer(m, p)
+    b.set_tuning(eye_overlap=1)
+    done = 0
+    for n, tuning in [(1, None), (1, None), (2, None), (4, dict(passes_per_graph=1)), (1, dict(eye_overlap=0)),
+                      (3, dict(eye_overlap=1, tile_wave_width=8)), (2, dict(passes_per_graph=0)), (7, None)]:
+        print("passes %d..%d tuning %s" % (done, done + n, tuning), flush=True)
+        if tuning:
+            b.set_tuning(**tuning)
+        b.run(n)
+        a.run(n)
+        done += n
+        rep = compare_all(b, a, STATE_BUFFERS)
+        assert not format_report(rep), format_report(rep)
+
+
 def test_autotune_keeps_results_and_picks_a_valid_shape():
     m = spheres(9, 6, 2, 64, 48)
     p = g.default_params(64, image_height=48, max_path_length=5)

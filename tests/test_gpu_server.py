@@ -84,18 +84,25 @@ def test_admission_waits_for_memory_and_keeps_results():
     meshes = scenes()
     jobs = job_list()
     states = [g.state_footprint(p) for (_, _, p, _) in jobs]
-    scene_bytes = sum(g.scene_footprint(m, jobs[0][2]) - g.state_footprint(jobs[0][2]) for m in meshes)
-    # room for both scenes and two jobs at a time, not three
+    scenes_each = [g.scene_footprint(m, jobs[0][2]) - g.state_footprint(jobs[0][2]) for m in meshes]
+    scene_bytes = sum(scenes_each)
+    # room for both scenes and two jobs at a time, not three; a scene that no resident job renders
+    # may give its memory back, which leaves room for a few more jobs of the other scene
     budget = scene_bytes + 2 * max(states) + min(states) // 2
     assert scene_bytes + 3 * min(states) > budget
+    most = max(k for k in range(len(jobs) + 1) if min(scenes_each) + k * min(states) <= budget)
+    assert most < len(jobs)
     srv = g.Server(hbm_budget=budget, passes_per_step=2)
     sids = [srv.add_scene(m) for m in meshes]
     ids = [(srv.submit(sids[si], p, n, camera=g.make_camera(*cam)), si, cam, p, n) for (si, cam, p, n) in jobs]
     peak = 0
     while not all(srv.info(j)["state"] == "done" for (j, *_rest) in ids):
-        peak = max(peak, srv.stats()["resident"])
+        st = srv.stats()
+        assert st["device_bytes"] <= budget
+        peak = max(peak, st["resident"])
         time.sleep(0.002)
-    assert peak <= 2
+    print("budget %d, most resident %d, peak %d" % (budget, most, peak), flush=True)
+    assert 1 <= peak <= most
     for (jid, si, cam, p, n) in ids:
         check(srv, jid, meshes[si], cam, p, n)
     assert srv.stats()["device_bytes"] <= budget
