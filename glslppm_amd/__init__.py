@@ -144,6 +144,7 @@ BUFFERS = {
     "slot_rep": (18, np.int32, 1, "h"),
 }
 KERNELS = ("eye", "clear", "photon", "gather")
+PROFILE_KINDS = 0x100  # ppm_profile_enable: time only the kernel kinds whose bits are set
 BUF_COUNTERS = 19
 BUF_GRID = 20
 
@@ -426,9 +427,18 @@ class Renderer:
         self._check(library().ppm_autotune(self._h, int(passes_per_trial), ctypes.byref(t)))
         return t.as_dict()
 
-    def profile(self, enable=True):
-        """Starts (and clears) or stops per-kernel event timing."""
-        self._check(library().ppm_profile_enable(self._h, 1 if enable else 0))
+    def profile(self, enable=True, kinds=None):
+        """Starts (and clears) or stops per-kernel event timing; `kinds` (names from KERNELS)
+        limits the timing to those kernel kinds."""
+        if not enable:
+            arg = 0
+        elif kinds is None:
+            arg = 1
+        else:
+            arg = PROFILE_KINDS
+            for k in kinds:
+                arg |= 1 << KERNELS.index(k)
+        self._check(library().ppm_profile_enable(self._h, arg))
 
     def profile_read(self):
         """{kernel: (total ms, launches)} accumulated since profile(True)."""

@@ -105,6 +105,7 @@ struct ppm_ctx {
     size_t device_bytes = 0;
     // per-kernel event timing (ppm_profile_enable)
     bool profiling = false;
+    unsigned prof_kinds = 0;  // kernel kinds timed while profiling (bit per PPM_KERNEL_*)
     std::vector<hipEvent_t> ev_pool;
     std::vector<std::pair<int, std::pair<hipEvent_t, hipEvent_t>>> ev_pending;
     double prof_ms[PPM_KERNEL_KINDS] = {0, 0, 0, 0};

@@ -340,8 +340,8 @@ hipEvent_t pooled_event(ppm_ctx* c) {
 }
 
 // Records an event before a launch when profiling; returns it (or null).
-hipEvent_t prof_begin(ppm_ctx* c, hipStream_t st) {
-    if (!c->profiling) return nullptr;
+hipEvent_t prof_begin(ppm_ctx* c, int kind, hipStream_t st) {
+    if (!c->profiling || !(c->prof_kinds >> kind & 1u)) return nullptr;
     hipEvent_t e = pooled_event(c);
     if (e) (void)hipEventRecord(e, st);
     return e;
@@ -401,7 +401,7 @@ int rep_limit(const ppm_params& p) {
 // and the grid set-up.  `ahead` begins the next eye pass's stage instead of this pass's.
 void queue_eye_stage(ppm_ctx* c, hipStream_t es, int ahead) {
     const Job& J = c->job;
-    hipEvent_t e0 = prof_begin(c, es);
+    hipEvent_t e0 = prof_begin(c, PPM_KERNEL_EYE, es);
     hipLaunchKernelGGL(k_eye_begin, dim3(1), dim3(64), 0, es, J, ahead);
     hipLaunchKernelGGL(k_eye, dim3(J.tile_blocks), dim3(256), walk_lds_bytes(J.walk_stack, 256), es, J);
     hipLaunchKernelGGL(k_grid, dim3(1), dim3(1024), 0, es, J);
@@ -432,10 +432,10 @@ void queue_pass(ppm_ctx* c, hipStream_t st, int kind) {
     }
     if (stage) queue_eye_stage(c, es, 0);
     if (fork) (void)hipEventRecord(c->ev_join, es);
-    hipEvent_t e1 = prof_begin(c, st);
+    hipEvent_t e1 = prof_begin(c, PPM_KERNEL_CLEAR, st);
     hipLaunchKernelGGL(k_clear_slots, dim3(J.clear_blocks), dim3(256), 0, st, J);
     prof_end(c, PPM_KERNEL_CLEAR, e1, st);
-    hipEvent_t e2 = prof_begin(c, st);
+    hipEvent_t e2 = prof_begin(c, PPM_KERNEL_PHOTON, st);
     hipLaunchKernelGGL(k_photon_gen, dim3(J.photon_blocks), dim3(256), 0, st, J);
     hipLaunchKernelGGL(k_bucket_scan, dim3(1), dim3(1024), 0, st, J);
     hipLaunchKernelGGL(k_bucket_scatter, dim3(J.photon_blocks), dim3(256), 0, st, J);
@@ -443,7 +443,7 @@ void queue_pass(ppm_ctx* c, hipStream_t st, int kind) {
                        walk_lds_bytes(J.walk_stack, c->trace_block), st, J);
     prof_end(c, PPM_KERNEL_PHOTON, e2, st);
     if (fork) (void)hipStreamWaitEvent(st, c->ev_join, 0);
-    hipEvent_t e3 = prof_begin(c, st);
+    hipEvent_t e3 = prof_begin(c, PPM_KERNEL_GATHER, st);
     if (stage) hipLaunchKernelGGL(k_hash, dim3(J.photon_blocks), dim3(256), 0, st, J);
     hipLaunchKernelGGL(k_gather, dim3(J.tile_blocks), dim3(256), 0, st, J);
     prof_end(c, PPM_KERNEL_GATHER, e3, st);
@@ -1162,6 +1162,8 @@ int ppm_readback(ppm_ctx* c, ppm_buf which, void* dst, size_t bytes) {
 
 int ppm_profile_enable(ppm_ctx* c, int enable) {
     if (!c) return PPM_E_INVAL;
+    if (enable != 0 && enable != 1 && ((enable & ~(PPM_PROFILE_KINDS | 0xf)) != 0 || !(enable & PPM_PROFILE_KINDS)))
+        return set_err(c, PPM_E_INVAL, "profile: 0, 1 or PPM_PROFILE_KINDS with kernel kind bits");
     HIP_TRY(c, sync_streams(c));
     for (auto& pe : c->ev_pending) {
         c->ev_pool.push_back(pe.second.first);
@@ -1173,6 +1175,7 @@ int ppm_profile_enable(ppm_ctx* c, int enable) {
         c->prof_n[k] = 0;
     }
     c->profiling = enable != 0;
+    c->prof_kinds = enable == 1 ? 0xfu : (unsigned)enable & 0xfu;
     return PPM_OK;
 }
 

@@ -193,14 +193,17 @@ int ppm_sync(ppm_ctx* ctx);
 float ppm_last_pass_ms(ppm_ctx* ctx);
 
 /* Per-kernel device timing with HIP events recorded around each launch on the context stream.
- * ppm_profile_enable(ctx, 1) clears the accumulators and starts recording; ppm_profile_read waits
- * for the stream and returns the accumulated milliseconds and launch counts per kernel kind.
+ * ppm_profile_enable(ctx, 1) clears the accumulators and starts recording every kind;
+ * ppm_profile_enable(ctx, PPM_PROFILE_KINDS | 1 << kind | ...) records only those kinds (each event
+ * costs the stream a few microseconds, so time only what is needed); 0 stops.  ppm_profile_read
+ * waits for the streams and returns the accumulated milliseconds and launch counts per kernel kind.
  * No counterpart in the reference (it times whole frames with glutGet, glslppm.cpp:526-535). */
 #define PPM_KERNEL_EYE 0      /* eyeraytrace.fs + max/min */
 #define PPM_KERNEL_CLEAR 1    /* glClear of the hash and correction targets */
 #define PPM_KERNEL_PHOTON 2   /* photontrace.fs + sum + hash + scatter + correction */
 #define PPM_KERNEL_GATHER 3   /* progressive.fs + re.fs */
 #define PPM_KERNEL_KINDS 4
+#define PPM_PROFILE_KINDS 0x100  /* ppm_profile_enable: the low bits select kernel kinds */
 int ppm_profile_enable(ppm_ctx* ctx, int enable);
 int ppm_profile_read(ppm_ctx* ctx, double* ms, uint64_t* launches);
 

@@ -172,6 +172,34 @@ def test_eye_stage_ahead_across_calls_and_tuning_changes():
         assert not format_report(rep), format_report(rep)
 
 
+def test_profiling_times_the_chosen_kernel_kinds_and_keeps_results():
+    """ppm_profile_enable: 1 times every kind, PPM_PROFILE_KINDS | bits only the chosen ones; the
+    launch counts follow the pass cadence (L=3: eye passes 0, 3 and 6 in 7 passes, and 4 eye stages,
+    the last one started ahead of eye pass 9)."""
+    m = spheres(5, 4, 2, 50, 37)
+    p = g.default_params(64, image_width=50, image_height=37, photon_width=40, photon_height=33, max_path_length=3)
+    a = g.Renderer(m, p)
+    b = g.Renderer(m, p)
+    b.set_tuning(eye_overlap=1)
+    b.profile(True)
+    b.run(7)
+    every = b.profile_read()
+    assert [every[k][1] for k in g.KERNELS] == [4, 7, 7, 7], every
+    assert all(every[k][0] > 0.0 for k in g.KERNELS)
+    b.profile(True, kinds=["photon"])
+    b.run(5)
+    only = b.profile_read()
+    assert only["photon"][1] == 5 and only["photon"][0] > 0.0
+    assert all(only[k] == (0.0, 0) for k in ("eye", "clear", "gather")), only
+    b.profile(False)
+    a.run(12)
+    rep = compare_all(b, a, STATE_BUFFERS)
+    assert not format_report(rep), format_report(rep)
+    for bad in (2, 0x100 | 0x10, -1):
+        with pytest.raises(g.PPMError):
+            b._check(g.library().ppm_profile_enable(b._h, bad))
+
+
 def test_autotune_keeps_results_and_picks_a_valid_shape():
     m = spheres(9, 6, 2, 64, 48)
     p = g.default_params(64, image_height=48, max_path_length=5)
