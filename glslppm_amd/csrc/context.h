@@ -91,7 +91,8 @@ struct ppm_ctx {
     hipEvent_t ev_fork = nullptr, ev_join = nullptr;
     // the eye stage of the next eye pass, started on `side` after an eye pass (ppm_pass)
     hipEvent_t ev_ahead = nullptr;
-    bool ahead_pending = false;
+    bool ahead_due = false;      // an eye pass has passed and the next one's stage is not queued yet
+    bool ahead_pending = false;  // that stage is queued on `side`; the next eye pass waits for it
     ppm_internal::Arena* arena = nullptr;  // null: hipMalloc / hipFree
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     bool timed = false;

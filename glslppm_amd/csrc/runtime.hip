@@ -415,8 +415,11 @@ void queue_eye_stage(ppm_ctx* c, hipStream_t es, int ahead) {
 // the deep eye paths, chains of dependent walks that keep few lanes busy, then share the GPU with
 // the photon walk.  An eye pass whose stage ran ahead (kEyePassReady) deposits at once.
 // kEyeAhead is the stage of the next eye pass alone, for the side stream after an eye pass: the
-// passes in between read the other half of the double-buffered hit points.
-void queue_pass(ppm_ctx* c, hipStream_t st, int kind) {
+// passes in between read the other half of the double-buffered hit points.  `fork_ahead` starts
+// that stage from inside a photon pass instead, after the photon generator and the bucket sort:
+// k_eye's blocks live for most of a millisecond and, started first, would hold the CUs that the
+// generator's short blocks wait for.
+void queue_pass(ppm_ctx* c, hipStream_t st, int kind, bool fork_ahead = false) {
     if (kind == kEyeAhead) {
         queue_eye_stage(c, st, 1);
         return;
@@ -439,6 +442,12 @@ void queue_pass(ppm_ctx* c, hipStream_t st, int kind) {
     hipLaunchKernelGGL(k_photon_gen, dim3(J.photon_blocks), dim3(256), 0, st, J);
     hipLaunchKernelGGL(k_bucket_scan, dim3(1), dim3(1024), 0, st, J);
     hipLaunchKernelGGL(k_bucket_scatter, dim3(J.photon_blocks), dim3(256), 0, st, J);
+    if (fork_ahead) {
+        (void)hipEventRecord(c->ev_fork, st);
+        (void)hipStreamWaitEvent(c->side, c->ev_fork, 0);
+        queue_eye_stage(c, c->side, 1);
+        (void)hipEventRecord(c->ev_ahead, c->side);
+    }
     hipLaunchKernelGGL(k_photon_trace, dim3(J.trace_blocks), dim3(c->trace_block),
                        walk_lds_bytes(J.walk_stack, c->trace_block), st, J);
     prof_end(c, PPM_KERNEL_PHOTON, e2, st);
@@ -974,7 +983,9 @@ int ppm_pass(ppm_ctx* c, uint32_t npasses) {
     const uint64_t L = (uint64_t)c->prm.max_path_length;
     const bool graph = c->use_graph && !c->profiling;
     // after each eye pass the next eye pass's stage starts on `side`, beside the photon passes in
-    // between, and that eye pass waits for it; with one pass per eye pass there is nothing between
+    // between, and that eye pass waits for it; with one pass per eye pass there is nothing between.
+    // Launched stream by stream, it starts inside the first photon pass (queue_pass); a captured
+    // pass cannot fork mid-way, so with graphs it starts right after the eye pass.
     const bool ahead = c->side && c->eye_overlap && L >= 2;
     if (graph) {
         int rc;
@@ -982,8 +993,21 @@ int ppm_pass(ppm_ctx* c, uint32_t npasses) {
             if ((k != kEyeAhead || c->side) && (rc = capture_pass(c, k))) return rc;
     }
     HIP_TRY(c, hipEventRecord(c->ev0, c->stream));
+    auto launch_ahead = [&]() -> int {
+        HIP_TRY(c, hipEventRecord(c->ev_fork, c->stream));
+        HIP_TRY(c, hipStreamWaitEvent(c->side, c->ev_fork, 0));
+        if (graph) HIP_TRY(c, hipGraphLaunch(c->graph_pass[kEyeAhead], c->side));
+        else queue_pass(c, c->side, kEyeAhead);
+        HIP_TRY(c, hipEventRecord(c->ev_ahead, c->side));
+        c->ahead_due = false;
+        c->ahead_pending = true;
+        return PPM_OK;
+    };
     for (uint32_t it = 0; it < npasses; it++) {
         const bool eye = c->frame % L == 0;
+        int rc;
+        if (c->ahead_due && !ahead) c->ahead_due = false;  // the eye pass runs its own stage
+        if (c->ahead_due && (graph || eye) && (rc = launch_ahead())) return rc;
         int kind = kPhotonPass;
         if (eye && c->ahead_pending) {
             HIP_TRY(c, hipStreamWaitEvent(c->stream, c->ev_ahead, 0));
@@ -992,15 +1016,19 @@ int ppm_pass(ppm_ctx* c, uint32_t npasses) {
         } else if (eye) {
             kind = kEyePass;
         }
-        if (graph) HIP_TRY(c, hipGraphLaunch(c->graph_pass[kind], c->stream));
-        else queue_pass(c, c->stream, kind);
+        if (graph) {
+            HIP_TRY(c, hipGraphLaunch(c->graph_pass[kind], c->stream));
+        } else {
+            const bool fork = c->ahead_due && kind == kPhotonPass;
+            queue_pass(c, c->stream, kind, fork);
+            if (fork) {
+                c->ahead_due = false;
+                c->ahead_pending = true;
+            }
+        }
         if (eye && ahead) {
-            HIP_TRY(c, hipEventRecord(c->ev_fork, c->stream));
-            HIP_TRY(c, hipStreamWaitEvent(c->side, c->ev_fork, 0));
-            if (graph) HIP_TRY(c, hipGraphLaunch(c->graph_pass[kEyeAhead], c->side));
-            else queue_pass(c, c->side, kEyeAhead);
-            HIP_TRY(c, hipEventRecord(c->ev_ahead, c->side));
-            c->ahead_pending = true;
+            c->ahead_due = true;
+            if (graph && (rc = launch_ahead())) return rc;
         }
         c->frame++;
     }
