@@ -212,8 +212,8 @@ int check_tuning(const ppm_tuning* t, int* wave_shift);
 // Sets the launch shape of a context's job (invalidates its captured graphs).
 void set_launch_shape(ppm_ctx* c, int trace_block, int wave_shift, int walk_stack);
 // Coordinate search over the launch shapes: for each knob in turn, `trial(t)` applies t, runs
-// passes and returns their milliseconds; the fastest value of each knob is kept.  The winner is
-// applied last and written to *best.
+// passes and returns their milliseconds; a knob's value replaces the kept one only when it is 1%
+// faster.  The winner is applied last and written to *best.
 template <class Trial>
 int autotune(ppm_tuning start, bool batch, Trial trial, ppm_tuning* best) {
     ppm_tuning cur = start;
@@ -242,7 +242,8 @@ int autotune(ppm_tuning start, bool batch, Trial trial, ppm_tuning* best) {
                 (knob == 3 && t.walk_stack == base.walk_stack) || (knob == 4 && t.eye_overlap == base.eye_overlap))
                 continue;
             if ((rc = trial(t, &ms))) return rc;
-            if (ms < cur.ms_per_pass) {
+            // a shape must beat the kept one by 1%: trials of a few eye cadences vary by about that
+            if (ms < cur.ms_per_pass * 0.99f) {
                 cur = t;
                 cur.ms_per_pass = ms;
             }

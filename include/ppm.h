@@ -211,7 +211,7 @@ int ppm_profile_read(ppm_ctx* ctx, double* ms, uint64_t* launches);
  * shape gives bit-identical results; only the speed differs.  ppm_autotune runs real passes of
  * the context (they count as passes) under each candidate value of one knob at a time -- photon
  * walk block size, pixel tile shape, graph replay, walk stack depth, eye-stage overlap -- and keeps
- * the fastest;
+ * the fastest (a new value must beat the kept one by 1%);
  * call it before the passes whose speed matters. */
 typedef struct ppm_tuning {
     int32_t trace_block;       /* threads per block of the photon walk: 64, 128, ..., 512 */
