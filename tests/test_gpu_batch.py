@@ -172,6 +172,24 @@ def test_eye_stage_ahead_across_calls_and_tuning_changes():
         assert not format_report(rep), format_report(rep)
 
 
+@pytest.mark.parametrize("max_path,passes_per_graph", [(1, 0), (1, 1), (2, 0), (2, 1)])
+def test_eye_stage_ahead_at_the_shortest_cadences(max_path, passes_per_graph):
+    """L=2: one photon pass between eye passes, so the stage that starts inside it is waited for by
+    the very next pass; L=1: every pass is an eye pass and runs its own stage."""
+    print("L=%d passes_per_graph=%d" % (max_path, passes_per_graph), flush=True)
+    m = spheres(5, 4, 2, 50, 37)
+    p = g.default_params(64, image_width=50, image_height=37, photon_width=40, photon_height=33,
+                         max_path_length=max_path)
+    a = g.Renderer(m, p)
+    b = g.Renderer(m, p)
+    b.set_tuning(eye_overlap=1, passes_per_graph=passes_per_graph)
+    for n in (1, 2, 3, 5):
+        a.run(n)
+        b.run(n)
+        rep = compare_all(b, a, STATE_BUFFERS)
+        assert not format_report(rep), format_report(rep)
+
+
 def test_profiling_times_the_chosen_kernel_kinds_and_keeps_results():
     """ppm_profile_enable: 1 times every kind, PPM_PROFILE_KINDS | bits only the chosen ones; the
     launch counts follow the pass cadence (L=3: eye passes 0, 3 and 6 in 7 passes, and their 3 eye
