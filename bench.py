@@ -2,9 +2,11 @@
 
 A step is one pass (m_display, glslppm.cpp:214-536) over synthetic input: photon trace of every
 photon slot, hash build, gather and progressive update of every hit point, draw, and the eye
-trace every max_path_length passes.  The default workload is the ~200k-triangle scene: the Cornell
-room plus 40 jittered icospheres of subdivision 4 (50% diffuse, 30% mirror, 20% glass, IOR 1.5),
-1024x1024 eye rays, 1024x1024 photons per pass, 1024x1024 hash slots, 10 bounces, alpha 0.7.
+trace every max_path_length passes.  The default workload is the largest single-GPU one, the
+caustic-heavy terrain: a 1024x1024-cell value-noise heightfield plus 8 glass icospheres (2,260,994
+triangles), a small area light, 1920x1080 eye rays, 2048^2 photons per pass, 2^24 hash slots, 12
+bounces, alpha 0.7.  spheres-1024 (the Cornell room plus 40 icospheres, ~200k triangles, 1024^2
+everything, 10 bounces) and cornell-256 (36 triangles, 256^2, 8 bounces) are the smaller ones.
 
 Prints one JSON line.  value = photon segments per second (P x passes / s) with all state resident
 in HBM; eye rays/s, passes/s, emitted photons/s and the reference's "M paths/s" are reported beside
@@ -39,12 +41,13 @@ KERNEL_NAMES = {"photon": "photon stage (k_photon_gen + k_bucket_scan + k_bucket
 
 WORKLOADS = {
     "spheres-1024": dict(kind=1, seed=0x67736C70, count=40, detail=4, res=1024, L=10, camera=((0, 0, 1.667), (0, 0, 0), 45.0),
-                         desc="Cornell room + 40 icospheres (subdiv 4): Q=P=H=1024^2, 10 bounces, alpha 0.7"),
+                         cpu_passes=50, desc="Cornell room + 40 icospheres (subdiv 4): Q=P=H=1024^2, 10 bounces, alpha 0.7"),
     "cornell-256": dict(kind=0, seed=0, count=0, detail=0, res=256, L=8, camera=((0, 0, 1.667), (0, 0, 0), 45.0),
-                        fixed_radius=0.02,
+                        fixed_radius=0.02, cpu_passes=20,
                         desc="Cornell room, 36 triangles: Q=P=H=256^2, 8 bounces, radius 1/50 of the room"),
     "terrain-c2": dict(kind=2, seed=0x67736C72, count=8, detail=1024, res=2048, L=12,
                        camera=((0.0, 0.55, 1.9), (0.0, -0.25, 0.0), 50.0), image=(1920, 1080), hash=(4096, 4096),
+                       cpu_passes=12,
                        desc="1024^2-cell value-noise heightfield + 8 glass icospheres (subdiv 5), 2,260,994 "
                             "triangles, small area light: Q=1920x1080, P=2048^2, H=2^24, 12 bounces"),
 }
@@ -316,7 +319,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=50)
     ap.add_argument("--warmup", type=int, default=10)
-    ap.add_argument("--workload", default="spheres-1024",
+    ap.add_argument("--workload", default="terrain-c2",
                     choices=sorted(WORKLOADS) + sorted(BATCH_WORKLOADS) + ["serve"])
     ap.add_argument("--rate", type=float, default=20.0, help="serve: job arrivals per second")
     ap.add_argument("--serve-seconds", type=float, default=20.0, help="serve: length of the arrival window")
@@ -331,7 +334,9 @@ def main():
                     help="plain workloads: pick launch shapes with ppm_autotune during the warm-up")
     ap.add_argument("--tuning", default=None,
                     help='launch shapes to set first (plain and batch workloads), as JSON, e.g. \'{"walk_stack": 8}\'')
-    ap.add_argument("--cpu-passes", type=int, default=50, help="passes of the CPU baseline sample (0: skip)")
+    ap.add_argument("--cpu-passes", type=int, default=None,
+                    help="passes of the CPU baseline sample (0: skip; default: the workload's, about 10-20 s of "
+                         "oracle time on 16 cores)")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="first run a plain device copy of MIB MiB (torch), the known-size reference for "
                          "rocprofv3 FETCH_SIZE/WRITE_SIZE counters")
@@ -417,8 +422,9 @@ def main():
             traffic = None
 
     cpu = None
-    if args.cpu_passes > 0:
-        cpu = cpu_baseline(mesh, params, args.cpu_passes)
+    cpu_passes = w["cpu_passes"] if args.cpu_passes is None else args.cpu_passes
+    if cpu_passes > 0:
+        cpu = cpu_baseline(mesh, params, cpu_passes)
 
     line = {
         "metric": METRIC,

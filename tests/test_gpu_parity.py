@@ -106,25 +106,16 @@ def c2_terrain():
     return m, p
 
 
-def test_c2_terrain_full_size():
-    m, p = c2_terrain()
-    assert m.num_triangles == 2260994
-    # bit-exact against the oracle over the first two passes (one eye pass, two photon passes)
-    _check(m, p, 2, "c2-terrain")
-    # size-independent properties over a full eye cycle and two more passes, GPU only
-    r = g.Renderer(m, p)
-    r.run(12)
-    tr12 = r.readback("flux_radius").copy()
-    en12 = r.readback("emission_count").copy()
-    r.run(2)
+def check_hash_properties(r, p):
+    """Size-independent properties of the hash after a pass: per-slot counts are the histogram of
+    the photons' slots (correction), and the representative is the smallest eligible photon index
+    of its slot (scatter, with the depth-clip exclusion of the last row's right half)."""
     slot = r.readback("photon_slot")
     count = r.readback("slot_count")
     rep = r.readback("slot_rep")
     H = p.hash_width * p.hash_height
     valid = slot >= 0
-    # correction: per-slot counts are the histogram of the photons' slots
     assert np.array_equal(np.bincount(slot[valid], minlength=H).astype(np.uint32), count)
-    # scatter: the representative is the smallest eligible photon index of its slot
     P = p.photon_width * p.photon_height
     limit = P - p.photon_width // 2
     idx = np.nonzero(valid & (np.arange(P) < limit))[0]
@@ -134,13 +125,72 @@ def test_c2_terrain_full_size():
     first = np.r_[True, s_sorted[1:] != s_sorted[:-1]]
     best[s_sorted[first]] = i_sorted[first]
     assert np.array_equal(best.astype(np.int32), rep)
-    # progressive update: R never grows, N never shrinks, between passes 12 and 14 (no eye pass)
-    tr14, en14 = r.readback("flux_radius"), r.readback("emission_count")
-    hit = r.readback("hit_triangle")[:, 0] >= 0
-    assert np.all(tr14[hit, 3] <= tr12[hit, 3]) and np.all(en14[hit, 3] >= en12[hit, 3])
+
+
+def check_progress_properties(r, p, passes, checkpoints):
+    """Runs `passes` more passes in `checkpoints` steps: at each step the hash properties hold, R
+    never grows and N never shrinks at any hit point (eye passes move hit points but never reset
+    the statistics), and the counters advance by exactly P per pass."""
+    P = p.photon_width * p.photon_height
+    c0 = r.counters()
+    tr = r.readback("flux_radius").copy()
+    en = r.readback("emission_count").copy()
+    step = passes // checkpoints
+    for k in range(checkpoints):
+        r.run(step)
+        check_hash_properties(r, p)
+        tr1, en1 = r.readback("flux_radius").copy(), r.readback("emission_count").copy()
+        assert np.all(tr1[:, 3] <= tr[:, 3]) and np.all(en1[:, 3] >= en[:, 3])
+        assert np.all(np.isfinite(r.readback("radiance")))
+        tr, en = tr1, en1
     c = r.counters()
-    assert c["frame_count"] == 14 and c["eye_samples"] == 2 and c["photon_segments"] == 14 * P
-    assert 0 < c["photons_emitted"] <= 14 * P
+    done = step * checkpoints
+    assert c["frame_count"] == c0["frame_count"] + done
+    assert c["photon_segments"] == c0["photon_segments"] + done * P
+    assert c0["photons_emitted"] < c["photons_emitted"] <= c0["photons_emitted"] + done * P
+
+
+def c1_spheres():
+    """C1: the Cornell room plus 40 jittered icospheres of subdivision 4 (generator seed
+    0x67736C70; 50% diffuse, 30% mirror, 20% glass), 204,836 triangles, so the BVH comes from the
+    threaded build; 1024^2 eye rays, photons and hash slots, 10 bounces, the footprint radius rule.
+    This is the configuration bench.py's spheres-1024 workload times."""
+    m = g.Mesh.generate(g.SCENE_SPHERES, seed=0x67736C70, count=40, detail=4)
+    m.set_camera((0, 0, 1.667), (0, 0, 0), 1024, 1024, 45.0).build()
+    return m, g.default_params(1024, max_path_length=10)
+
+
+def test_c1_spheres_full_size():
+    m, p = c1_spheres()
+    assert m.num_triangles == 204836
+    # Bit-exact against the oracle over one eye cycle plus two passes: eye passes 0 and 10, and
+    # at 2^20 pixels the second eye stage runs ahead on the side stream beside photon passes 1-9
+    # (eye_overlap is on by default), so the double-buffered hit points are checked too.  Then
+    # both sides run on to pass 32 (two more eye passes) and are compared again.
+    gpu, cpu = _check(m, p, 12, "c1-spheres")
+    assert gpu.tuning()["eye_overlap"] == 1
+    gpu.run(20)
+    cpu.run(20)
+    rep = compare_all(gpu, cpu)
+    bad = format_report(rep)
+    assert not bad, "c1-spheres after 32 passes: mismatching elements: %s" % bad
+    cpu.close()
+    # size-independent properties through two more eye cycles on the GPU alone (passes 32-52)
+    check_progress_properties(gpu, p, 20, 4)
+    assert gpu.counters()["eye_samples"] == 6
+
+
+def test_c2_terrain_full_size():
+    m, p = c2_terrain()
+    assert m.num_triangles == 2260994
+    # bit-exact against the oracle over one eye cycle and one more pass: eye passes 0 and 12, the
+    # second of which ran ahead beside photon passes 1-11
+    gpu, cpu = _check(m, p, 13, "c2-terrain")
+    cpu.close()
+    # size-independent properties through the next eye cycle, GPU only (passes 13-25)
+    check_progress_properties(gpu, p, 12, 3)
+    c = gpu.counters()
+    assert c["frame_count"] == 25 and c["eye_samples"] == 3
 
 
 def test_textured_materials():
