@@ -69,6 +69,10 @@ def library():
         lib.oracle_fresnel.argtypes = [fp, fp, ctypes.c_float, ctypes.c_float]
         lib.oracle_photon_bounce.restype = ctypes.c_int
         lib.oracle_photon_bounce.argtypes = [vp, ctypes.c_int, fp, fp, fp, fp, fp, fp, fp]
+        lib.oracle_glossy_reflect.restype = None
+        lib.oracle_glossy_reflect.argtypes = [fp, fp, ctypes.c_float, fp, fp]
+        lib.oracle_eye_path.restype = ctypes.c_int
+        lib.oracle_eye_path.argtypes = [vp, vp, ctypes.c_int, ctypes.c_int, ctypes.c_float, ctypes.c_float, fp, fp]
         _lib = lib
     return _lib
 
@@ -150,6 +154,28 @@ def camera_ray(scene_ptr, params, i, j, aa_x, aa_y, rnd):
     library().oracle_camera_ray(ctypes.cast(scene_ptr, ctypes.c_void_p), ctypes.cast(ctypes.pointer(params), ctypes.c_void_p),
                                 int(i), int(j), float(aa_x), float(aa_y), _fptr(r), _fptr(o), _fptr(d), _fptr(f))
     return o, d, f[0], r
+
+
+def glossy_reflect(d, n, g, rnd):
+    """glossy_reflect (eyeraytrace.fs:370-392, photontrace.fs:258-280): (direction, advanced rnd)."""
+    r = np.array(rnd, np.float32)
+    out = np.zeros(3, np.float32)
+    library().oracle_glossy_reflect(_fptr(np.asarray(d, np.float32)), _fptr(np.asarray(n, np.float32)),
+                                    float(np.float32(g)), _fptr(r), _fptr(out))
+    return out, r
+
+
+def eye_path(scene_ptr, params, i, j, aa_x, aa_y, rnd):
+    """One eye path of pixel (i, j) (eyeraytrace.fs:396-547): a dict of the last segment's triangle
+    (-1: miss), the hit point's position, colour, normal, the last ray direction, the emission
+    before averaging, material and t; and the advanced rnd."""
+    r = np.array(rnd, np.float32)
+    out = np.zeros(17, np.float32)
+    tri = library().oracle_eye_path(ctypes.cast(scene_ptr, ctypes.c_void_p),
+                                    ctypes.cast(ctypes.pointer(params), ctypes.c_void_p), int(i), int(j),
+                                    float(aa_x), float(aa_y), _fptr(r), _fptr(out))
+    return dict(tri=int(tri), pos=out[0:3], col=out[3:6], nrm=out[6:9], dir=out[9:12], emission=out[12:15],
+                mat=out[15], t=out[16]), r
 
 
 def light_sample(scene_ptr, rnd):

@@ -548,19 +548,27 @@ static void camera_ray(const ppm_scene* s, const ppm_params* prm, int i, int j, 
     *org = lens;
 }
 
-static void eye_fragment(Oracle* o, int i, int j, float aax, float aay, int num_eye_samples) {
-    const ppm_scene* s = o->s;
-    const int px = j * o->Qw + i;
-    v4 rndv = o->qrng[px];
+/* One eye path (eyeraytrace.fs:396-547): the camera ray, then up to max_path_length segments
+ * through the specular BRDFs; the outputs are what main() writes to the hit-point textures
+ * (eyeraytrace.fs:548-553) before the emission average. */
+typedef struct {
+    v3 pos, col, nrm, dir, emi;
+    int tri;
+    float mat, t;
+} EyePath;
+
+static void eye_path(const ppm_scene* s, const ppm_params* prm, int i, int j, float aax, float aay, v4* rng,
+                     EyePath* out) {
+    v4 rndv = *rng;
     v3 rorg, rdir;
     float filter;
-    camera_ray(s, &o->prm, i, j, aax, aay, &rndv, &rorg, &rdir, &filter);
+    camera_ray(s, prm, i, j, aax, aay, &rndv, &rorg, &rdir, &filter);
     v3 col = vscale(V3(1.0f, 1.0f, 1.0f), filter);
     v3 nrm = V3(0, 0, 0), pos = V3(0, 0, 0), emi = V3(0, 0, 0);
     const float eps = 1e-5f;
     int last_tri = -1;
     float last_mat = -1.0f, last_t = 1.0e20f;
-    for (int k = 0; k < o->prm.max_path_length; k++) {
+    for (int k = 0; k < prm->max_path_length; k++) {
         Isect I = raytrace(s, rorg, rdir, k == 0, 0);
         last_tri = I.isect_tri; last_mat = I.isect_mat; last_t = I.isect_t;
         if (I.t == 1.0e30f) {
@@ -628,18 +636,34 @@ static void eye_fragment(Oracle* o, int i, int j, float aax, float aay, int num_
             }
         }
     }
+    *rng = rndv;
+    out->pos = pos;
+    out->col = col;
+    out->nrm = nrm;
+    out->dir = rdir;
+    out->emi = emi;
+    out->tri = last_tri;
+    out->mat = last_tri < 0 ? -1.0f : last_mat;
+    out->t = last_t;
+}
+
+static void eye_fragment(Oracle* o, int i, int j, float aax, float aay, int num_eye_samples) {
+    const int px = j * o->Qw + i;
+    v4 rndv = o->qrng[px];
+    EyePath e;
+    eye_path(o->s, &o->prm, i, j, aax, aay, &rndv, &e);
     v4 en = o->qemi_n[px];
     float a = 1.0f / (float)num_eye_samples;
-    en.x = en.x * (1.0f - a) + emi.x * a;
-    en.y = en.y * (1.0f - a) + emi.y * a;
-    en.z = en.z * (1.0f - a) + emi.z * a;
-    o->qpos[px] = V4(pos.x, pos.y, pos.z, rdir.x);
-    o->qrefl[px] = V4(col.x, col.y, col.z, rdir.y);
-    o->qnrm[px] = V4(nrm.x, nrm.y, nrm.z, rdir.z);
+    en.x = en.x * (1.0f - a) + e.emi.x * a;
+    en.y = en.y * (1.0f - a) + e.emi.y * a;
+    en.z = en.z * (1.0f - a) + e.emi.z * a;
+    o->qpos[px] = V4(e.pos.x, e.pos.y, e.pos.z, e.dir.x);
+    o->qrefl[px] = V4(e.col.x, e.col.y, e.col.z, e.dir.y);
+    o->qnrm[px] = V4(e.nrm.x, e.nrm.y, e.nrm.z, e.dir.z);
     o->qrng[px] = rndv;
-    o->qtri[px] = last_tri;
-    o->qmat[px] = last_tri < 0 ? -1.0f : last_mat;
-    o->qt[px] = last_t;
+    o->qtri[px] = e.tri;
+    o->qmat[px] = e.mat;
+    o->qt[px] = e.t;
     o->qemi_n[px] = en;
 }
 
@@ -816,8 +840,8 @@ static int photon_bounce(const ppm_scene* s, int tri, v3 ipos, v3 pnrm, v3 pd, v
 }
 
 /* Exported for the oracle's own tests: the camera ray, the light and IBL emission samples, the
- * Fresnel term and the photon bounce above, on caller-held generator states (rnd: 4 floats, read
- * and advanced). */
+ * glossy perturbation, one eye path, the Fresnel term and the photon bounce above, on caller-held
+ * generator states (rnd: 4 floats, read and advanced). */
 void oracle_camera_ray(const ppm_scene* s, const ppm_params* p, int i, int j, float aax, float aay, float* rnd,
                        float* org, float* dir, float* filter) {
     v4 r = V4(rnd[0], rnd[1], rnd[2], rnd[3]);
@@ -845,6 +869,28 @@ void oracle_ibl_sample(const ppm_scene* s, float* rnd, float* flux, float* dir, 
     flux[0] = f.x; flux[1] = f.y; flux[2] = f.z;
     dir[0] = d.x; dir[1] = d.y; dir[2] = d.z;
     org[0] = o.x; org[1] = o.y; org[2] = o.z;
+}
+void oracle_glossy_reflect(const float* d, const float* n, float g, float* rnd, float* out) {
+    v4 r = V4(rnd[0], rnd[1], rnd[2], rnd[3]);
+    v3 res = glossy_reflect(V3(d[0], d[1], d[2]), V3(n[0], n[1], n[2]), g, &r);
+    rnd[0] = r.x; rnd[1] = r.y; rnd[2] = r.z; rnd[3] = r.w;
+    out[0] = res.x; out[1] = res.y; out[2] = res.z;
+}
+/* out: pos xyz, col xyz, nrm xyz, dir xyz, emission xyz, material, t (17 floats); returns the
+ * triangle of the last segment, -1 for a miss. */
+int oracle_eye_path(const ppm_scene* s, const ppm_params* p, int i, int j, float aax, float aay, float* rnd,
+                    float* out) {
+    v4 r = V4(rnd[0], rnd[1], rnd[2], rnd[3]);
+    EyePath e;
+    eye_path(s, p, i, j, aax, aay, &r, &e);
+    rnd[0] = r.x; rnd[1] = r.y; rnd[2] = r.z; rnd[3] = r.w;
+    const v3 v[5] = {e.pos, e.col, e.nrm, e.dir, e.emi};
+    for (int k = 0; k < 5; k++) {
+        out[3 * k] = v[k].x; out[3 * k + 1] = v[k].y; out[3 * k + 2] = v[k].z;
+    }
+    out[15] = e.mat;
+    out[16] = e.t;
+    return e.tri;
 }
 float oracle_fresnel(const float* incom, const float* normal, float ni, float ne) {
     return fresnel(V3(incom[0], incom[1], incom[2]), V3(normal[0], normal[1], normal[2]), ni, ne);
