@@ -36,7 +36,7 @@ HBM_PEAK_GBS = 8000.0
 #   eye ray (eye stage, every L passes): 104 B
 #   hash slot (k_clear_slots): clear plus read of count and representative, 16 B
 BYTES_PER = {"photon": 216.0, "gather": 120.0, "eye": 104.0, "clear": 16.0}
-KERNEL_NAMES = {"photon": "photon stage (k_photon_gen + k_bucket_scan + k_bucket_scatter + k_photon_trace)",
+KERNEL_NAMES = {"photon": "photon stage (k_photon_stage + k_photon_sum)",
                 "gather": "k_gather", "eye": "eye stage (k_eye + k_grid)", "clear": "k_clear_slots"}
 
 WORKLOADS = {

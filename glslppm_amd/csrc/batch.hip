@@ -16,6 +16,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cstdlib>
 #include <cstring>
 #include <new>
 #include <string>
@@ -91,14 +92,9 @@ void queue_batch_pass(ppm_batch* b, hipStream_t st) {
     if (b->eye_overlap) (void)hipEventRecord(b->ev_join, es);
     hipLaunchKernelGGL(k_clear_slots_batch, dim3(b->n_blocks[kMapClear]), dim3(256), 0, st, J,
                        (const BlockRef*)b->d_map[kMapClear]);
-    hipLaunchKernelGGL(k_photon_gen_batch, dim3(b->n_blocks[kMapPhoton]), dim3(256), 0, st, J,
-                       (const BlockRef*)b->d_map[kMapPhoton]);
-    hipLaunchKernelGGL(k_bucket_scan_batch, dim3(n), dim3(1024), 0, st, J);
-    hipLaunchKernelGGL(k_bucket_scatter_batch, dim3(b->n_blocks[kMapPhoton]), dim3(256), 0, st, J,
-                       (const BlockRef*)b->d_map[kMapPhoton]);
-    hipLaunchKernelGGL(k_photon_trace_batch, dim3(b->n_blocks[kMapTrace]), dim3(b->trace_block),
-                       walk_lds_bytes(b->walk_stack, b->trace_block), st, J,
-                       (const BlockRef*)b->d_map[kMapTrace]);
+    hipLaunchKernelGGL(k_photon_stage_batch, dim3(b->n_blocks[kMapTrace]), dim3(b->trace_block),
+                       walk_lds_bytes(b->walk_stack, b->trace_block), st, J, (const BlockRef*)b->d_map[kMapTrace]);
+    hipLaunchKernelGGL(k_photon_sum_batch, dim3(n), dim3(1024), 0, st, J);
     if (b->eye_overlap) (void)hipStreamWaitEvent(st, b->ev_join, 0);
     hipLaunchKernelGGL(k_hash_batch, dim3(b->n_blocks[kMapPhoton]), dim3(256), 0, st, J,
                        (const BlockRef*)b->d_map[kMapPhoton]);

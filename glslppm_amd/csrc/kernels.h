@@ -88,7 +88,7 @@ struct DevGlobals {
     unsigned long long eye_samples;  // eye passes begun; the latest one's buffers are (eye_samples - 1) & 1
     float num_photons_f;
     unsigned int emitted_this_pass;
-    unsigned int queue_len;          // continuing photons of this pass (written by k_bucket_scan)
+    unsigned int pad0;
     unsigned int xs[4];              // XORShift state after the seed grids (AAOffset draws)
     unsigned int passes_left;        // passes this job still runs; 0xffffffff: no limit
     int active;                      // this pass advances the job
@@ -136,19 +136,7 @@ struct PhotonState {
     float4* rng;
     int2* state;
     int* slot;
-    unsigned int* partial;  // [blocks] emitted counts
-};
-
-// Rays of the photons that continue this pass, compacted in slot order: each block of 256 slots
-// counts its continuing rays (ranks from wave ballots), one scan turns the counts into first
-// positions, and every ray is placed at its block's position plus its rank.
-struct RayQueue {
-    float4* org;   // [P] by photon slot: origin, flux.x
-    float4* dir;   // [P] direction, flux.y
-    int4* aux;     // [P] flux.z bits, bounce | emitted << 16, photon slot, 0
-    int* rank;             // [P] rank of a continuing ray among its block's, -1 when the photon ended
-    int* perm;             // [rays] photon slots of the continuing rays
-    unsigned int* block;   // [photon blocks] continuing rays per block, then (after the scan) first positions
+    unsigned int* partial;  // [k_photon_stage blocks] emitted counts
 };
 
 struct HashState {
@@ -199,7 +187,7 @@ constexpr int kTraceBlockDefault = 64;
 // of LDS); deeper walks continue on the pre-order records.
 constexpr int kWalkStack = 16;
 constexpr int kWalkStackMax = 32;
-// Dynamic LDS of a walk launch (k_eye, k_photon_trace): the lanes' stacks.
+// Dynamic LDS of a walk launch (k_eye, k_photon_stage): the lanes' stacks.
 inline size_t walk_lds_bytes(int walk_stack, int block) { return (size_t)walk_stack * (size_t)block * 4; }
 
 // Everything the kernels need about one job (one scene, camera and state set).  A plain pass loop
@@ -214,12 +202,11 @@ struct Job {
     HitState hs;
     PhotonState ps;
     HashState h;
-    RayQueue rq;
     DevGlobals* g;
     int tile_blocks;     // pixel tiles (k_eye, k_gather)
-    int photon_blocks;   // 256 photon slots per block (k_photon_gen, k_bucket_scatter)
+    int photon_blocks;   // 256 photon slots per block (k_hash)
     int clear_blocks;    // 256 hash slots or ray buckets per block (k_clear_slots)
-    int trace_blocks;    // rays per block = the launch's block size (k_photon_trace)
+    int trace_blocks;    // photon slots per block = the launch's block size (k_photon_stage)
     int walk_stack;      // stack entries per lane of the walks (LDS: walk_stack x block x 4 bytes)
 };
 
@@ -231,18 +218,16 @@ struct BlockRef {
 
 // Kernels of one job: grid = that job's blocks.  The _batch forms take the job array and, for
 // the kernels with several blocks per job, the block map of that kernel kind; k_pass_begin_batch
-// runs one thread per job, k_grid_batch and k_bucket_scan_batch one block per job.  k_photon_trace
+// runs one thread per job, k_grid_batch and k_photon_sum_batch one block per job.  k_photon_stage
 // runs with any block size up to kTraceBlockMax (a multiple of 64); the others with 256 (1024
-// for k_grid and k_bucket_scan).
+// for k_grid and k_photon_sum).
 __global__ void k_pass_begin(Job J);
 __global__ void k_clear_slots(Job J);
 __global__ void k_eye_begin(Job J, int ahead);
 __global__ void k_eye(Job J);
 __global__ void k_grid(Job J);
-__global__ void k_photon_gen(Job J);
-__global__ void k_bucket_scan(Job J);
-__global__ void k_bucket_scatter(Job J);
-__global__ void k_photon_trace(Job J);
+__global__ void k_photon_stage(Job J);
+__global__ void k_photon_sum(Job J);
 __global__ void k_hash(Job J);
 __global__ void k_gather(Job J);
 __global__ void k_pass_begin_batch(const Job* jobs, int n_jobs);
@@ -250,10 +235,8 @@ __global__ void k_clear_slots_batch(const Job* jobs, const BlockRef* map);
 __global__ void k_eye_begin_batch(const Job* jobs, int n_jobs);
 __global__ void k_eye_batch(const Job* jobs, const BlockRef* map);
 __global__ void k_grid_batch(const Job* jobs);
-__global__ void k_photon_gen_batch(const Job* jobs, const BlockRef* map);
-__global__ void k_bucket_scan_batch(const Job* jobs);
-__global__ void k_bucket_scatter_batch(const Job* jobs, const BlockRef* map);
-__global__ void k_photon_trace_batch(const Job* jobs, const BlockRef* map);
+__global__ void k_photon_stage_batch(const Job* jobs, const BlockRef* map);
+__global__ void k_photon_sum_batch(const Job* jobs);
 __global__ void k_hash_batch(const Job* jobs, const BlockRef* map);
 __global__ void k_gather_batch(const Job* jobs, const BlockRef* map);
 __global__ void k_batch_status(const Job* jobs, int n_jobs, unsigned long long* out);

@@ -3,11 +3,9 @@
 //   k_eye      every max_path passes: eyeraytrace.fs + per-block max.fs/min.fs partials
 //   k_grid     the hit-point box and the grid uniforms of glslppm.cpp:292-311 (one block)
 //   k_clear_slots
-//   k_photon_gen    photontrace.fs emission/bounce; continuing rays ranked within their block
-//   k_bucket_scan   per-block ray offsets + sum.fs and NumPhotons (one block)
-//   k_bucket_scatter  compaction of the continuing rays
-//   k_photon_trace  photontrace.fs walk + deposit + hash.fs + scatter (atomic min of the photon
-//                   index) + correction (atomic count), one thread per continuing ray
+//   k_photon_stage  photontrace.fs emission/bounce, walk and deposit + hash.fs + scatter (atomic
+//                   min of the photon index) + correction (atomic count), one thread per slot
+//   k_photon_sum    sum.fs and NumPhotons (one block)
 //   k_gather   progressive.fs (8-cell gather and the N/R/tau update) + re.fs draw to RGBA8
 // Arithmetic follows the GLSL statement by statement in strict fp32 (see fmath.h).
 //
@@ -930,17 +928,17 @@ PPM_DEV void ibl_sample(const DevScene& s, float4& rndv, f3& flux, f3& dir, f3& 
 
 }  // namespace
 
-// photontrace.fs:464-720, hash.fs and the scatter/correction draws, as four kernels.
+// photontrace.fs:464-720, hash.fs and the scatter/correction draws, as one kernel.
 //
-// k_photon_gen runs one thread per photon slot: emission or bounce with Russian roulette
-// (photontrace.fs:466-686).  A terminated photon writes its final state at once; a continuing
-// one writes its ray and its rank among its block's continuing rays.  k_bucket_scan turns the
-// per-block counts into first positions, k_bucket_scatter compacts the rays in slot order, and
-// k_photon_trace runs one thread per continuing ray: the walk, then the deposit
-// (photontrace.fs:690-710), hash.fs, scatter and correction.  (Sorting the rays by direction and
-// origin cell instead made the postponed-leaf walk slower: its waves then reach their leaves in
-// lock step.)  Each photon's arithmetic is independent of which lane runs it and the slot table
-// only sees commutative atomics, so the result equals one thread per slot.
+// k_photon_stage runs one thread per photon slot: emission or bounce with Russian roulette
+// (photontrace.fs:466-686); a terminated photon writes its final state at once, a continuing one
+// goes on through the walk and the deposit (photontrace.fs:690-710), hash.fs, scatter and
+// correction.  k_photon_sum adds up the blocks' emitted flags (sum.fs).  The slot table only sees
+// commutative atomics, so the result equals the reference's one fragment per photon texel.
+// (An earlier form compacted the continuing rays into a queue for a separate walk kernel; the
+// queue's 56 bytes per photon written and read again, and two more launches, cost more than the
+// idle lanes of the photons that end: 1.95 -> 2.02 G photon segments/s on the 205k-triangle
+// spheres, 2.61 -> 2.81 G on the 2.26M-triangle terrain.)
 
 namespace {
 
@@ -1084,7 +1082,6 @@ struct Walk {
     Hit h;
 };
 
-// Deposit (photontrace.fs:690-710), hash.fs and the scatter/correction updates of slot p.
 // The slot of a photon that deposits at hpos (hash.fs), and the scatter and correction updates of
 // that slot; -1 for a slot past the last row, which lands outside the hash viewport and is clipped.
 PPM_DEV int deposit(const PhotonParams& pp, const HashState& h, const DevGlobals* g, int p, f3 hpos) {
@@ -1141,135 +1138,66 @@ PPM_DEV void hash_deferred(const Job& J, int blk) {
     J.ps.slot[p] = deposit(J.pp, J.h, g, p, xyz(J.ps.pos[p]));
 }
 
-PPM_DEV unsigned long long lanes_below(int lane) { return lane == 0 ? 0ull : (~0ull >> (64 - lane)); }
-
-PPM_DEV void photon_gen_blk(const Job& J, int blk) {
-    if (!J.g->active) return;
+// The photon stage in one launch: each thread takes one photon slot through emission or bounce
+// (photon_gen), then a continuing photon through its walk and deposit (photon_finish).  No ray
+// queue: a photon ended by Russian roulette (about a sixth of them per pass) leaves its lane idle
+// during the walk, which costs little, since the walk's time follows the lanes' node fetches.
+PPM_DEV void photon_stage(const Job& J, int blk) {
+    const DevGlobals* g = J.g;
+    if (!g->active) return;
     const DevScene& s = J.s;
     const PhotonParams& pp = J.pp;
-    const PhotonState& ps = J.ps;
-    const RayQueue& rq = J.rq;
-    const int p = blk * kBlock + threadIdx.x;
+    const int p = blk * (int)blockDim.x + (int)threadIdx.x;
     unsigned int emitted = 0;
     bool live = false;
     f3 org, dir, flux;
     int out_bits = 0;
-    if (p < pp.count) live = photon_gen(s, pp, ps, p, org, dir, flux, out_bits, emitted);
-    if (live) {
-        rq.org[p] = make_float4(org.x, org.y, org.z, flux.x);
-        rq.dir[p] = make_float4(dir.x, dir.y, dir.z, flux.y);
-        rq.aux[p] = make_int4(__float_as_int(flux.z), out_bits, p, 0);
-    }
-    // rank of each continuing ray in its block, and per-block counts of continuing rays and of
-    // emitted flags (sum.fs), added up by k_bucket_scan
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    const unsigned long long m = __ballot(live);
+    if (p < pp.count) live = photon_gen(s, pp, J.ps, p, org, dir, flux, out_bits, emitted);
+    // sum.fs: this block's emitted flags, added up by k_photon_sum
     const unsigned int es = wave_sum(emitted);
-    __shared__ unsigned int wl[kWaves], we[kWaves];
-    if (lane == 0) {
-        wl[wave] = (unsigned int)__popcll(m);
-        we[wave] = es;
-    }
+    __shared__ unsigned int we[kTraceBlockMax / 64];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    if (lane == 0) we[wave] = es;
     __syncthreads();
-    unsigned int before = 0;
-    for (int w = 0; w < wave; w++) before += wl[w];
-    if (p < pp.count) rq.rank[p] = live ? (int)(before + (unsigned int)__popcll(m & lanes_below(lane))) : -1;
     if (threadIdx.x == 0) {
-        unsigned int nl = 0, ne = 0;
-        for (int w = 0; w < kWaves; w++) {
-            nl += wl[w];
-            ne += we[w];
-        }
-        rq.block[blk] = nl;
-        ps.partial[blk] = ne;
+        unsigned int ne = 0;
+        for (int w = 0; w < (int)(blockDim.x >> 6); w++) ne += we[w];
+        J.ps.partial[blk] = ne;
     }
-}
-
-// Exclusive scan of the per-block ray counts into first positions, the number of continuing
-// rays, and NumPhotons += int(sum of emitted flags) (glslppm.cpp:396-397); one block of 1024
-// threads, each owning a contiguous run of blocks.
-PPM_DEV void bucket_scan(const Job& J) {
-    DevGlobals* g = J.g;
-    if (!g->active) return;
-    const RayQueue& rq = J.rq;
-    const unsigned int* partial = J.ps.partial;
-    const int n = J.photon_blocks;
-    const int per = (n + 1023) / 1024;
-    __shared__ unsigned int wsum[16];
-    __shared__ unsigned int esum[16];
-    const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
-    const int lo = t * per, hi = min(n, lo + per);
-    unsigned int local = 0, e = 0;
-    for (int b = lo; b < hi; b++) {
-        local += rq.block[b];
-        e += partial[b];
-    }
-    unsigned int inc = local;
-    for (int o = 1; o < 64; o <<= 1) {
-        const unsigned int y = __shfl_up(inc, o, 64);
-        if (lane >= o) inc += y;
-    }
-    e = wave_sum(e);
-    if (lane == 63) wsum[wave] = inc;
-    if (lane == 0) esum[wave] = e;
-    __syncthreads();
-    unsigned int before = 0;
-    for (int w = 0; w < wave; w++) before += wsum[w];
-    unsigned int run = before + inc - local;
-    for (int b = lo; b < hi; b++) {
-        const unsigned int c = rq.block[b];
-        rq.block[b] = run;
-        run += c;
-    }
-    if (t == 0) {
-        unsigned int total = 0, emitted = 0;
-        for (int w = 0; w < 16; w++) {
-            total += wsum[w];
-            emitted += esum[w];
-        }
-        g->queue_len = total;
-        g->num_photons_f = g->num_photons_f + (float)(int)emitted;
-        g->photons_emitted += emitted;
-        g->emitted_this_pass = emitted;
-    }
-}
-
-// Places every continuing ray at its block's first position plus its rank.
-PPM_DEV void bucket_scatter(const Job& J, int blk) {
-    if (!J.g->active) return;
-    const RayQueue& rq = J.rq;
-    const int p = blk * kBlock + threadIdx.x;
-    if (p >= J.pp.count) return;
-    const int r = rq.rank[p];
-    if (r >= 0) rq.perm[rq.block[blk] + (unsigned int)r] = p;
-}
-
-PPM_DEV void photon_trace(const Job& J, int blk) {
-    const DevGlobals* g = J.g;
-    if (!g->active) return;
-    const unsigned int n_rays = g->queue_len;
-    const unsigned int t = blk * blockDim.x + threadIdx.x;
-    if (t >= n_rays) return;
-    const DevScene& s = J.s;
-    const RayQueue& rq = J.rq;
-    const int p = rq.perm[t];
-    const float* po = reinterpret_cast<const float*>(rq.org + p);
-    const float* pd = reinterpret_cast<const float*>(rq.dir + p);
-    const f3 o = mk3(po[0], po[1], po[2]), d = mk3(pd[0], pd[1], pd[2]);
+    if (!live) return;
     Walk W;
-    W.org = o;
-    W.dir = d;
-    W.rcp = mk3(1.0f / d.x, 1.0f / d.y, 1.0f / d.z);
-    W.t0 = mk3(-W.rcp.x * o.x, -W.rcp.y * o.y, -W.rcp.z * o.z);
-    W.face = cube_face(W.dir);
+    W.org = org;
+    W.dir = dir;
+    W.rcp = mk3(1.0f / dir.x, 1.0f / dir.y, 1.0f / dir.z);
+    W.t0 = mk3(-W.rcp.x * org.x, -W.rcp.y * org.y, -W.rcp.z * org.z);
+    W.face = cube_face(dir);
     W.h.tri = -1;
     W.h.t = 1.0e30f;
     W.h.bx = W.h.by = W.h.bz = 0.0f;
     walk<true>(s, W.face, W.org, W.dir, W.rcp, W.t0, false, W.h, J.walk_stack);
-    // the ray's flux and bookkeeping are read again rather than held in registers through the walk
-    const float fx = po[3], fy = pd[3];
-    const int4 y = rq.aux[p];
-    photon_finish(s, J.pp, J.ps, J.h, g, y.z, W, mk3(fx, fy, __int_as_float(y.x)), y.y);
+    photon_finish(s, pp, J.ps, J.h, g, p, W, flux, out_bits);
+}
+
+// NumPhotons += int(sum of emitted flags) (glslppm.cpp:396-397) from k_photon_stage's per-block
+// counts; one block of 1024 threads.
+PPM_DEV void photon_sum(const Job& J) {
+    DevGlobals* g = J.g;
+    if (!g->active) return;
+    const int n = J.trace_blocks;
+    unsigned int e = 0;
+    for (int b = threadIdx.x; b < n; b += 1024) e += J.ps.partial[b];
+    e = wave_sum(e);
+    __shared__ unsigned int esum[16];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    if (lane == 0) esum[wave] = e;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        unsigned int emitted = 0;
+        for (int w = 0; w < 16; w++) emitted += esum[w];
+        g->num_photons_f = g->num_photons_f + (float)(int)emitted;
+        g->photons_emitted += emitted;
+        g->emitted_this_pass = emitted;
+    }
 }
 
 // re.fs: tone map of one channel
@@ -1421,10 +1349,8 @@ __global__ void k_eye_begin_batch(const Job* __restrict__ jobs, int n_jobs) {
     if (j < n_jobs) eye_begin(jobs[j], 0);
 }
 PPM_JOB_KERNEL1(k_grid, grid, 1024)
-PPM_JOB_KERNEL(k_photon_gen, photon_gen_blk, kBlock)
-PPM_JOB_KERNEL1(k_bucket_scan, bucket_scan, 1024)
-PPM_JOB_KERNEL(k_bucket_scatter, bucket_scatter, kBlock)
-PPM_JOB_KERNEL_WAVES(k_photon_trace, photon_trace, kTraceBlockMax, 7)
+PPM_JOB_KERNEL_WAVES(k_photon_stage, photon_stage, kTraceBlockMax, 7)
+PPM_JOB_KERNEL1(k_photon_sum, photon_sum, 1024)
 PPM_JOB_KERNEL(k_hash, hash_deferred, kBlock)
 PPM_JOB_KERNEL(k_gather, gather, kBlock)
 

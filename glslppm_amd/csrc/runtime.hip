@@ -12,6 +12,7 @@
 #include <array>
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <new>
 #include <string>
@@ -416,9 +417,7 @@ void queue_eye_stage(ppm_ctx* c, hipStream_t es, int ahead) {
 // the photon walk.  An eye pass whose stage ran ahead (kEyePassReady) deposits at once.
 // kEyeAhead is the stage of the next eye pass alone, for the side stream after an eye pass: the
 // passes in between read the other half of the double-buffered hit points.  `fork_ahead` starts
-// that stage from inside a photon pass instead, after the photon generator and the bucket sort:
-// k_eye's blocks live for most of a millisecond and, started first, would hold the CUs that the
-// generator's short blocks wait for.
+// that stage from inside a photon pass instead, after the slot clear, beside the photon stage.
 void queue_pass(ppm_ctx* c, hipStream_t st, int kind, bool fork_ahead = false) {
     if (kind == kEyeAhead) {
         queue_eye_stage(c, st, 1);
@@ -439,17 +438,15 @@ void queue_pass(ppm_ctx* c, hipStream_t st, int kind, bool fork_ahead = false) {
     hipLaunchKernelGGL(k_clear_slots, dim3(J.clear_blocks), dim3(256), 0, st, J);
     prof_end(c, PPM_KERNEL_CLEAR, e1, st);
     hipEvent_t e2 = prof_begin(c, PPM_KERNEL_PHOTON, st);
-    hipLaunchKernelGGL(k_photon_gen, dim3(J.photon_blocks), dim3(256), 0, st, J);
-    hipLaunchKernelGGL(k_bucket_scan, dim3(1), dim3(1024), 0, st, J);
-    hipLaunchKernelGGL(k_bucket_scatter, dim3(J.photon_blocks), dim3(256), 0, st, J);
     if (fork_ahead) {
         (void)hipEventRecord(c->ev_fork, st);
         (void)hipStreamWaitEvent(c->side, c->ev_fork, 0);
         queue_eye_stage(c, c->side, 1);
         (void)hipEventRecord(c->ev_ahead, c->side);
     }
-    hipLaunchKernelGGL(k_photon_trace, dim3(J.trace_blocks), dim3(c->trace_block),
+    hipLaunchKernelGGL(k_photon_stage, dim3(J.trace_blocks), dim3(c->trace_block),
                        walk_lds_bytes(J.walk_stack, c->trace_block), st, J);
+    hipLaunchKernelGGL(k_photon_sum, dim3(1), dim3(1024), 0, st, J);
     prof_end(c, PPM_KERNEL_PHOTON, e2, st);
     if (fork) (void)hipStreamWaitEvent(st, c->ev_join, 0);
     hipEvent_t e3 = prof_begin(c, PPM_KERNEL_GATHER, st);
@@ -625,9 +622,7 @@ size_t state_footprint(const ppm_params* p) {
         Q * 32, Q * 32, Q * 32, Q * 32, Q * 32, Q * 16, Q * 16, Q * 8, Q * 16, Q * 4, (size_t)max_tile_blocks(p) * 24,
         (size_t)max_tile_blocks(p) * 4, (size_t)max_tile_blocks(p) * 4,  // tile costs and order
         // photons: pos, flux, dir, rng, state, slot, emitted partials
-        P * 16, P * 16, P * 16, P * 16, P * 8, P * 4, (size_t)photon_blocks(p) * 4,
-        // rays: org, dir, aux, rank, perm, per-block counts
-        P * 16, P * 16, P * 16, P * 4, P * 4, (size_t)photon_blocks(p) * 4,
+        P * 16, P * 16, P * 16, P * 16, P * 8, P * 4, (P + 63) / 64 * 4,
         // hash (count, representative) records; globals; resolve targets
         H * 8, sizeof(DevGlobals), Q * 16, Q * 16};
     size_t tot = 0;
@@ -834,7 +829,6 @@ int create_context(ppm_ctx* c, std::shared_ptr<SceneDev> scene, const ppm_camera
     const size_t Q = c->Q, P = c->P, H = c->H;
     HitState& hs = J.hs;
     PhotonState& ps = J.ps;
-    RayQueue& rq = J.rq;
     hs.q = (int)Q;
     if ((rc = dev_alloc(c, &hs.pos, 2 * Q)) || (rc = dev_alloc(c, &hs.refl, 2 * Q)) ||
         (rc = dev_alloc(c, &hs.nrm, 2 * Q)) || (rc = dev_alloc(c, &hs.rng, 2 * Q)) || (rc = dev_alloc(c, &hs.em, 2 * Q)) ||
@@ -845,9 +839,7 @@ int create_context(ppm_ctx* c, std::shared_ptr<SceneDev> scene, const ppm_camera
         (rc = dev_alloc(c, &hs.tile_order, (size_t)max_tile_blocks(p))) || (rc = dev_alloc(c, &ps.pos, P)) ||
         (rc = dev_alloc(c, &ps.flux, P)) || (rc = dev_alloc(c, &ps.dir, P)) || (rc = dev_alloc(c, &ps.rng, P)) ||
         (rc = dev_alloc(c, &ps.state, P)) || (rc = dev_alloc(c, &ps.slot, P)) ||
-        (rc = dev_alloc(c, &ps.partial, (size_t)J.photon_blocks)) || (rc = dev_alloc(c, &rq.org, P)) ||
-        (rc = dev_alloc(c, &rq.dir, P)) || (rc = dev_alloc(c, &rq.aux, P)) || (rc = dev_alloc(c, &rq.rank, P)) ||
-        (rc = dev_alloc(c, &rq.perm, P)) || (rc = dev_alloc(c, &rq.block, (size_t)J.photon_blocks)) ||
+        (rc = dev_alloc(c, &ps.partial, (size_t)(P + 63) / 64)) ||
         (rc = dev_alloc(c, &J.h.slots, 2 * H)) || (rc = dev_alloc(c, &J.g, 1)) ||
         (rc = dev_alloc(c, &c->resolve_lin, Q)) || (rc = dev_alloc(c, &c->resolve_map, Q)))
         return rc;

@@ -1,12 +1,15 @@
-"""Batched pass loop (ppm_batch_*) and graph replay against the plain loop, bit for bit.
+"""Batched pass loop (ppm_batch_*) and graph replay against the plain loop and the oracle, bit
+for bit.
 
 A job advanced inside a batch -- next to other scenes, resolutions, hash sizes, bounce caps and
 pass budgets, added late or removed early -- must end with exactly the buffers of the same scene
-and parameters run alone through ppm_create / ppm_pass for the same number of its passes.
+and parameters run alone through ppm_create / ppm_pass for the same number of its passes, and
+with the oracle's buffers for the same passes.
 """
 import pytest
 
 import glslppm_amd as g
+import oracle
 from parity import STATE_BUFFERS, compare_all, format_report
 
 pytestmark = pytest.mark.gpu
@@ -57,7 +60,9 @@ def job_specs():
     ]
 
 
-def check_equal_alone(view, mesh, params, passes, label):
+def check_equal_alone(view, mesh, params, passes, label, against_oracle=True):
+    """The batch job against the same job alone in the plain loop and, unless told otherwise,
+    against the oracle."""
     print("compare %s after %d passes" % (label, passes), flush=True)
     alone = g.Renderer(mesh, params)
     alone.run(passes)
@@ -65,6 +70,14 @@ def check_equal_alone(view, mesh, params, passes, label):
     bad = format_report(rep)
     assert not bad, "%s: batch job differs from the job alone: %s" % (label, bad)
     assert view.counters()["frame_count"] == passes
+    alone.close()
+    if against_oracle:
+        cpu = oracle.OracleRenderer(mesh, params)
+        cpu.run(passes)
+        rep = compare_all(view, cpu, STATE_BUFFERS)
+        bad = format_report(rep)
+        assert not bad, "%s: batch job differs from the oracle: %s" % (label, bad)
+        cpu.close()
 
 
 @pytest.mark.parametrize("passes_per_graph,eye_overlap", [(1, 0), (4, 0), (0, 0), (4, 1), (0, 1)])

@@ -10,7 +10,7 @@ import numpy as np
 import pytest
 
 import glslppm_amd as g
-from parity import compare_all, format_report, run_pair
+from parity import check_hash_properties, compare_all, format_report, run_pair
 
 pytestmark = pytest.mark.gpu
 
@@ -104,27 +104,6 @@ def c2_terrain():
     p = g.default_params(2048, image_width=1920, image_height=1080, hash_width=4096, hash_height=4096,
                          max_path_length=12)
     return m, p
-
-
-def check_hash_properties(r, p):
-    """Size-independent properties of the hash after a pass: per-slot counts are the histogram of
-    the photons' slots (correction), and the representative is the smallest eligible photon index
-    of its slot (scatter, with the depth-clip exclusion of the last row's right half)."""
-    slot = r.readback("photon_slot")
-    count = r.readback("slot_count")
-    rep = r.readback("slot_rep")
-    H = p.hash_width * p.hash_height
-    valid = slot >= 0
-    assert np.array_equal(np.bincount(slot[valid], minlength=H).astype(np.uint32), count)
-    P = p.photon_width * p.photon_height
-    limit = P - p.photon_width // 2
-    idx = np.nonzero(valid & (np.arange(P) < limit))[0]
-    best = np.full(H, -1, np.int64)
-    order = np.argsort(slot[idx], kind="stable")
-    s_sorted, i_sorted = slot[idx][order], idx[order]
-    first = np.r_[True, s_sorted[1:] != s_sorted[:-1]]
-    best[s_sorted[first]] = i_sorted[first]
-    assert np.array_equal(best.astype(np.int32), rep)
 
 
 def check_progress_properties(r, p, passes, checkpoints):

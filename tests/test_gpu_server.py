@@ -1,7 +1,8 @@
 """Serving loop (ppm_server_*): jobs arriving while others run give the results they give alone.
 
 Each job's radiance, (tau, R), (E, N), RGBA8 and counters after its target passes are compared,
-bit for bit, with a Renderer of the same scene, camera and parameters run alone.
+bit for bit, with a Renderer of the same scene, camera and parameters run alone, and with the
+oracle run for the same passes.
 """
 import threading
 import time
@@ -10,6 +11,7 @@ import numpy as np
 import pytest
 
 import glslppm_amd as g
+import oracle
 
 pytestmark = pytest.mark.gpu
 
@@ -38,17 +40,22 @@ def job_list():
 
 
 def check(server, jid, mesh, cam, params, passes):
+    """A served job's results against the job alone in the plain loop and against the oracle."""
     o, at, w, h, fov = cam
     mesh.set_camera(o, at, w, h, fov)
     r = g.Renderer(mesh, params)
     r.run(passes)
-    for n in g.SERVER_BUFFERS:
-        a, b = server.result(jid, n), r.readback(n)
-        bad = int(np.any(a.view(np.uint8).reshape(a.shape[0], -1) != b.view(np.uint8).reshape(b.shape[0], -1),
-                         axis=1).sum())
-        assert bad == 0, "job %d: %s differs in %d elements" % (jid, n, bad)
-    assert server.counters(jid) == r.counters()
+    cpu = oracle.OracleRenderer(mesh, params)
+    cpu.run(passes)
+    for ref, label in ((r, "the job alone"), (cpu, "the oracle")):
+        for n in g.SERVER_BUFFERS:
+            a, b = server.result(jid, n), ref.readback(n)
+            bad = int(np.any(a.view(np.uint8).reshape(a.shape[0], -1) != b.view(np.uint8).reshape(b.shape[0], -1),
+                             axis=1).sum())
+            assert bad == 0, "job %d: %s differs from %s in %d elements" % (jid, n, label, bad)
+        assert server.counters(jid) == ref.counters()
     r.close()
+    cpu.close()
 
 
 def test_jobs_in_the_mix_equal_jobs_alone():
