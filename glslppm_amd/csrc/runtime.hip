@@ -24,6 +24,7 @@ namespace {
 
 thread_local std::string g_create_error;
 
+
 int set_err(ppm_ctx* c, int code, const std::string& msg) {
     if (c) c->err = msg;
     else g_create_error = msg;
@@ -354,9 +355,10 @@ void prof_end(ppm_ctx* c, int kind, hipEvent_t start, hipStream_t st) {
     if (!e) return;
     (void)hipEventRecord(e, st);
     c->ev_pending.push_back({kind, {start, e}});
-    // bound the number of outstanding events
+    // bound the number of outstanding events: wait for both streams (pairs of the eye stage are on
+    // the second one), then fold every pair in
     if (c->ev_pending.size() >= 4096) {
-        (void)hipEventSynchronize(e);
+        (void)sync_streams(c);
         for (auto& pe : c->ev_pending) {
             float ms = 0.0f;
             if (hipEventElapsedTime(&ms, pe.second.first, pe.second.second) == hipSuccess) {
@@ -400,8 +402,7 @@ int rep_limit(const ppm_params& p) {
 
 // An eye stage of job J on stream es: AAOffset and NumEyeSamples (k_eye_begin), the eye trace
 // and the grid set-up.  `ahead` begins the next eye pass's stage instead of this pass's.
-void queue_eye_stage(ppm_ctx* c, hipStream_t es, int ahead) {
-    const Job& J = c->job;
+void queue_eye_stage(ppm_ctx* c, const Job& J, hipStream_t es, int ahead) {
     hipEvent_t e0 = prof_begin(c, PPM_KERNEL_EYE, es);
     hipLaunchKernelGGL(k_eye_begin, dim3(1), dim3(64), 0, es, J, ahead);
     hipLaunchKernelGGL(k_eye, dim3(J.tile_blocks), dim3(256), walk_lds_bytes(J.walk_stack, 256), es, J);
@@ -418,12 +419,11 @@ void queue_eye_stage(ppm_ctx* c, hipStream_t es, int ahead) {
 // kEyeAhead is the stage of the next eye pass alone, for the side stream after an eye pass: the
 // passes in between read the other half of the double-buffered hit points.  `fork_ahead` starts
 // that stage from inside a photon pass instead, after the slot clear, beside the photon stage.
-void queue_pass(ppm_ctx* c, hipStream_t st, int kind, bool fork_ahead = false) {
+void queue_pass(ppm_ctx* c, const Job& J, hipStream_t st, int kind, bool fork_ahead = false) {
     if (kind == kEyeAhead) {
-        queue_eye_stage(c, st, 1);
+        queue_eye_stage(c, J, st, 1);
         return;
     }
-    const Job& J = c->job;
     const bool stage = kind == kEyePass;
     hipLaunchKernelGGL(k_pass_begin, dim3(1), dim3(64), 0, st, J);
     const bool fork = stage && c->side && c->eye_overlap;
@@ -432,7 +432,7 @@ void queue_pass(ppm_ctx* c, hipStream_t st, int kind, bool fork_ahead = false) {
         (void)hipEventRecord(c->ev_fork, st);
         (void)hipStreamWaitEvent(es, c->ev_fork, 0);
     }
-    if (stage) queue_eye_stage(c, es, 0);
+    if (stage) queue_eye_stage(c, J, es, 0);
     if (fork) (void)hipEventRecord(c->ev_join, es);
     hipEvent_t e1 = prof_begin(c, PPM_KERNEL_CLEAR, st);
     hipLaunchKernelGGL(k_clear_slots, dim3(J.clear_blocks), dim3(256), 0, st, J);
@@ -441,7 +441,7 @@ void queue_pass(ppm_ctx* c, hipStream_t st, int kind, bool fork_ahead = false) {
     if (fork_ahead) {
         (void)hipEventRecord(c->ev_fork, st);
         (void)hipStreamWaitEvent(c->side, c->ev_fork, 0);
-        queue_eye_stage(c, c->side, 1);
+        queue_eye_stage(c, J, c->side, 1);
         (void)hipEventRecord(c->ev_ahead, c->side);
     }
     hipLaunchKernelGGL(k_photon_stage, dim3(J.trace_blocks), dim3(c->trace_block),
@@ -463,7 +463,7 @@ int capture_pass(ppm_ctx* c, int kind) {
     HIP_TRY(c, hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
     const bool prof = c->profiling;
     c->profiling = false;
-    queue_pass(c, st, kind);
+    queue_pass(c, c->job, st, kind);
     c->profiling = prof;
     HIP_TRY(c, hipStreamEndCapture(st, &g));
     const hipError_t e = hipGraphInstantiate(&c->graph_pass[kind], g, nullptr, nullptr, 0);
@@ -471,6 +471,68 @@ int capture_pass(ppm_ctx* c, int kind) {
     HIP_TRY(c, e);
     return PPM_OK;
 }
+
+// The pass loop of ppm_pass.  J is the job record the launches take; graphs are captured with the
+// context's own record, so they are replayed only when allow_graph says J is that record.
+int run_passes(ppm_ctx* c, const Job& J, uint32_t npasses, bool allow_graph) {
+    const uint64_t L = (uint64_t)c->prm.max_path_length;
+    const bool graph = allow_graph && c->use_graph && !c->profiling;
+    // after each eye pass the next eye pass's stage starts on `side`, beside the photon passes in
+    // between, and that eye pass waits for it; with one pass per eye pass there is nothing between.
+    // Launched stream by stream, it starts inside the first photon pass (queue_pass); a captured
+    // pass cannot fork mid-way, so with graphs it starts right after the eye pass.
+    const bool ahead = c->side && c->eye_overlap && L >= 2;
+    if (graph) {
+        int rc;
+        for (int k = 0; k < kPassKinds; k++)
+            if ((k != kEyeAhead || c->side) && (rc = capture_pass(c, k))) return rc;
+    }
+    HIP_TRY(c, hipEventRecord(c->ev0, c->stream));
+    auto launch_ahead = [&]() -> int {
+        HIP_TRY(c, hipEventRecord(c->ev_fork, c->stream));
+        HIP_TRY(c, hipStreamWaitEvent(c->side, c->ev_fork, 0));
+        if (graph) HIP_TRY(c, hipGraphLaunch(c->graph_pass[kEyeAhead], c->side));
+        else queue_pass(c, J, c->side, kEyeAhead);
+        HIP_TRY(c, hipEventRecord(c->ev_ahead, c->side));
+        c->ahead_due = false;
+        c->ahead_pending = true;
+        return PPM_OK;
+    };
+    for (uint32_t it = 0; it < npasses; it++) {
+        const bool eye = c->frame % L == 0;
+        int rc;
+        if (c->ahead_due && !ahead) c->ahead_due = false;  // the eye pass runs its own stage
+        if (c->ahead_due && (graph || eye) && (rc = launch_ahead())) return rc;
+        int kind = kPhotonPass;
+        if (eye && c->ahead_pending) {
+            HIP_TRY(c, hipStreamWaitEvent(c->stream, c->ev_ahead, 0));
+            c->ahead_pending = false;
+            kind = kEyePassReady;
+        } else if (eye) {
+            kind = kEyePass;
+        }
+        if (graph) {
+            HIP_TRY(c, hipGraphLaunch(c->graph_pass[kind], c->stream));
+        } else {
+            const bool fork = c->ahead_due && kind == kPhotonPass;
+            queue_pass(c, J, c->stream, kind, fork);
+            if (fork) {
+                c->ahead_due = false;
+                c->ahead_pending = true;
+            }
+        }
+        if (eye && ahead) {
+            c->ahead_due = true;
+            if (graph && (rc = launch_ahead())) return rc;
+        }
+        c->frame++;
+    }
+    HIP_TRY(c, hipGetLastError());
+    HIP_TRY(c, hipEventRecord(c->ev1, c->stream));
+    c->timed = true;
+    return PPM_OK;
+}
+
 
 }  // namespace
 
@@ -972,62 +1034,7 @@ int ppm_pass(ppm_ctx* c, uint32_t npasses) {
     if (!c->own_stream) return set_err(c, PPM_E_INVAL, "this context belongs to a batch; use ppm_batch_pass");
     HIP_TRY(c, hipSetDevice(c->device));
     if (npasses == 0) return PPM_OK;
-    const uint64_t L = (uint64_t)c->prm.max_path_length;
-    const bool graph = c->use_graph && !c->profiling;
-    // after each eye pass the next eye pass's stage starts on `side`, beside the photon passes in
-    // between, and that eye pass waits for it; with one pass per eye pass there is nothing between.
-    // Launched stream by stream, it starts inside the first photon pass (queue_pass); a captured
-    // pass cannot fork mid-way, so with graphs it starts right after the eye pass.
-    const bool ahead = c->side && c->eye_overlap && L >= 2;
-    if (graph) {
-        int rc;
-        for (int k = 0; k < kPassKinds; k++)
-            if ((k != kEyeAhead || c->side) && (rc = capture_pass(c, k))) return rc;
-    }
-    HIP_TRY(c, hipEventRecord(c->ev0, c->stream));
-    auto launch_ahead = [&]() -> int {
-        HIP_TRY(c, hipEventRecord(c->ev_fork, c->stream));
-        HIP_TRY(c, hipStreamWaitEvent(c->side, c->ev_fork, 0));
-        if (graph) HIP_TRY(c, hipGraphLaunch(c->graph_pass[kEyeAhead], c->side));
-        else queue_pass(c, c->side, kEyeAhead);
-        HIP_TRY(c, hipEventRecord(c->ev_ahead, c->side));
-        c->ahead_due = false;
-        c->ahead_pending = true;
-        return PPM_OK;
-    };
-    for (uint32_t it = 0; it < npasses; it++) {
-        const bool eye = c->frame % L == 0;
-        int rc;
-        if (c->ahead_due && !ahead) c->ahead_due = false;  // the eye pass runs its own stage
-        if (c->ahead_due && (graph || eye) && (rc = launch_ahead())) return rc;
-        int kind = kPhotonPass;
-        if (eye && c->ahead_pending) {
-            HIP_TRY(c, hipStreamWaitEvent(c->stream, c->ev_ahead, 0));
-            c->ahead_pending = false;
-            kind = kEyePassReady;
-        } else if (eye) {
-            kind = kEyePass;
-        }
-        if (graph) {
-            HIP_TRY(c, hipGraphLaunch(c->graph_pass[kind], c->stream));
-        } else {
-            const bool fork = c->ahead_due && kind == kPhotonPass;
-            queue_pass(c, c->stream, kind, fork);
-            if (fork) {
-                c->ahead_due = false;
-                c->ahead_pending = true;
-            }
-        }
-        if (eye && ahead) {
-            c->ahead_due = true;
-            if (graph && (rc = launch_ahead())) return rc;
-        }
-        c->frame++;
-    }
-    HIP_TRY(c, hipGetLastError());
-    HIP_TRY(c, hipEventRecord(c->ev1, c->stream));
-    c->timed = true;
-    return PPM_OK;
+    return run_passes(c, c->job, npasses, true);
 }
 
 int ppm_sync(ppm_ctx* c) {
@@ -1252,6 +1259,11 @@ int ppm_autotune(ppm_ctx* c, uint32_t passes_per_trial, ppm_tuning* best) {
         if ((rc = ppm_set_tuning(c, &t))) return rc;
         if ((rc = ppm_pass(c, 1))) return rc;  // capture and first-launch costs stay out of the timing
         if ((rc = ppm_pass(c, passes_per_trial))) return rc;
+        // an eye stage started ahead inside the trial belongs to it: the timed region ends after it
+        if (c->ahead_pending) {
+            HIP_TRY(c, hipStreamWaitEvent(c->stream, c->ev_ahead, 0));
+            HIP_TRY(c, hipEventRecord(c->ev1, c->stream));
+        }
         const float v = ppm_last_pass_ms(c);
         if (v < 0.0f) return set_err(c, PPM_E_HIP, "trial timing failed");
         *ms = v / (float)passes_per_trial;
