@@ -1183,9 +1183,17 @@ PPM_DEV void photon_stage(const Job& J, int blk) {
 PPM_DEV void photon_sum(const Job& J) {
     DevGlobals* g = J.g;
     if (!g->active) return;
-    const int n = J.trace_blocks;
+    // four counts per 16-byte load, eight loads in flight per thread (2^22 photons in one-wave
+    // blocks leave 65,536 counts: a load per count took 28 us, one after another)
+    const int n = J.trace_blocks, n4 = n >> 2;
+    const uint4* __restrict__ p4 = reinterpret_cast<const uint4*>(J.ps.partial);
     unsigned int e = 0;
-    for (int b = threadIdx.x; b < n; b += 1024) e += J.ps.partial[b];
+#pragma unroll 8
+    for (int b = threadIdx.x; b < n4; b += 1024) {
+        const uint4 v = p4[b];
+        e += (v.x + v.y) + (v.z + v.w);
+    }
+    for (int b = 4 * n4 + threadIdx.x; b < n; b += 1024) e += J.ps.partial[b];
     e = wave_sum(e);
     __shared__ unsigned int esum[16];
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
