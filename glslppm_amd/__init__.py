@@ -164,6 +164,7 @@ def library():
         "ppm_params_default": (None, [ctypes.POINTER(Params), ctypes.c_int32]),
         "ppm_create": (ctypes.c_int, [ctypes.POINTER(Scene), ctypes.POINTER(Params), ctypes.POINTER(vp)]),
         "ppm_pass": (ctypes.c_int, [vp, ctypes.c_uint32]),
+        "ppm_pass_static": (ctypes.c_int, [vp, ctypes.c_uint32]),
         "ppm_readback": (ctypes.c_int, [vp, ctypes.c_int, vp, ctypes.c_size_t]),
         "ppm_buffer_bytes": (ctypes.c_size_t, [vp, ctypes.c_int]),
         "ppm_sync": (ctypes.c_int, [vp]),
@@ -401,6 +402,12 @@ class Renderer:
     def run(self, npasses=1):
         """Queues npasses passes (m_display) on the GPU stream; returns immediately."""
         self._check(library().ppm_pass(self._h, int(npasses)))
+        return self
+
+    def run_static(self, npasses=1):
+        """ppm_pass_static: the same passes with their partial results in the library's static
+        device scratch (no allocation, no graphs); calls on one GPU must not overlap."""
+        self._check(library().ppm_pass_static(self._h, int(npasses)))
         return self
 
     def sync(self):

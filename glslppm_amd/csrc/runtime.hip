@@ -14,16 +14,42 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <mutex>
 #include <new>
 #include <string>
 #include <vector>
 
 #include "context.h"
 
+// Scratch of ppm_pass_static: the photon stage's per-block emitted counts, then the eye kernel's
+// per-block min/max partials.  Each GPU holds its own copy for the life of the process, and every
+// ppm_pass_static call on that GPU uses it.  Host code uses the array only through the device
+// address taken once per device at set-up (static_scratch_address), never by its name.
+constexpr size_t kStaticScratchBytes = 1u << 20;
+static __device__ __align__(256) unsigned char scratch[kStaticScratchBytes];
+
 namespace {
 
 thread_local std::string g_create_error;
 
+// Device addresses of `scratch`, one per device, taken by the first context created there.
+constexpr int kScratchDevices = 64;
+std::mutex g_scratch_mu;
+bool g_scratch_taken[kScratchDevices] = {};
+void* g_scratch_ptr[kScratchDevices] = {};
+hipError_t g_scratch_err[kScratchDevices] = {};
+
+// With `device` current: takes the address of this device's copy of `scratch` once.
+void static_scratch_address(int device) {
+    if (device < 0 || device >= kScratchDevices) return;
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (g_scratch_taken[device]) return;
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    g_scratch_err[device] = e;
+    g_scratch_ptr[device] = e == hipSuccess ? p : nullptr;
+    g_scratch_taken[device] = true;
+}
 
 int set_err(ppm_ctx* c, int code, const std::string& msg) {
     if (c) c->err = msg;
@@ -38,6 +64,18 @@ int set_err(ppm_ctx* c, int code, const std::string& msg) {
             return set_err((ctx), e_ == hipErrorOutOfMemory ? PPM_E_OOM : PPM_E_HIP,         \
                            std::string(#call) + ": " + hipGetErrorString(e_));               \
     } while (0)
+
+// The address static_scratch_address took for the context's device.
+int static_scratch(ppm_ctx* c, void** out) {
+    const int d = c->device;
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (d < 0 || d >= kScratchDevices || !g_scratch_taken[d])
+        return set_err(c, PPM_E_INVAL, "ppm_pass_static: no static scratch on device " + std::to_string(d));
+    if (!g_scratch_ptr[d])
+        return set_err(c, PPM_E_HIP, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(g_scratch_err[d]));
+    *out = g_scratch_ptr[d];
+    return PPM_OK;
+}
 
 template <typename T>
 int dev_alloc(ppm_ctx* c, T** out, size_t count) {
@@ -472,8 +510,9 @@ int capture_pass(ppm_ctx* c, int kind) {
     return PPM_OK;
 }
 
-// The pass loop of ppm_pass.  J is the job record the launches take; graphs are captured with the
-// context's own record, so they are replayed only when allow_graph says J is that record.
+// The pass loop of ppm_pass and ppm_pass_static.  J is the job record the launches take: the
+// context's own, or a copy whose partial results live in the static scratch (then no graphs,
+// which are captured with the context's own record).
 int run_passes(ppm_ctx* c, const Job& J, uint32_t npasses, bool allow_graph) {
     const uint64_t L = (uint64_t)c->prm.max_path_length;
     const bool graph = allow_graph && c->use_graph && !c->profiling;
@@ -840,6 +879,7 @@ int create_context(ppm_ctx* c, std::shared_ptr<SceneDev> scene, const ppm_camera
     if (p->device < 0 || p->device >= ndev) return set_err(c, PPM_E_INVAL, "device ordinal out of range");
     c->device = p->device;
     HIP_TRY(c, hipSetDevice(c->device));
+    static_scratch_address(c->device);
     if (stream) {
         c->stream = stream;
         c->own_stream = false;
@@ -1035,6 +1075,33 @@ int ppm_pass(ppm_ctx* c, uint32_t npasses) {
     HIP_TRY(c, hipSetDevice(c->device));
     if (npasses == 0) return PPM_OK;
     return run_passes(c, c->job, npasses, true);
+}
+
+int ppm_pass_static(ppm_ctx* c, uint32_t npasses) {
+    if (!c) return PPM_E_INVAL;
+    if (!c->own_stream) return set_err(c, PPM_E_INVAL, "this context belongs to a batch; use ppm_batch_pass");
+    HIP_TRY(c, hipSetDevice(c->device));
+    // the photon stage's per-block emitted counts, then the eye kernel's per-block min/max
+    const size_t photon_bytes = ((size_t)c->job.trace_blocks * 4 + 255) / 256 * 256;
+    const size_t eye_bytes = (size_t)c->job.tile_blocks * 6 * 4;
+    if (photon_bytes + eye_bytes > kStaticScratchBytes)
+        return set_err(c, PPM_E_INVAL,
+                       "ppm_pass_static: this context's partial results need " +
+                           std::to_string(photon_bytes + eye_bytes) + " bytes of scratch; the static scratch holds " +
+                           std::to_string(kStaticScratchBytes) + " (use ppm_pass, or larger photon-walk blocks)");
+    void* base = nullptr;
+    int rc = static_scratch(c, &base);
+    if (rc) return rc;
+    if (npasses == 0) return PPM_OK;
+    Job J = c->job;
+    J.ps.partial = static_cast<unsigned int*>(base);
+    J.hs.partial = reinterpret_cast<float*>(static_cast<char*>(base) + photon_bytes);
+    // per-kernel timing would create events: it stays off for these passes
+    const bool prof = c->profiling;
+    c->profiling = false;
+    rc = run_passes(c, J, npasses, false);
+    c->profiling = prof;
+    return rc;
 }
 
 int ppm_sync(ppm_ctx* c) {
