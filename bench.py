@@ -34,10 +34,10 @@ HBM_PEAK_GBS = 8000.0
 #   hit point per pass (k_gather, with the draw fused): gather and update 80 B (60 read, 20
 #     written) + draw 40 B = 120 B
 #   eye ray (eye stage, every L passes): 104 B
-#   hash slot (k_clear_slots): clear plus read of count and representative, 16 B
+#   hash slot (k_pass_begin): clear plus read of count and representative, 16 B
 BYTES_PER = {"photon": 216.0, "gather": 120.0, "eye": 104.0, "clear": 16.0}
-KERNEL_NAMES = {"photon": "photon stage (k_photon_stage + k_photon_sum)",
-                "gather": "k_gather", "eye": "eye stage (k_eye + k_grid)", "clear": "k_clear_slots"}
+KERNEL_NAMES = {"photon": "photon stage (k_photon_stage)", "gather": "k_gather",
+                "eye": "eye stage (k_eye_begin + k_eye + k_grid)", "clear": "k_pass_begin (pass begin and hash clear)"}
 
 WORKLOADS = {
     "spheres-1024": dict(kind=1, seed=0x67736C70, count=40, detail=4, res=1024, L=10, camera=((0, 0, 1.667), (0, 0, 0), 45.0),

@@ -79,7 +79,8 @@ void retire_graph(ppm_batch* b) {
 void queue_batch_pass(ppm_batch* b, hipStream_t st) {
     const int n = b->n_jobs;
     const Job* J = b->d_jobs;
-    hipLaunchKernelGGL(k_pass_begin_batch, dim3((n + 63) / 64), dim3(64), 0, st, J, n);
+    hipLaunchKernelGGL(k_pass_begin_batch, dim3(b->n_blocks[kMapClear]), dim3(256), 0, st, J,
+                       (const BlockRef*)b->d_map[kMapClear]);
     hipStream_t es = b->eye_overlap ? b->side : st;
     if (b->eye_overlap) {
         (void)hipEventRecord(b->ev_fork, st);
@@ -90,11 +91,8 @@ void queue_batch_pass(ppm_batch* b, hipStream_t st) {
                        (const BlockRef*)b->d_map[kMapTile]);
     hipLaunchKernelGGL(k_grid_batch, dim3(n), dim3(1024), 0, es, J);
     if (b->eye_overlap) (void)hipEventRecord(b->ev_join, es);
-    hipLaunchKernelGGL(k_clear_slots_batch, dim3(b->n_blocks[kMapClear]), dim3(256), 0, st, J,
-                       (const BlockRef*)b->d_map[kMapClear]);
     hipLaunchKernelGGL(k_photon_stage_batch, dim3(b->n_blocks[kMapTrace]), dim3(b->trace_block),
                        walk_lds_bytes(b->walk_stack, b->trace_block), st, J, (const BlockRef*)b->d_map[kMapTrace]);
-    hipLaunchKernelGGL(k_photon_sum_batch, dim3(n), dim3(1024), 0, st, J);
     if (b->eye_overlap) (void)hipStreamWaitEvent(st, b->ev_join, 0);
     hipLaunchKernelGGL(k_hash_batch, dim3(b->n_blocks[kMapPhoton]), dim3(256), 0, st, J,
                        (const BlockRef*)b->d_map[kMapPhoton]);
@@ -222,9 +220,8 @@ int batch_fetch(ppm_batch* b, int32_t job, float4* radiance, float4* flux_radius
         std::memset(counters, 0, sizeof(*counters));
         counters->frame_count = g.frame;
         counters->eye_samples = g.eye_samples;
-        counters->photons_emitted = g.photons_emitted;
+        folded_photon_counts(g, &counters->photons_emitted, &counters->num_photons_f);
         counters->photon_segments = (uint64_t)c->P * g.frame;
-        counters->num_photons_f = g.num_photons_f;
     }
     return PPM_OK;
 }

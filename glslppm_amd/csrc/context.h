@@ -120,6 +120,16 @@ struct ppm_ctx {
     std::string err;
 };
 
+// NumPhotons and the exact emitted count as a reader sees them after the passes queued so far: the
+// last pass's per-wave counts are still in the job's counters, and are folded in here exactly as
+// the next pass's k_pass_begin folds them (NumPhotons += int(sum) in float, glslppm.cpp:396-397).
+inline void folded_photon_counts(const DevGlobals& g, uint64_t* emitted, float* num_photons) {
+    unsigned int sum = 0;
+    for (int k = 0; k < kEmitShards; k++) sum += g.emit_shard[k * kEmitStride];
+    *emitted = g.photons_emitted + sum;
+    *num_photons = g.num_photons_f + (float)(int)sum;
+}
+
 // Waits for everything queued on a context's streams.
 inline hipError_t sync_streams(ppm_ctx* c) {
     hipError_t e = hipStreamSynchronize(c->stream);

@@ -79,6 +79,12 @@ struct GridScalars {
     float pad;
 };
 
+// Emitted-flag counters of the pass in progress: k_photon_stage adds each block's count to one of
+// kEmitShards counters (block b to shard b % kEmitShards), each in a 128-byte line of its own so
+// that no line takes many atomics; the next pass's begin folds them into NumPhotons.
+constexpr int kEmitShards = 64;
+constexpr int kEmitStride = 32;  // unsigned ints between shards
+
 struct DevGlobals {
     // of the latest eye pass of each parity: the passes after eye pass e read grid[e & 1], as they
     // read that eye pass's hit points (HitState buffers)
@@ -86,8 +92,8 @@ struct DevGlobals {
     unsigned long long photons_emitted;
     unsigned long long frame;        // passes run (FrameCount)
     unsigned long long eye_samples;  // eye passes begun; the latest one's buffers are (eye_samples - 1) & 1
-    float num_photons_f;
-    unsigned int emitted_this_pass;
+    float num_photons_f;             // NumPhotons up to the pass before the last begun one
+    unsigned int pad2;
     unsigned int pad0;
     unsigned int xs[4];              // XORShift state after the seed grids (AAOffset draws)
     unsigned int passes_left;        // passes this job still runs; 0xffffffff: no limit
@@ -103,6 +109,8 @@ struct DevGlobals {
     int defer;                       // this pass's deposits wait for k_hash (its grid is not set up)
     unsigned int eye_stages_done;    // eye stages whose grid is set up (k_grid)
     int pad1;
+    // emitted flags of the last begun pass, not yet in num_photons_f / photons_emitted
+    unsigned int emit_shard[kEmitShards * kEmitStride];
 };
 
 // The eye pass's outputs are double-buffered by eye-pass parity: eye pass e writes buffer e & 1
@@ -136,7 +144,6 @@ struct PhotonState {
     float4* rng;
     int2* state;
     int* slot;
-    unsigned int* partial;  // [k_photon_stage blocks] emitted counts
 };
 
 struct HashState {
@@ -205,7 +212,7 @@ struct Job {
     DevGlobals* g;
     int tile_blocks;     // pixel tiles (k_eye, k_gather)
     int photon_blocks;   // 256 photon slots per block (k_hash)
-    int clear_blocks;    // 256 hash slots or ray buckets per block (k_clear_slots)
+    int clear_blocks;    // 256 hash slots per block (k_pass_begin)
     int trace_blocks;    // photon slots per block = the launch's block size (k_photon_stage)
     int walk_stack;      // stack entries per lane of the walks (LDS: walk_stack x block x 4 bytes)
 };
@@ -217,26 +224,22 @@ struct BlockRef {
 };
 
 // Kernels of one job: grid = that job's blocks.  The _batch forms take the job array and, for
-// the kernels with several blocks per job, the block map of that kernel kind; k_pass_begin_batch
-// runs one thread per job, k_grid_batch and k_photon_sum_batch one block per job.  k_photon_stage
-// runs with any block size up to kTraceBlockMax (a multiple of 64); the others with 256 (1024
-// for k_grid and k_photon_sum).
+// the kernels with several blocks per job, the block map of that kernel kind (k_pass_begin the
+// hash-clear map); k_eye_begin_batch runs one thread per job, k_grid_batch one block per job.
+// k_photon_stage runs with any block size up to kTraceBlockMax (a multiple of 64); the others with
+// 256 (1024 for k_grid).
 __global__ void k_pass_begin(Job J);
-__global__ void k_clear_slots(Job J);
 __global__ void k_eye_begin(Job J, int ahead);
 __global__ void k_eye(Job J);
 __global__ void k_grid(Job J);
 __global__ void k_photon_stage(Job J);
-__global__ void k_photon_sum(Job J);
 __global__ void k_hash(Job J);
 __global__ void k_gather(Job J);
-__global__ void k_pass_begin_batch(const Job* jobs, int n_jobs);
-__global__ void k_clear_slots_batch(const Job* jobs, const BlockRef* map);
+__global__ void k_pass_begin_batch(const Job* jobs, const BlockRef* map);
 __global__ void k_eye_begin_batch(const Job* jobs, int n_jobs);
 __global__ void k_eye_batch(const Job* jobs, const BlockRef* map);
 __global__ void k_grid_batch(const Job* jobs);
 __global__ void k_photon_stage_batch(const Job* jobs, const BlockRef* map);
-__global__ void k_photon_sum_batch(const Job* jobs);
 __global__ void k_hash_batch(const Job* jobs, const BlockRef* map);
 __global__ void k_gather_batch(const Job* jobs, const BlockRef* map);
 __global__ void k_batch_status(const Job* jobs, int n_jobs, unsigned long long* out);

@@ -1,11 +1,12 @@
 // Kernels of the SPPM pass loop for gfx950.  One pass (m_display, glslppm.cpp:214-536) is:
-//   k_pass_begin    frame count, eye cadence and AAOffset of the pass (one thread per job)
-//   k_eye      every max_path passes: eyeraytrace.fs + per-block max.fs/min.fs partials
-//   k_grid     the hit-point box and the grid uniforms of glslppm.cpp:292-311 (one block)
-//   k_clear_slots
+//   k_pass_begin    frame count, eye cadence, NumPhotons of the last pass (block 0) and the clear
+//                   of the hash slots (every block)
+//   k_eye_begin, k_eye, k_grid  every max_path passes: AAOffset, eyeraytrace.fs + per-block
+//                   max.fs/min.fs partials, the hit-point box and the grid uniforms of
+//                   glslppm.cpp:292-311 (one block)
 //   k_photon_stage  photontrace.fs emission/bounce, walk and deposit + hash.fs + scatter (atomic
-//                   min of the photon index) + correction (atomic count), one thread per slot
-//   k_photon_sum    sum.fs and NumPhotons (one block)
+//                   min of the photon index) + correction (atomic count), one thread per slot;
+//                   sum.fs as per-block counts
 //   k_gather   progressive.fs (8-cell gather and the N/R/tau update) + re.fs draw to RGBA8
 // Arithmetic follows the GLSL statement by statement in strict fp32 (see fmath.h).
 //
@@ -554,15 +555,14 @@ PPM_DEV float xs_frand(unsigned int* s) {
 
 // Per-pass uniforms of m_display (glslppm.cpp:214-287): eye pass when FrameCount % MaxPathLength
 // is 0, with two AAOffset draws from the job's XORShift stream and NumEyeSamples = count + 1;
-// FrameCount 0 is the first pass.  A job whose pass budget is spent stops advancing.
+// FrameCount 0 is the first pass.  A job whose pass budget is spent stops advancing; the budget
+// itself is counted down by the pass's gather (k_pass_begin's other blocks read it meanwhile).
 PPM_DEV void pass_begin(const Job& J) {
     DevGlobals* g = J.g;
-    const unsigned int left = g->passes_left;
-    if (left == 0u) {
+    if (g->passes_left == 0u) {
         g->active = 0;
         return;
     }
-    if (left != 0xffffffffu) g->passes_left = left - 1u;
     const unsigned long long frame = g->frame;
     const int eye = (frame % (unsigned long long)J.ep.max_path) == 0ull;
     g->active = 1;
@@ -598,12 +598,39 @@ PPM_DEV void eye_begin(const Job& J, int ahead) {
 // The buffer of the latest eye pass, which the passes after it read.
 PPM_DEV int cur_buffer(const DevGlobals* g) { return (int)((g->eye_samples + 1ull) & 1ull); }
 
-PPM_DEV void clear_slots(const Job& J, int blk) {
-    if (!J.g->active) return;
+// The start of a pass: block 0 folds the last pass's emitted counts into NumPhotons
+// (glslppm.cpp:396-397, `NumPhotons += int(sum)` once per pass, in float), clears the counters and
+// sets the pass's uniforms (pass_begin); every block clears its hash slots (the glClear of
+// glslppm.cpp:421, 450).  Whether the pass runs is the budget left, which no block changes here.
+PPM_DEV void begin_and_clear(const Job& J, int blk) {
+    DevGlobals* g = J.g;
+    if (blk == 0 && threadIdx.x < 64) {
+        unsigned int* sh = g->emit_shard + threadIdx.x * kEmitStride;
+        const unsigned int v = wave_sum(*sh);
+        *sh = 0u;
+        if (threadIdx.x == 0) {
+            g->num_photons_f = g->num_photons_f + (float)(int)v;
+            g->photons_emitted += v;
+            pass_begin(J);
+        }
+    }
+    if (g->passes_left == 0u) return;
     const int i = blk * kBlock + threadIdx.x;
     if (i < J.h.size) {
         reinterpret_cast<uint2*>(J.h.slots)[i] = make_uint2(0u, 0xffffffffu);
     }
+}
+
+// NumPhotons after the pass in progress: the folded count plus the pass's counters (every
+// thread of the block calls it; one wave reads the counters).
+PPM_DEV float num_photons_now(const DevGlobals* g) {
+    __shared__ unsigned int pend;
+    if (threadIdx.x < 64) {
+        const unsigned int v = wave_sum(g->emit_shard[threadIdx.x * kEmitStride]);
+        if (threadIdx.x == 0) pend = v;
+    }
+    __syncthreads();
+    return g->num_photons_f + (float)(int)pend;
 }
 
 }  // namespace
@@ -933,8 +960,9 @@ PPM_DEV void ibl_sample(const DevScene& s, float4& rndv, f3& flux, f3& dir, f3& 
 // k_photon_stage runs one thread per photon slot: emission or bounce with Russian roulette
 // (photontrace.fs:466-686); a terminated photon writes its final state at once, a continuing one
 // goes on through the walk and the deposit (photontrace.fs:690-710), hash.fs, scatter and
-// correction.  k_photon_sum adds up the blocks' emitted flags (sum.fs).  The slot table only sees
-// commutative atomics, so the result equals the reference's one fragment per photon texel.
+// correction.  Each wave adds its emitted flags (sum.fs) to one of the job's counters.  The slot
+// table only sees commutative atomics, so the result equals the reference's one fragment per
+// photon texel.
 // (An earlier form compacted the continuing rays into a queue for a separate walk kernel; the
 // queue's 56 bytes per photon written and read again, and two more launches, cost more than the
 // idle lanes of the photons that end: 1.95 -> 2.02 G photon segments/s on the 205k-triangle
@@ -1153,17 +1181,10 @@ PPM_DEV void photon_stage(const Job& J, int blk) {
     f3 org, dir, flux;
     int out_bits = 0;
     if (p < pp.count) live = photon_gen(s, pp, J.ps, p, org, dir, flux, out_bits, emitted);
-    // sum.fs: this block's emitted flags, added up by k_photon_sum
+    // sum.fs: each wave's emitted flags go to its block's counter (folded in by the next pass's
+    // k_pass_begin; the gather's draw adds them itself)
     const unsigned int es = wave_sum(emitted);
-    __shared__ unsigned int we[kTraceBlockMax / 64];
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    if (lane == 0) we[wave] = es;
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        unsigned int ne = 0;
-        for (int w = 0; w < (int)(blockDim.x >> 6); w++) ne += we[w];
-        J.ps.partial[blk] = ne;
-    }
+    if ((threadIdx.x & 63) == 0 && es != 0u) atomicAdd(J.g->emit_shard + (blk % kEmitShards) * kEmitStride, es);
     if (!live) return;
     Walk W;
     W.org = org;
@@ -1176,36 +1197,6 @@ PPM_DEV void photon_stage(const Job& J, int blk) {
     W.h.bx = W.h.by = W.h.bz = 0.0f;
     walk<true>(s, W.face, W.org, W.dir, W.rcp, W.t0, false, W.h, J.walk_stack);
     photon_finish(s, pp, J.ps, J.h, g, p, W, flux, out_bits);
-}
-
-// NumPhotons += int(sum of emitted flags) (glslppm.cpp:396-397) from k_photon_stage's per-block
-// counts; one block of 1024 threads.
-PPM_DEV void photon_sum(const Job& J) {
-    DevGlobals* g = J.g;
-    if (!g->active) return;
-    // four counts per 16-byte load, eight loads in flight per thread (2^22 photons in one-wave
-    // blocks leave 65,536 counts: a load per count took 28 us, one after another)
-    const int n = J.trace_blocks, n4 = n >> 2;
-    const uint4* __restrict__ p4 = reinterpret_cast<const uint4*>(J.ps.partial);
-    unsigned int e = 0;
-#pragma unroll 8
-    for (int b = threadIdx.x; b < n4; b += 1024) {
-        const uint4 v = p4[b];
-        e += (v.x + v.y) + (v.z + v.w);
-    }
-    for (int b = 4 * n4 + threadIdx.x; b < n; b += 1024) e += J.ps.partial[b];
-    e = wave_sum(e);
-    __shared__ unsigned int esum[16];
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    if (lane == 0) esum[wave] = e;
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        unsigned int emitted = 0;
-        for (int w = 0; w < 16; w++) emitted += esum[w];
-        g->num_photons_f = g->num_photons_f + (float)(int)emitted;
-        g->photons_emitted += emitted;
-        g->emitted_this_pass = emitted;
-    }
 }
 
 // re.fs: tone map of one channel
@@ -1236,6 +1227,9 @@ PPM_DEV void gather(const Job& J, int blk) {
     const DevGlobals* g = J.g;
     if (!g->active) return;
     const GatherParams& gp = J.gp;
+    const float np = gp.draw ? num_photons_now(g) : 0.0f;
+    // the pass is done with the budget: count it down (k_pass_begin read it)
+    if (blk == 0 && threadIdx.x == 0 && g->passes_left != 0xffffffffu) J.g->passes_left = g->passes_left - 1u;
     const Tiles& tl = J.tl;
     const HitState& hs = J.hs;
     const PhotonState& ps = J.ps;
@@ -1310,7 +1304,7 @@ PPM_DEV void gather(const Job& J, int blk) {
     hs.e_n[p] = en;
     if (gp.draw) {
         float4 lin, mapped;
-        radiance(tr, en, g->num_photons_f, lin, mapped);
+        radiance(tr, en, np, lin, mapped);
         hs.rgba8[p] = make_uchar4(unorm8(mapped.x), unorm8(mapped.y), unorm8(mapped.z), unorm8(mapped.w));
     }
 }
@@ -1345,7 +1339,7 @@ PPM_DEV void gather(const Job& J, int blk) {
         body(jobs[blockIdx.x]);                                                                     \
     }
 
-PPM_JOB_KERNEL(k_clear_slots, clear_slots, kBlock)
+PPM_JOB_KERNEL(k_pass_begin, begin_and_clear, kBlock)
 PPM_JOB_KERNEL_WAVES(k_eye, eye, kBlock, 6)
 
 __global__ void k_eye_begin(Job J, int ahead) {
@@ -1358,18 +1352,9 @@ __global__ void k_eye_begin_batch(const Job* __restrict__ jobs, int n_jobs) {
 }
 PPM_JOB_KERNEL1(k_grid, grid, 1024)
 PPM_JOB_KERNEL_WAVES(k_photon_stage, photon_stage, kTraceBlockMax, 7)
-PPM_JOB_KERNEL1(k_photon_sum, photon_sum, 1024)
 PPM_JOB_KERNEL(k_hash, hash_deferred, kBlock)
 PPM_JOB_KERNEL(k_gather, gather, kBlock)
 
-__global__ void k_pass_begin(Job J) {
-    if (threadIdx.x == 0) pass_begin(J);
-}
-
-__global__ void k_pass_begin_batch(const Job* __restrict__ jobs, int n_jobs) {
-    const int j = blockIdx.x * blockDim.x + threadIdx.x;
-    if (j < n_jobs) pass_begin(jobs[j]);
-}
 
 // (frame, passes_left) of every job of a batch, for the host's progress snapshot.
 __global__ void k_batch_status(const Job* __restrict__ jobs, int n_jobs, unsigned long long* out) {
@@ -1381,10 +1366,11 @@ __global__ void k_batch_status(const Job* __restrict__ jobs, int n_jobs, unsigne
 }
 
 __global__ void k_resolve(int count, HitState hs, const DevGlobals* g, float4* linear, float4* mapped) {
+    const float np = num_photons_now(g);
     const int p = blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= count) return;
     float4 lin, mp;
-    radiance(hs.tau_r[p], hs.e_n[p], g->num_photons_f, lin, mp);
+    radiance(hs.tau_r[p], hs.e_n[p], np, lin, mp);
     linear[p] = lin;
     mapped[p] = mp;
     hs.rgba8[p] = make_uchar4(unorm8(mp.x), unorm8(mp.y), unorm8(mp.z), unorm8(mp.w));

@@ -21,8 +21,8 @@
 
 #include "context.h"
 
-// Scratch of ppm_pass_static: the photon stage's per-block emitted counts, then the eye kernel's
-// per-block min/max partials.  Each GPU holds its own copy for the life of the process, and every
+// Scratch of ppm_pass_static: the eye kernel's per-block min/max partials (24 bytes per tile of
+// 256 pixels: images up to about 11M pixels).  Each GPU holds its own copy for the life of the process, and every
 // ppm_pass_static call on that GPU uses it.  Host code uses the array only through the device
 // address taken once per device at set-up (static_scratch_address), never by its name.
 constexpr size_t kStaticScratchBytes = 1u << 20;
@@ -463,7 +463,9 @@ void queue_pass(ppm_ctx* c, const Job& J, hipStream_t st, int kind, bool fork_ah
         return;
     }
     const bool stage = kind == kEyePass;
-    hipLaunchKernelGGL(k_pass_begin, dim3(1), dim3(64), 0, st, J);
+    hipEvent_t e1 = prof_begin(c, PPM_KERNEL_CLEAR, st);
+    hipLaunchKernelGGL(k_pass_begin, dim3(J.clear_blocks), dim3(256), 0, st, J);
+    prof_end(c, PPM_KERNEL_CLEAR, e1, st);
     const bool fork = stage && c->side && c->eye_overlap;
     hipStream_t es = fork ? c->side : st;
     if (fork) {
@@ -472,9 +474,6 @@ void queue_pass(ppm_ctx* c, const Job& J, hipStream_t st, int kind, bool fork_ah
     }
     if (stage) queue_eye_stage(c, J, es, 0);
     if (fork) (void)hipEventRecord(c->ev_join, es);
-    hipEvent_t e1 = prof_begin(c, PPM_KERNEL_CLEAR, st);
-    hipLaunchKernelGGL(k_clear_slots, dim3(J.clear_blocks), dim3(256), 0, st, J);
-    prof_end(c, PPM_KERNEL_CLEAR, e1, st);
     hipEvent_t e2 = prof_begin(c, PPM_KERNEL_PHOTON, st);
     if (fork_ahead) {
         (void)hipEventRecord(c->ev_fork, st);
@@ -484,7 +483,6 @@ void queue_pass(ppm_ctx* c, const Job& J, hipStream_t st, int kind, bool fork_ah
     }
     hipLaunchKernelGGL(k_photon_stage, dim3(J.trace_blocks), dim3(c->trace_block),
                        walk_lds_bytes(J.walk_stack, c->trace_block), st, J);
-    hipLaunchKernelGGL(k_photon_sum, dim3(1), dim3(1024), 0, st, J);
     prof_end(c, PPM_KERNEL_PHOTON, e2, st);
     if (fork) (void)hipStreamWaitEvent(st, c->ev_join, 0);
     hipEvent_t e3 = prof_begin(c, PPM_KERNEL_GATHER, st);
@@ -722,8 +720,8 @@ size_t state_footprint(const ppm_params* p) {
         // (two buffers each), rgba8, eye partials
         Q * 32, Q * 32, Q * 32, Q * 32, Q * 32, Q * 16, Q * 16, Q * 8, Q * 16, Q * 4, (size_t)max_tile_blocks(p) * 24,
         (size_t)max_tile_blocks(p) * 4, (size_t)max_tile_blocks(p) * 4,  // tile costs and order
-        // photons: pos, flux, dir, rng, state, slot, emitted partials
-        P * 16, P * 16, P * 16, P * 16, P * 8, P * 4, (P + 63) / 64 * 4,
+        // photons: pos, flux, dir, rng, state, slot
+        P * 16, P * 16, P * 16, P * 16, P * 8, P * 4,
         // hash (count, representative) records; globals; resolve targets
         H * 8, sizeof(DevGlobals), Q * 16, Q * 16};
     size_t tot = 0;
@@ -941,7 +939,7 @@ int create_context(ppm_ctx* c, std::shared_ptr<SceneDev> scene, const ppm_camera
         (rc = dev_alloc(c, &hs.tile_order, (size_t)max_tile_blocks(p))) || (rc = dev_alloc(c, &ps.pos, P)) ||
         (rc = dev_alloc(c, &ps.flux, P)) || (rc = dev_alloc(c, &ps.dir, P)) || (rc = dev_alloc(c, &ps.rng, P)) ||
         (rc = dev_alloc(c, &ps.state, P)) || (rc = dev_alloc(c, &ps.slot, P)) ||
-        (rc = dev_alloc(c, &ps.partial, (size_t)(P + 63) / 64)) ||
+
         (rc = dev_alloc(c, &J.h.slots, 2 * H)) || (rc = dev_alloc(c, &J.g, 1)) ||
         (rc = dev_alloc(c, &c->resolve_lin, Q)) || (rc = dev_alloc(c, &c->resolve_map, Q)))
         return rc;
@@ -1081,21 +1079,19 @@ int ppm_pass_static(ppm_ctx* c, uint32_t npasses) {
     if (!c) return PPM_E_INVAL;
     if (!c->own_stream) return set_err(c, PPM_E_INVAL, "this context belongs to a batch; use ppm_batch_pass");
     HIP_TRY(c, hipSetDevice(c->device));
-    // the photon stage's per-block emitted counts, then the eye kernel's per-block min/max
-    const size_t photon_bytes = ((size_t)c->job.trace_blocks * 4 + 255) / 256 * 256;
+    // the eye kernel's per-block min/max of hit positions
     const size_t eye_bytes = (size_t)c->job.tile_blocks * 6 * 4;
-    if (photon_bytes + eye_bytes > kStaticScratchBytes)
+    if (eye_bytes > kStaticScratchBytes)
         return set_err(c, PPM_E_INVAL,
-                       "ppm_pass_static: this context's partial results need " +
-                           std::to_string(photon_bytes + eye_bytes) + " bytes of scratch; the static scratch holds " +
-                           std::to_string(kStaticScratchBytes) + " (use ppm_pass, or larger photon-walk blocks)");
+                       "ppm_pass_static: this context's partial results need " + std::to_string(eye_bytes) +
+                           " bytes of scratch; the static scratch holds " + std::to_string(kStaticScratchBytes) +
+                           " (use ppm_pass for images above about 11M pixels)");
     void* base = nullptr;
     int rc = static_scratch(c, &base);
     if (rc) return rc;
     if (npasses == 0) return PPM_OK;
     Job J = c->job;
-    J.ps.partial = static_cast<unsigned int*>(base);
-    J.hs.partial = reinterpret_cast<float*>(static_cast<char*>(base) + photon_bytes);
+    J.hs.partial = static_cast<float*>(base);
     // per-kernel timing would create events: it stays off for these passes
     const bool prof = c->profiling;
     c->profiling = false;
@@ -1231,9 +1227,8 @@ int ppm_readback(ppm_ctx* c, ppm_buf which, void* dst, size_t bytes) {
             std::memset(&k, 0, sizeof(k));
             k.frame_count = g.frame;
             k.eye_samples = g.eye_samples;
-            k.photons_emitted = g.photons_emitted;
+            folded_photon_counts(g, &k.photons_emitted, &k.num_photons_f);
             k.photon_segments = (uint64_t)c->P * g.frame;
-            k.num_photons_f = g.num_photons_f;
             std::memcpy(dst, &k, sizeof(k));
             return PPM_OK;
         }

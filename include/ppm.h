@@ -182,15 +182,13 @@ int ppm_pass(ppm_ctx* ctx, uint32_t npasses);
 /* ppm_pass for callers that cannot allocate device memory (code under HIP graph capture, tight
  * serving loops, harnesses that forbid hipMalloc).  No counterpart in the reference.  The same
  * passes with bit-identical results, launched stream by stream (never as graphs, whatever
- * PPM_FLAG_GRAPH says), with the passes' partial results (the photon stage's per-block emitted
- * counts and the eye kernel's per-block min/max of hit positions) in a static device array of
- * 1 MiB per GPU instead of the context's own buffers; the call allocates nothing and records no
- * per-kernel timing events.  The context's partial results must fit: 4 bytes per photon-walk
- * block, rounded up to 256 bytes, plus 24 bytes per pixel tile (PPM_E_INVAL otherwise; about
- * 460 KB for 2048^2 photons and a 1920x1080 image with the default launch shapes).  Every call on
- * one GPU shares that array: calls whose passes overlap on the GPU (contexts on different
- * streams, or an eye stage a call started ahead that is still running; ppm_sync waits for both)
- * race and give wrong results. */
+ * PPM_FLAG_GRAPH says), with the passes' partial results (the eye kernel's per-block min/max of
+ * hit positions, 24 bytes per tile of 256 pixels) in a static device array of 1 MiB per GPU
+ * instead of the context's own buffer; the call allocates nothing and records no per-kernel
+ * timing events.  Images above about 11M pixels (43,690 tiles) do not fit: PPM_E_INVAL.  Every
+ * call on one GPU shares that array: calls whose passes overlap on the GPU (contexts on
+ * different streams, or an eye stage a call started ahead that is still running; ppm_sync waits
+ * for both) race and give wrong results. */
 int ppm_pass_static(ppm_ctx* ctx, uint32_t npasses);
 
 /* Waits for queued passes and copies one buffer to host memory. */

@@ -1,7 +1,7 @@
 """ppm_pass_static, the pass loop that allocates nothing, against ppm_pass, bit for bit.
 
-ppm_pass_static runs the same passes as ppm_pass with the passes' partial results (per-block
-emitted counts, per-block min/max of hit positions) in the library's static device scratch,
+ppm_pass_static runs the same passes as ppm_pass with the passes' partial results (the eye
+kernel's per-block min/max of hit positions) in the library's static device scratch,
 which keeps the last call's contents.  Two calls in a row on different contexts, the second
 smaller, must each give ppm_pass's results; so must a context that alternates the two entry
 points.  A context whose partial results do not fit is refused before anything is queued.
@@ -59,11 +59,10 @@ def test_static_and_plain_passes_alternate_on_one_context():
 
 
 def test_contexts_too_large_for_the_scratch_are_refused():
-    """2048^2 photons in one-wave walk blocks need 256 KiB of counts, a 4096x2304 image 864 KiB
-    of min/max partials (36,864 tiles of 16x16 pixels): together more than the 1 MiB scratch."""
-    m = spheres(9, 4096, 2304)
-    p = g.default_params(2048, image_width=4096, image_height=2304, hash_width=64, hash_height=64,
-                         max_path_length=4)
+    """A 6144x2048 image has 49,152 tiles of 16x16 pixels, whose min/max partials (24 bytes each)
+    exceed the 1 MiB scratch."""
+    m = spheres(9, 6144, 2048)
+    p = g.default_params(64, image_width=6144, image_height=2048, max_path_length=4)
     r = g.Renderer(m, p)
     with pytest.raises(g.PPMError, match="static scratch"):
         r.run_static(1)
