@@ -96,6 +96,19 @@ def batch_jobs(name, n_jobs=64):
         return list(ex.map(make, specs))
 
 
+def amdahl_bound(plain_s, plain_by_res, work_by_res):
+    """Upper bound on the batch's speedup over the plain loop: the jobs with the largest image in
+    the mix (1024^2 here, a million photons per pass) already keep the whole GPU busy in the plain
+    loop, so their photon segments per second there is the rate the GPU sustains on these scenes;
+    batching hides the small jobs' idle launch gaps but must still do their work, so the batch takes
+    at least (all photon segments) / (that rate)."""
+    big = max(plain_by_res)
+    if plain_by_res[big] <= 0 or work_by_res.get(big, 0) <= 0:
+        return None
+    rate = work_by_res[big] / plain_by_res[big]
+    return plain_s / (sum(work_by_res.values()) / rate)
+
+
 def run_batch_workload(args):
     """Aggregate photon segments/s of one batch against the plain one-scene loop, job after job,
     over the same jobs and passes; scene upload and state creation are outside both timings."""
@@ -109,13 +122,17 @@ def run_batch_workload(args):
 
     # plain loop: one Renderer per job, its passes, synchronised
     plain_s = 0.0
+    plain_by_res, work_by_res = {}, {}
     for (m, p, n) in jobs:
         r = g.Renderer(m, p)
         r.sync()
         t0 = time.perf_counter()
         r.run(n)
         r.sync()
-        plain_s += time.perf_counter() - t0
+        dt = time.perf_counter() - t0
+        plain_s += dt
+        plain_by_res[p.image_width] = plain_by_res.get(p.image_width, 0.0) + dt
+        work_by_res[p.image_width] = work_by_res.get(p.image_width, 0) + p.photon_width * p.photon_height * n
         r.close()
 
     b = g.Batch(passes_per_graph=args.passes_per_graph)
@@ -158,9 +175,11 @@ def run_batch_workload(args):
                    "passes_per_graph": args.passes_per_graph, "scene_build_s": round(build_s, 2),
                    "batch_hbm_bytes": hbm},
         "eye_rays_per_s": eye / batch_s,
-        "plain_loop": {"value": work / plain_s, "unit": "photons/s", "seconds": plain_s},
+        "plain_loop": {"value": work / plain_s, "unit": "photons/s", "seconds": plain_s,
+                       "seconds_by_resolution": {str(k): round(v, 4) for k, v in sorted(plain_by_res.items())}},
         "batch_seconds": batch_s,
         "speedup_vs_plain_loop": plain_s / batch_s,
+        "amdahl_bound": amdahl_bound(plain_s, plain_by_res, work_by_res),
         "largest_job_equal_alone": same,
     }
     print(json.dumps(line), flush=True)
@@ -211,13 +230,25 @@ def run_serve_workload(args):
                 time.sleep(delay)
             si, _, p, target = specs[k]
             ids[k] = srv.submit(sids[si], p, target, camera=cams[k])
+            if (k + 1) % 500 == 0:
+                st = srv.stats()
+                print("serve: %d of %d jobs submitted at %.1f s, %d done, %d resident, %d queued" % (
+                    k + 1, n_jobs, srv.now() - t0, st["done"], st["resident"], st["queued"]), file=sys.stderr, flush=True)
 
+    # jobs are collected in arrival order as they finish; their host copies are released at once,
+    # except those of the three jobs checked against the job alone below
+    keep = {0, n_jobs // 2, n_jobs - 1}
+    infos = [None] * n_jobs
     th = threading.Thread(target=feeder)
     th.start()
+    for k in range(n_jobs):
+        while ids[k] is None:
+            time.sleep(0.001)
+        srv.wait(ids[k])
+        infos[k] = srv.info(ids[k])
+        if k not in keep:
+            srv.release(ids[k])
     th.join()
-    for j in ids:
-        srv.wait(j)
-    infos = [srv.info(j) for j in ids]
     st = srv.stats()
     first = np.array([i["t_first_pass"] - i["t_submit"] for i in infos])
     target_t = np.array([i["t_done"] - i["t_submit"] for i in infos])
@@ -238,6 +269,7 @@ def run_serve_workload(args):
     # the plain one-scene loop on a sample of the same jobs, one after another
     sample = list(range(0, n_jobs, max(1, n_jobs // args.plain_sample)))
     plain_s, plain_work = 0.0, 0
+    plain_by_res, work_by_res = {}, {}
     for k in sample:
         si, cam, p, target = specs[k]
         m = pool[si][0]
@@ -247,7 +279,10 @@ def run_serve_workload(args):
         t1 = time.perf_counter()
         r.run(target)
         r.sync()
-        plain_s += time.perf_counter() - t1
+        dt = time.perf_counter() - t1
+        plain_s += dt
+        plain_by_res[p.image_width] = plain_by_res.get(p.image_width, 0.0) + dt
+        work_by_res[p.image_width] = work_by_res.get(p.image_width, 0) + p.photon_width * p.photon_height * target
         plain_work += p.photon_width * p.photon_height * target
         r.close()
     pct = lambda a, q: float(np.percentile(a, q))
@@ -271,8 +306,10 @@ def run_serve_workload(args):
         "time_to_target_s": {"p50": pct(target_t, 50), "p99": pct(target_t, 99), "max": float(target_t.max())},
         "span_s": span,
         "gpu_busy_fraction": st["busy_seconds"] / span,
-        "plain_loop": {"value": plain_work / plain_s, "unit": "photons/s", "jobs": len(sample)},
+        "plain_loop": {"value": plain_work / plain_s, "unit": "photons/s", "jobs": len(sample),
+                       "seconds_by_resolution": {str(k): round(v, 4) for k, v in sorted(plain_by_res.items())}},
         "speedup_vs_plain_loop": (work / span) / (plain_work / plain_s),
+        "amdahl_bound": amdahl_bound(plain_s, plain_by_res, work_by_res),
         "jobs_equal_alone": same,
     }
     print(json.dumps(line), flush=True)

@@ -73,24 +73,42 @@ void retire_graph(ppm_batch* b) {
     b->graph = nullptr;
 }
 
-// One batched pass over the current tables: m_display's stage order for every job.  The deposits
-// of the jobs in an eye pass wait for k_hash; with eye_overlap the eye stage runs on the side
-// stream beside the photon stage, as in the plain loop.
-void queue_batch_pass(ppm_batch* b, hipStream_t st) {
+// One batched pass over the current tables: m_display's stage order for every job.  With
+// eye_overlap the eye stages run beside the photon stage: a job's stage that has not begun when its
+// eye pass comes runs on `side` and that pass's deposits wait for k_hash after the join (kEyeNow);
+// with `run_ahead` (plain launches), a job's next eye stage begins on `ahead` in the pass after
+// each of its eye passes and runs beside the photon passes until the next one (kEyeAhead), so the
+// eye stages of many jobs whose eye passes coincide spread over the passes in between instead of
+// lengthening the eye pass.  Pass p waits for the ahead launches of pass p - lag before its photon
+// stage, which covers every ahead stage an eye pass needs (eye_begin starts one only with at least
+// `lag` passes to go).  Without eye_overlap everything runs in order on one stream.
+void queue_batch_pass(ppm_batch* b, hipStream_t st, bool run_ahead) {
     const int n = b->n_jobs;
     const Job* J = b->d_jobs;
+    const unsigned int pass = run_ahead ? b->pass_no : 0u;
     hipLaunchKernelGGL(k_pass_begin_batch, dim3(b->n_blocks[kMapClear]), dim3(256), 0, st, J,
-                       (const BlockRef*)b->d_map[kMapClear]);
+                       (const BlockRef*)b->d_map[kMapClear], pass);
     hipStream_t es = b->eye_overlap ? b->side : st;
     if (b->eye_overlap) {
         (void)hipEventRecord(b->ev_fork, st);
         (void)hipStreamWaitEvent(es, b->ev_fork, 0);
     }
-    hipLaunchKernelGGL(k_eye_begin_batch, dim3((n + 63) / 64), dim3(64), 0, es, J, n);
-    hipLaunchKernelGGL(k_eye_batch, dim3(b->n_blocks[kMapTile]), dim3(256), walk_lds_bytes(b->walk_stack, 256), es, J,
-                       (const BlockRef*)b->d_map[kMapTile]);
-    hipLaunchKernelGGL(k_grid_batch, dim3(n), dim3(1024), 0, es, J);
+    auto eye_stage = [&](hipStream_t s, int mode, int lag) {
+        hipLaunchKernelGGL(k_eye_begin_batch, dim3((n + 63) / 64), dim3(64), 0, s, J, n, mode, pass, lag);
+        hipLaunchKernelGGL(k_eye_batch, dim3(b->n_blocks[kMapTile]), dim3(256), walk_lds_bytes(b->walk_stack, 256), s,
+                           J, (const BlockRef*)b->d_map[kMapTile], mode);
+        hipLaunchKernelGGL(k_grid_batch, dim3(n), dim3(1024), 0, s, J, mode);
+    };
+    eye_stage(es, kEyeNow, 0);
     if (b->eye_overlap) (void)hipEventRecord(b->ev_join, es);
+    if (run_ahead) {
+        (void)hipStreamWaitEvent(b->ahead, b->ev_fork, 0);
+        eye_stage(b->ahead, kEyeAhead, b->lag);
+        (void)hipEventRecord(b->ring[pass % ppm_batch::kAheadRing], b->ahead);
+        if (pass >= b->ring_from + (uint32_t)b->lag)
+            (void)hipStreamWaitEvent(st, b->ring[(pass - (uint32_t)b->lag) % ppm_batch::kAheadRing], 0);
+        b->pass_no++;
+    }
     hipLaunchKernelGGL(k_photon_stage_batch, dim3(b->n_blocks[kMapTrace]), dim3(b->trace_block),
                        walk_lds_bytes(b->walk_stack, b->trace_block), st, J, (const BlockRef*)b->d_map[kMapTrace]);
     if (b->eye_overlap) (void)hipStreamWaitEvent(st, b->ev_join, 0);
@@ -98,6 +116,17 @@ void queue_batch_pass(ppm_batch* b, hipStream_t st) {
                        (const BlockRef*)b->d_map[kMapPhoton]);
     hipLaunchKernelGGL(k_gather_batch, dim3(b->n_blocks[kMapTile]), dim3(256), 0, st, J, (const BlockRef*)b->d_map[kMapTile]);
 }
+
+// Makes the pass stream wait for every eye stage queued on `ahead` so far, and starts a new ring
+// epoch (later passes need not wait for ring events recorded before this point).  Called before
+// the job tables or a job's memory change hands and before the host reads job state.
+void join_ahead(ppm_batch* b) {
+    if (b->pass_no > b->ring_from)
+        (void)hipStreamWaitEvent(b->stream, b->ring[(b->pass_no - 1) % ppm_batch::kAheadRing], 0);
+    b->ring_from = b->pass_no;
+}
+
+bool ahead_mode(const ppm_batch* b) { return b->eye_overlap && b->passes_per_graph == 0; }
 
 // Rebuilds the job array, the block maps and the graph after jobs were added or removed.
 int rebuild(ppm_batch* b) {
@@ -115,8 +144,9 @@ int rebuild(ppm_batch* b) {
         for (int k = 0; k < kMaps; k++)
             for (int i = 0; i < per[k]; i++) maps[k].push_back(BlockRef{j, i});
     }
-    // the previous tables may still be read by queued passes: hand them back to the arena, which
-    // reuses them only after the stream has passed this point
+    // the previous tables may still be read by queued passes (eye stages on `ahead` included): hand
+    // them back to the arena, which reuses them only after the stream has passed this point
+    join_ahead(b);
     if (b->d_jobs) b->arena->release(b->d_jobs, b->stream);
     for (int k = 0; k < kMaps; k++)
         if (b->d_map[k]) b->arena->release(b->d_map[k], b->stream);
@@ -125,6 +155,9 @@ int rebuild(ppm_batch* b) {
     retire_graph(b);
     b->n_jobs = (int)jobs.size();
     for (int k = 0; k < kMaps; k++) b->n_blocks[k] = (int)maps[k].size();
+    int shortest = 0x7fffffff;
+    for (const Job& j : jobs) shortest = std::min(shortest, j.ep.max_path);
+    b->lag = jobs.empty() ? 0 : std::min(shortest - 1, ppm_batch::kAheadRing - 1);
     if (b->n_jobs == 0) {
         b->dirty = false;
         return PPM_OK;
@@ -140,7 +173,7 @@ int rebuild(ppm_batch* b) {
     if (b->passes_per_graph > 0) {
         hipGraph_t g = nullptr;
         BHIP(b, hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
-        for (uint32_t k = 0; k < b->passes_per_graph; k++) queue_batch_pass(b, b->stream);
+        for (uint32_t k = 0; k < b->passes_per_graph; k++) queue_batch_pass(b, b->stream, false);
         BHIP(b, hipStreamEndCapture(b->stream, &g));
         const hipError_t e = hipGraphInstantiate(&b->graph, g, nullptr, nullptr, 0);
         (void)hipGraphDestroy(g);
@@ -255,6 +288,9 @@ int ppm_batch_create(const ppm_batch_config* cfg, ppm_batch** out) {
     if (e == hipSuccess) e = hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking);
     if (e == hipSuccess) e = hipStreamCreateWithFlags(&b->upload, hipStreamNonBlocking);
     if (e == hipSuccess) e = hipStreamCreateWithFlags(&b->side, hipStreamNonBlocking);
+    if (e == hipSuccess) e = hipStreamCreateWithFlags(&b->ahead, hipStreamNonBlocking);
+    for (int k = 0; k < ppm_batch::kAheadRing && e == hipSuccess; k++)
+        e = hipEventCreateWithFlags(&b->ring[k], hipEventDisableTiming);
     if (e == hipSuccess) e = hipEventCreateWithFlags(&b->ev_fork, hipEventDisableTiming);
     if (e == hipSuccess) e = hipEventCreateWithFlags(&b->ev_join, hipEventDisableTiming);
     if (e == hipSuccess) e = hipEventCreate(&b->ev0);
@@ -295,7 +331,9 @@ int ppm_batch_add(ppm_batch* b, const ppm_scene* scene, const ppm_params* params
 int ppm_batch_remove(ppm_batch* b, int32_t job) {
     ppm_ctx* c = job_ctx(b, job);
     if (!c) return berr(b, PPM_E_INVAL, "no such job");
-    // blocks return to the arena once the passes queued so far are done with them
+    // blocks return to the arena once the passes queued so far (eye stages on `ahead` included)
+    // are done with them
+    join_ahead(b);
     ppm_internal::release_context(c);
     delete c;
     b->slots[(size_t)job] = nullptr;
@@ -329,7 +367,8 @@ int ppm_batch_pass(ppm_batch* b, uint32_t npasses) {
             left -= b->passes_per_graph;
         }
     }
-    for (; left > 0; left--) queue_batch_pass(b, b->stream);
+    const bool run_ahead = ahead_mode(b);
+    for (; left > 0; left--) queue_batch_pass(b, b->stream, run_ahead);
     BHIP(b, hipGetLastError());
     BHIP(b, hipEventRecord(b->ev1, b->stream));
     b->timed = true;
@@ -338,6 +377,8 @@ int ppm_batch_pass(ppm_batch* b, uint32_t npasses) {
 
 int ppm_batch_sync(ppm_batch* b) {
     if (!b) return PPM_E_INVAL;
+    BHIP(b, hipSetDevice(b->device));
+    join_ahead(b);
     BHIP(b, hipStreamSynchronize(b->stream));
     reap(b, false);
     return PPM_OK;
@@ -355,6 +396,7 @@ int ppm_batch_progress(ppm_batch* b, int32_t job, uint64_t* frames, uint32_t* pa
     ppm_ctx* c = job_ctx(b, job);
     if (!c) return berr(b, PPM_E_INVAL, "no such job");
     BHIP(b, hipSetDevice(b->device));
+    join_ahead(b);
     BHIP(b, hipStreamSynchronize(b->stream));
     DevGlobals g;
     BHIP(b, hipMemcpy(&g, c->job.g, sizeof(g), hipMemcpyDeviceToHost));
@@ -366,6 +408,8 @@ int ppm_batch_progress(ppm_batch* b, int32_t job, uint64_t* frames, uint32_t* pa
 int ppm_batch_readback(ppm_batch* b, int32_t job, ppm_buf which, void* dst, size_t bytes) {
     ppm_ctx* c = job_ctx(b, job);
     if (!c) return berr(b, PPM_E_INVAL, "no such job");
+    BHIP(b, hipSetDevice(b->device));
+    join_ahead(b);
     const int rc = ppm_readback(c, which, dst, bytes);
     if (rc != PPM_OK) return berr(b, rc, c->err);
     return PPM_OK;
@@ -435,6 +479,9 @@ int ppm_batch_autotune(ppm_batch* b, uint32_t passes_per_trial, ppm_tuning* best
         if ((rc = ppm_batch_set_tuning(b, &t))) return rc;
         if ((rc = ppm_batch_pass(b, 1))) return rc;  // table upload and graph capture stay out of the timing
         if ((rc = ppm_batch_pass(b, passes_per_trial))) return rc;
+        // eye stages begun ahead inside the trial belong to it: the timed region ends after them
+        join_ahead(b);
+        BHIP(b, hipEventRecord(b->ev1, b->stream));
         const float v = ppm_batch_last_pass_ms(b);
         if (v < 0.0f) return berr(b, PPM_E_HIP, "trial timing failed");
         *ms = v / (float)passes_per_trial;
@@ -450,6 +497,7 @@ uint64_t ppm_state_footprint(const ppm_params* params) {
 void ppm_batch_destroy(ppm_batch* b) {
     if (!b) return;
     (void)hipSetDevice(b->device);
+    if (b->ahead) (void)hipStreamSynchronize(b->ahead);
     if (b->stream) (void)hipStreamSynchronize(b->stream);
     if (b->upload) (void)hipStreamSynchronize(b->upload);
     for (ppm_ctx* c : b->slots)
@@ -471,8 +519,11 @@ void ppm_batch_destroy(ppm_batch* b) {
     if (b->ev1) (void)hipEventDestroy(b->ev1);
     if (b->ev_fork) (void)hipEventDestroy(b->ev_fork);
     if (b->ev_join) (void)hipEventDestroy(b->ev_join);
+    for (int k = 0; k < ppm_batch::kAheadRing; k++)
+        if (b->ring[k]) (void)hipEventDestroy(b->ring[k]);
     if (b->stream) (void)hipStreamDestroy(b->stream);
     if (b->side) (void)hipStreamDestroy(b->side);
+    if (b->ahead) (void)hipStreamDestroy(b->ahead);
     if (b->upload) (void)hipStreamDestroy(b->upload);
     delete b;
 }

@@ -79,7 +79,7 @@ struct SceneDev {
 
 // What one launch sequence of the plain loop does (queue_pass): a photon pass, an eye pass that
 // runs its own eye stage, an eye pass whose eye stage ran ahead, and that ahead stage alone.
-enum PassKind { kPhotonPass = 0, kEyePass = 1, kEyePassReady = 2, kEyeAhead = 3, kPassKinds = 4 };
+enum PassKind { kPhotonPass = 0, kEyePass = 1, kEyePassReady = 2, kEyeStageAhead = 3, kPassKinds = 4 };
 
 struct ppm_ctx {
     int device = 0;
@@ -143,6 +143,15 @@ struct ppm_batch {
     hipStream_t stream = nullptr;   // passes
     hipStream_t side = nullptr;     // the eye stage of every pass, beside the photon stage
     hipEvent_t ev_fork = nullptr, ev_join = nullptr;
+    // plain launches with eye_overlap: eye stages begun ahead of their eye passes run on `ahead`;
+    // ring[p % kAheadRing] is recorded after the ahead launches of pass p, and pass p waits (before
+    // its photon stage) for those of pass p - lag (lag + 1 = the shortest eye cadence of the jobs)
+    hipStream_t ahead = nullptr;
+    static constexpr int kAheadRing = 32;
+    hipEvent_t ring[kAheadRing] = {};
+    uint32_t pass_no = 0;           // ahead-mode passes queued
+    uint32_t ring_from = 0;         // the first pass whose ring event may be waited for
+    int lag = 0;
     hipStream_t upload = nullptr;   // scene uploads and job initialisation
     ppm_internal::Arena* arena = nullptr;
     uint32_t passes_per_graph = 1;  // 0: plain launches

@@ -79,6 +79,13 @@ struct GridScalars {
     float pad;
 };
 
+// Launch modes of an eye stage (k_eye_begin, k_eye, k_grid): the stage of the eye pass in progress
+// (run before that pass's deposits), or the stage of the job's next eye pass, begun ahead of it.
+constexpr int kEyeNow = 1;
+constexpr int kEyeAhead = 2;
+// k_eye_begin's pass argument when the caller orders its ahead stages itself (the plain loop).
+constexpr unsigned int kAnyPass = 0xffffffffu;
+
 // Emitted-flag counters of the pass in progress: k_photon_stage adds each block's count to one of
 // kEmitShards counters (block b to shard b % kEmitShards), each in a 128-byte line of its own so
 // that no line takes many atomics; the next pass's begin folds them into NumPhotons.
@@ -105,9 +112,13 @@ struct DevGlobals {
     int order_tiles;                 // the tile count tile_order was built for (0: none yet)
     unsigned int eye_stages;         // eye stages begun (k_eye_begin)
     int eye_stage;                   // the eye stage in progress; it writes buffers eye_stage & 1
-    int eye_go;                      // an eye stage runs in this launch sequence (k_eye_begin)
+    int eye_go;                      // the stage in progress and its launches' mode (kEyeNow or
+                                     // kEyeAhead; 0: none), set by k_eye_begin, cleared by k_grid
     int defer;                       // this pass's deposits wait for k_hash (its grid is not set up)
     unsigned int eye_stages_done;    // eye stages whose grid is set up (k_grid)
+    unsigned int begin_pass;         // the launch sequence's pass number at this pass's k_pass_begin
+    int ahead_frames;                // passes from this one to the next eye pass (k_pass_begin)
+    int ahead_ok;                    // the job's budget reaches that eye pass (k_pass_begin)
     int pad1;
     // emitted flags of the last begun pass, not yet in num_photons_f / photons_emitted
     unsigned int emit_shard[kEmitShards * kEmitStride];
@@ -228,17 +239,17 @@ struct BlockRef {
 // hash-clear map); k_eye_begin_batch runs one thread per job, k_grid_batch one block per job.
 // k_photon_stage runs with any block size up to kTraceBlockMax (a multiple of 64); the others with
 // 256 (1024 for k_grid).
-__global__ void k_pass_begin(Job J);
-__global__ void k_eye_begin(Job J, int ahead);
-__global__ void k_eye(Job J);
-__global__ void k_grid(Job J);
+__global__ void k_pass_begin(Job J, unsigned int pass);
+__global__ void k_eye_begin(Job J, int mode, unsigned int pass, int lag);
+__global__ void k_eye(Job J, int mode);
+__global__ void k_grid(Job J, int mode);
 __global__ void k_photon_stage(Job J);
 __global__ void k_hash(Job J);
 __global__ void k_gather(Job J);
-__global__ void k_pass_begin_batch(const Job* jobs, const BlockRef* map);
-__global__ void k_eye_begin_batch(const Job* jobs, int n_jobs);
-__global__ void k_eye_batch(const Job* jobs, const BlockRef* map);
-__global__ void k_grid_batch(const Job* jobs);
+__global__ void k_pass_begin_batch(const Job* jobs, const BlockRef* map, unsigned int pass);
+__global__ void k_eye_begin_batch(const Job* jobs, int n_jobs, int mode, unsigned int pass, int lag);
+__global__ void k_eye_batch(const Job* jobs, const BlockRef* map, int mode);
+__global__ void k_grid_batch(const Job* jobs, int mode);
 __global__ void k_photon_stage_batch(const Job* jobs, const BlockRef* map);
 __global__ void k_hash_batch(const Job* jobs, const BlockRef* map);
 __global__ void k_gather_batch(const Job* jobs, const BlockRef* map);

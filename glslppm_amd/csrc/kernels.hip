@@ -557,33 +557,53 @@ PPM_DEV float xs_frand(unsigned int* s) {
 // is 0, with two AAOffset draws from the job's XORShift stream and NumEyeSamples = count + 1;
 // FrameCount 0 is the first pass.  A job whose pass budget is spent stops advancing; the budget
 // itself is counted down by the pass's gather (k_pass_begin's other blocks read it meanwhile).
-PPM_DEV void pass_begin(const Job& J) {
+PPM_DEV void pass_begin(const Job& J, unsigned int pass) {
     DevGlobals* g = J.g;
-    if (g->passes_left == 0u) {
+    const unsigned int left = g->passes_left;
+    if (left == 0u) {
         g->active = 0;
         return;
     }
     const unsigned long long frame = g->frame;
-    const int eye = (frame % (unsigned long long)J.ep.max_path) == 0ull;
+    const unsigned long long L = (unsigned long long)J.ep.max_path;
+    const int eye = (frame % L) == 0ull;
     g->active = 1;
     g->eye_now = eye;
     g->first_pass = frame == 0ull;
     if (eye) g->eye_samples = g->eye_samples + 1ull;
     // the deposits of an eye pass wait for k_hash unless its eye stage has already set the grid up
     g->defer = eye && (unsigned long long)g->eye_stages_done < g->eye_samples;
+    // for an eye stage begun ahead in this pass: the passes until the next eye pass, and whether
+    // the budget lasts until then (left counts this pass)
+    const unsigned long long to_next = L - frame % L;
+    g->ahead_frames = (int)to_next;
+    g->ahead_ok = left == 0xffffffffu || (unsigned long long)(left - 1u) >= to_next;
+    g->begin_pass = pass;
     g->frame = frame + 1ull;
 }
 
 // The start of an eye stage (glslppm.cpp:238-287): the AAOffset draws of the job's XORShift
-// stream, NumEyeSamples, and the buffers the stage writes.  In an eye pass this is the stage of
-// that pass unless it has already begun; `ahead` begins the stage of the next eye pass instead
-// (after an eye pass, beside the photon passes that follow it).  Either way stages begin in eye
-// pass order and draw the same numbers, so the results do not depend on where a stage ran.
-PPM_DEV void eye_begin(const Job& J, int ahead) {
+// stream, NumEyeSamples, and the buffers the stage writes.  kEyeNow begins the stage of the eye pass
+// in progress unless it has already begun.  kEyeAhead begins the stage of the job's next eye pass
+// (after an eye pass, beside the photon passes that follow it) once the stage before it is done,
+// if the job's budget reaches that eye pass and, for a batch (pass != kAnyPass), only in the pass
+// whose k_pass_begin last ran (a launch that runs late leaves it to a later one) and with at least
+// `lag` passes left before that eye pass: the batch's passes wait for the ahead stages of the pass
+// `lag` passes back, which then covers this stage.  Stages begin in eye-pass order and draw the
+// same numbers wherever they run, so the results do not depend on where a stage ran.  A job has one
+// stage in progress at most: the two conditions exclude each other in any interleaving of their
+// launches, and eye_go (which tags the stage's launches) is written only here and by k_grid.
+PPM_DEV void eye_begin(const Job& J, int mode, unsigned int pass, int lag) {
     DevGlobals* g = J.g;
+    if (!g->active) return;
     const unsigned long long begun = g->eye_stages;
-    const int go = g->active && (ahead ? begun == g->eye_samples : g->eye_now && begun < g->eye_samples);
-    g->eye_go = go;
+    bool go;
+    if (mode == kEyeAhead) {
+        go = begun == g->eye_samples && (unsigned long long)g->eye_stages_done == begun && g->ahead_ok &&
+             (pass == kAnyPass || (g->begin_pass == pass && g->ahead_frames >= lag));
+    } else {
+        go = g->eye_now && begun < g->eye_samples;
+    }
     if (!go) return;
     unsigned int xs[4] = {g->xs[0], g->xs[1], g->xs[2], g->xs[3]};
     g->aa_x = (xs_frand(xs) - 0.5f) * 1.25f;
@@ -593,6 +613,7 @@ PPM_DEV void eye_begin(const Job& J, int ahead) {
     g->eye_stage = (int)e;
     g->num_eye_samples = (int)(e + 1u);
     g->eye_stages = e + 1u;
+    g->eye_go = mode;
 }
 
 // The buffer of the latest eye pass, which the passes after it read.
@@ -602,7 +623,7 @@ PPM_DEV int cur_buffer(const DevGlobals* g) { return (int)((g->eye_samples + 1ul
 // (glslppm.cpp:396-397, `NumPhotons += int(sum)` once per pass, in float), clears the counters and
 // sets the pass's uniforms (pass_begin); every block clears its hash slots (the glClear of
 // glslppm.cpp:421, 450).  Whether the pass runs is the budget left, which no block changes here.
-PPM_DEV void begin_and_clear(const Job& J, int blk) {
+PPM_DEV void begin_and_clear(const Job& J, int blk, unsigned int pass) {
     DevGlobals* g = J.g;
     if (blk == 0 && threadIdx.x < 64) {
         unsigned int* sh = g->emit_shard + threadIdx.x * kEmitStride;
@@ -611,7 +632,7 @@ PPM_DEV void begin_and_clear(const Job& J, int blk) {
         if (threadIdx.x == 0) {
             g->num_photons_f = g->num_photons_f + (float)(int)v;
             g->photons_emitted += v;
-            pass_begin(J);
+            pass_begin(J, pass);
         }
     }
     if (g->passes_left == 0u) return;
@@ -638,9 +659,9 @@ PPM_DEV float num_photons_now(const DevGlobals* g) {
 namespace {
 
 // eyeraytrace.fs:396-554 for one pixel, then the min/max of hit positions.
-PPM_DEV void eye(const Job& J, int blk) {
+PPM_DEV void eye(const Job& J, int blk, int mode) {
     const DevGlobals* g = J.g;
-    if (!g->eye_go) return;
+    if (g->eye_go != mode) return;
     const DevScene& s = J.s;
     const DevCamera& cam = J.cam;
     const EyeParams& ep = J.ep;
@@ -806,9 +827,9 @@ PPM_DEV void eye(const Job& J, int blk) {
 
 // The hit-point bounding box from the eye blocks' partials, then the grid scalars of
 // glslppm.cpp:294-311 (one block of 1024 threads).
-PPM_DEV void grid(const Job& J) {
+PPM_DEV void grid(const Job& J, int mode) {
     DevGlobals* g = J.g;
-    if (!g->eye_go) return;
+    if (g->eye_go != mode) return;
     GridScalars& gr = g->grid[g->eye_stage & 1];
     const EyeParams& ep = J.ep;
     const float* partial = J.hs.partial;
@@ -1333,24 +1354,36 @@ PPM_DEV void gather(const Job& J, int blk) {
         const BlockRef m = map[blockIdx.x];                                                         \
         body(jobs[m.job], m.block);                                                                 \
     }
-#define PPM_JOB_KERNEL1(name, body, threads)                                                      \
-    __global__ __launch_bounds__(threads) void name(Job J) { body(J); }                            \
-    __global__ __launch_bounds__(threads) void name##_batch(const Job* __restrict__ jobs) {        \
-        body(jobs[blockIdx.x]);                                                                     \
-    }
-
-PPM_JOB_KERNEL(k_pass_begin, begin_and_clear, kBlock)
-PPM_JOB_KERNEL_WAVES(k_eye, eye, kBlock, 6)
-
-__global__ void k_eye_begin(Job J, int ahead) {
-    if (threadIdx.x == 0) eye_begin(J, ahead);
+__global__ __launch_bounds__(kBlock) void k_pass_begin(Job J, unsigned int pass) {
+    begin_and_clear(J, (int)blockIdx.x, pass);
+}
+__global__ __launch_bounds__(kBlock) void k_pass_begin_batch(const Job* __restrict__ jobs,
+                                                             const BlockRef* __restrict__ map, unsigned int pass) {
+    const BlockRef m = map[blockIdx.x];
+    begin_and_clear(jobs[m.job], m.block, pass);
 }
 
-__global__ void k_eye_begin_batch(const Job* __restrict__ jobs, int n_jobs) {
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(6))) void k_eye(Job J, int mode) {
+    eye(J, (int)blockIdx.x, mode);
+}
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(6))) void k_eye_batch(
+    const Job* __restrict__ jobs, const BlockRef* __restrict__ map, int mode) {
+    const BlockRef m = map[blockIdx.x];
+    eye(jobs[m.job], m.block, mode);
+}
+
+__global__ void k_eye_begin(Job J, int mode, unsigned int pass, int lag) {
+    if (threadIdx.x == 0) eye_begin(J, mode, pass, lag);
+}
+__global__ void k_eye_begin_batch(const Job* __restrict__ jobs, int n_jobs, int mode, unsigned int pass, int lag) {
     const int j = blockIdx.x * blockDim.x + threadIdx.x;
-    if (j < n_jobs) eye_begin(jobs[j], 0);
+    if (j < n_jobs) eye_begin(jobs[j], mode, pass, lag);
 }
-PPM_JOB_KERNEL1(k_grid, grid, 1024)
+
+__global__ __launch_bounds__(1024) void k_grid(Job J, int mode) { grid(J, mode); }
+__global__ __launch_bounds__(1024) void k_grid_batch(const Job* __restrict__ jobs, int mode) {
+    grid(jobs[blockIdx.x], mode);
+}
 PPM_JOB_KERNEL_WAVES(k_photon_stage, photon_stage, kTraceBlockMax, 7)
 PPM_JOB_KERNEL(k_hash, hash_deferred, kBlock)
 PPM_JOB_KERNEL(k_gather, gather, kBlock)

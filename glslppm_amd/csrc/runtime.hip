@@ -442,9 +442,10 @@ int rep_limit(const ppm_params& p) {
 // and the grid set-up.  `ahead` begins the next eye pass's stage instead of this pass's.
 void queue_eye_stage(ppm_ctx* c, const Job& J, hipStream_t es, int ahead) {
     hipEvent_t e0 = prof_begin(c, PPM_KERNEL_EYE, es);
-    hipLaunchKernelGGL(k_eye_begin, dim3(1), dim3(64), 0, es, J, ahead);
-    hipLaunchKernelGGL(k_eye, dim3(J.tile_blocks), dim3(256), walk_lds_bytes(J.walk_stack, 256), es, J);
-    hipLaunchKernelGGL(k_grid, dim3(1), dim3(1024), 0, es, J);
+    const int mode = ahead ? kEyeAhead : kEyeNow;
+    hipLaunchKernelGGL(k_eye_begin, dim3(1), dim3(64), 0, es, J, mode, kAnyPass, 0);
+    hipLaunchKernelGGL(k_eye, dim3(J.tile_blocks), dim3(256), walk_lds_bytes(J.walk_stack, 256), es, J, mode);
+    hipLaunchKernelGGL(k_grid, dim3(1), dim3(1024), 0, es, J, mode);
     prof_end(c, PPM_KERNEL_EYE, e0, es);
 }
 
@@ -454,17 +455,17 @@ void queue_eye_stage(ppm_ctx* c, const Job& J, hipStream_t es, int ahead) {
 // stream beside the photon stage, whose deposits wait (kSlotPending) for k_hash after the join;
 // the deep eye paths, chains of dependent walks that keep few lanes busy, then share the GPU with
 // the photon walk.  An eye pass whose stage ran ahead (kEyePassReady) deposits at once.
-// kEyeAhead is the stage of the next eye pass alone, for the side stream after an eye pass: the
+// kEyeStageAhead is the stage of the next eye pass alone, for the side stream after an eye pass: the
 // passes in between read the other half of the double-buffered hit points.  `fork_ahead` starts
 // that stage from inside a photon pass instead, after the slot clear, beside the photon stage.
 void queue_pass(ppm_ctx* c, const Job& J, hipStream_t st, int kind, bool fork_ahead = false) {
-    if (kind == kEyeAhead) {
+    if (kind == kEyeStageAhead) {
         queue_eye_stage(c, J, st, 1);
         return;
     }
     const bool stage = kind == kEyePass;
     hipEvent_t e1 = prof_begin(c, PPM_KERNEL_CLEAR, st);
-    hipLaunchKernelGGL(k_pass_begin, dim3(J.clear_blocks), dim3(256), 0, st, J);
+    hipLaunchKernelGGL(k_pass_begin, dim3(J.clear_blocks), dim3(256), 0, st, J, 0u);
     prof_end(c, PPM_KERNEL_CLEAR, e1, st);
     const bool fork = stage && c->side && c->eye_overlap;
     hipStream_t es = fork ? c->side : st;
@@ -495,7 +496,7 @@ void queue_pass(ppm_ctx* c, const Job& J, hipStream_t st, int kind, bool fork_ah
 int capture_pass(ppm_ctx* c, int kind) {
     if (c->graph_pass[kind]) return PPM_OK;
     hipGraph_t g = nullptr;
-    hipStream_t st = kind == kEyeAhead ? c->side : c->stream;
+    hipStream_t st = kind == kEyeStageAhead ? c->side : c->stream;
     HIP_TRY(c, hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
     const bool prof = c->profiling;
     c->profiling = false;
@@ -522,14 +523,14 @@ int run_passes(ppm_ctx* c, const Job& J, uint32_t npasses, bool allow_graph) {
     if (graph) {
         int rc;
         for (int k = 0; k < kPassKinds; k++)
-            if ((k != kEyeAhead || c->side) && (rc = capture_pass(c, k))) return rc;
+            if ((k != kEyeStageAhead || c->side) && (rc = capture_pass(c, k))) return rc;
     }
     HIP_TRY(c, hipEventRecord(c->ev0, c->stream));
     auto launch_ahead = [&]() -> int {
         HIP_TRY(c, hipEventRecord(c->ev_fork, c->stream));
         HIP_TRY(c, hipStreamWaitEvent(c->side, c->ev_fork, 0));
-        if (graph) HIP_TRY(c, hipGraphLaunch(c->graph_pass[kEyeAhead], c->side));
-        else queue_pass(c, J, c->side, kEyeAhead);
+        if (graph) HIP_TRY(c, hipGraphLaunch(c->graph_pass[kEyeStageAhead], c->side));
+        else queue_pass(c, J, c->side, kEyeStageAhead);
         HIP_TRY(c, hipEventRecord(c->ev_ahead, c->side));
         c->ahead_due = false;
         c->ahead_pending = true;

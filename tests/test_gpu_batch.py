@@ -109,8 +109,11 @@ def test_batch_jobs_equal_alone(passes_per_graph, eye_overlap):
     b.close()
 
 
-def test_batch_budget_extension_and_reuse_of_freed_memory():
-    b = g.Batch(passes_per_graph=2)
+@pytest.mark.parametrize("passes_per_graph", [2, 0])
+def test_batch_budget_extension_and_reuse_of_freed_memory(passes_per_graph):
+    """passes_per_graph 0: plain launches, with each job's next eye stage begun ahead of its eye
+    pass; the budget runs out and is extended around an eye pass (L=6)."""
+    b = g.Batch(passes_per_graph=passes_per_graph)
     m, p = cornell(32), g.default_params(32, max_path_length=6)
     j0 = b.add(m, p, 3)
     b.run(5)

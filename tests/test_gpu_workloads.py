@@ -52,7 +52,10 @@ class JobView:
         return self.b.grid(self.j)
 
 
-def test_batch_of_uneven_scenes_equals_the_oracle_job_by_job():
+@pytest.mark.parametrize("passes_per_graph", [4, 0])
+def test_batch_of_uneven_scenes_equals_the_oracle_job_by_job(passes_per_graph):
+    """passes_per_graph 4: graph replay, each eye stage inside its eye pass; 0: plain launches,
+    each job's next eye stage begun ahead beside the photon passes before its eye pass."""
     rs = np.random.RandomState(0x67736C80)
     jobs = []
     for k in range(20):
@@ -63,8 +66,8 @@ def test_batch_of_uneven_scenes_equals_the_oracle_job_by_job():
         jobs.append((m, p, int(rs.randint(16, 41))))
     tris = [m.num_triangles for (m, _, _) in jobs]
     print("jobs %d, triangles %d-%d" % (len(jobs), min(tris), max(tris)), flush=True)
-    b = g.Batch(passes_per_graph=4)
-    b.set_tuning(trace_block=128, tile_wave_width=16)
+    b = g.Batch(passes_per_graph=passes_per_graph)
+    b.set_tuning(trace_block=128, tile_wave_width=16, passes_per_graph=passes_per_graph)
     ids = [b.add(m, p, n) for (m, p, n) in jobs]
     most = max(n for (_, _, n) in jobs)
     b.run(most + 3)  # every budget runs out; the last passes leave finished jobs untouched
