@@ -43,7 +43,7 @@ WORKLOADS = {
     "spheres-1024": dict(kind=1, seed=0x67736C70, count=40, detail=4, res=1024, L=10, camera=((0, 0, 1.667), (0, 0, 0), 45.0),
                          cpu_passes=50, desc="Cornell room + 40 icospheres (subdiv 4): Q=P=H=1024^2, 10 bounces, alpha 0.7"),
     "cornell-256": dict(kind=0, seed=0, count=0, detail=0, res=256, L=8, camera=((0, 0, 1.667), (0, 0, 0), 45.0),
-                        fixed_radius=0.02, cpu_passes=20,
+                        fixed_radius=0.02, cpu_passes=1500,
                         desc="Cornell room, 36 triangles: Q=P=H=256^2, 8 bounces, radius 1/50 of the room"),
     "terrain-c2": dict(kind=2, seed=0x67736C72, count=8, detail=1024, res=2048, L=12,
                        camera=((0.0, 0.55, 1.9), (0.0, -0.25, 0.0), 50.0), image=(1920, 1080), hash=(4096, 4096),

@@ -1286,10 +1286,32 @@ PPM_DEV void gather(const Job& J, int blk) {
         f3 flux = mk3(0.0f, 0.0f, 0.0f);
         float m = 0.0f;
         const int clamp_slot = (h.height - 1) * h.width;
+        // Each cell's box is (cell / GridScale + BBoxMin, (cell + 1) / GridScale + BBoxMin)
+        // (progressive.fs:79-80); along an axis the two cells' boxes share their middle bound, so the
+        // block of eight has three bounds per axis, computed once with the same operations.
+        const f3 c1 = mk3(ii.x + 1.0f, ii.y + 1.0f, ii.z + 1.0f);
+        const f3 c2 = mk3(c1.x + 1.0f, c1.y + 1.0f, c1.z + 1.0f);
+        const f3 k0 = mk3(ii.x + 0.0f, ii.y + 0.0f, ii.z + 0.0f) / gs + bbmin;
+        const f3 k1 = c1 / gs + bbmin;
+        const f3 k2 = c2 / gs + bbmin;
+        // A cell whose box lies at least R from the hit point holds no photon the test below can
+        // accept, so it is neither hashed nor read, and the sums stay bit-identical.  The test
+        // accepts a photon p strictly inside the box with length(p - q) < R.  Per axis, p > rmin
+        // and p < rmax give |fl(p - q)| >= g = max(fl(rmin - q), fl(q - rmax), 0), since rounding
+        // is monotone; squares, sums and sqrt of non-negative numbers are monotone too, so the
+        // shader's length(p - q) = sqrt((dx dx + dy dy) + dz dz) is at least the same expression
+        // of the gaps.  That expression >= R therefore rules out every photon of the cell.  (A NaN
+        // anywhere makes the comparison false: nothing is skipped.)
         for (int iz = 0; iz <= 1; iz++)
             for (int iy = 0; iy <= 1; iy++)
                 for (int ix = 0; ix <= 1; ix++) {
                     const f3 cell = mk3(ii.x + (float)ix, ii.y + (float)iy, ii.z + (float)iz);
+                    const f3 rmin = mk3(ix ? k1.x : k0.x, iy ? k1.y : k0.y, iz ? k1.z : k0.z);
+                    const f3 rmax = mk3(ix ? k2.x : k1.x, iy ? k2.y : k1.y, iz ? k2.z : k1.z);
+                    const f3 gap = mk3(fmaxf(fmaxf(rmin.x - qpos.x, qpos.x - rmax.x), 0.0f),
+                                       fmaxf(fmaxf(rmin.y - qpos.y, qpos.y - rmax.y), 0.0f),
+                                       fmaxf(fmaxf(rmin.z - qpos.z, qpos.z - rmax.z), 0.0f));
+                    if (length3(gap) >= R) continue;
                     const float sf = cell_hash(cell, gs, hsc, h.num_f);
                     // texel fetch clamps a slot past the last row to the start of that row; a
                     // NaN slot (zero-extent grid) clamps to texel (0, 0)
@@ -1300,8 +1322,6 @@ PPM_DEV void gather(const Job& J, int blk) {
                     unsigned int rep = rec.y;
                     if (rep == 0xffffffffu) rep = 0u;  // cleared slot reads photon texel (0, 0)
                     const f3 ppos = xyz(ps.pos[rep]);
-                    const f3 rmin = cell / gs + bbmin;
-                    const f3 rmax = (cell + mk3(1.0f, 1.0f, 1.0f)) / gs + bbmin;
                     if (ppos.x > rmin.x && ppos.y > rmin.y && ppos.z > rmin.z && ppos.x < rmax.x &&
                         ppos.y < rmax.y && ppos.z < rmax.z) {
                         const f3 diff = ppos - qpos;
