@@ -59,6 +59,7 @@ struct PackedScene {
     int T = 0, N = 0, M = 0, NL = 0;
     float light_area = 0.0f;
     float bsphere[4] = {0, 0, 0, 0};
+    float slab_eps = 0x1p-126f;  // DevScene::slab_eps
     ppm_camera camera;
     size_t device_bytes() const;
 };

@@ -54,7 +54,9 @@ struct DevScene {
     int tex_size;
     int num_materials;
     float material_rcp;
-    int pad_tex;
+    // 2^-22 times the largest node bound magnitude capped at 2^17 (at least 2^-126): scales the
+    // slack of the four-wide record test's fused slab values
+    float slab_eps;
 };
 
 struct DevCamera {

@@ -99,79 +99,138 @@ PPM_DEV float mul_half_hi(unsigned int w, float r) {
     return d;
 }
 
-// A ray's slab terms (rcp, t0) in the axis order of its face's node records: the major axis m =
-// face / 2, then the other two in xyz order; sh_a / sh_b rotate a record word so that its low half
-// is the near bound on that axis (16 when the direction is negative there).
+// float(binary16 in the low / high half of w) * r + t with one rounding (a fused multiply-add).
+PPM_DEV float fma_half_lo(unsigned int w, float r, float t) {
+    float d;
+    asm("v_fma_mix_f32 %0, %1, %2, %3 op_sel_hi:[1,0,0]" : "=v"(d) : "v"(w), "v"(r), "v"(t));
+    return d;
+}
+PPM_DEV float fma_half_hi(unsigned int w, float r, float t) {
+    float d;
+    asm("v_fma_mix_f32 %0, %1, %2, %3 op_sel:[1,0,0] op_sel_hi:[1,0,0]" : "=v"(d) : "v"(w), "v"(r), "v"(t));
+    return d;
+}
+
+// The reference's slab offset of one axis, t0 = -rcp * org (eyeraytrace.fs:88,
+// photontrace.fs:144).
+PPM_DEV float slab_t0(float rcp, float org) { return -rcp * org; }
+
+// The axes of a face's node records: the major axis m = face / 2, then the other two in xyz order.
+PPM_DEV f3 face_axes(int face, f3 v) {
+    const int m = face >> 1;
+    return mk3(m == 0 ? v.x : m == 1 ? v.y : v.z, m == 0 ? v.y : v.x, m == 2 ? v.y : v.z);
+}
+
+// The record tests serve a ray whose reciprocals lie in [2^-100, 2^100] and whose offsets in
+// [-2^120, 2^120] in magnitude (unit directions always do unless parallel to an axis).
+PPM_DEV bool sign_order_ok(f3 r, f3 t) {
+    const float am = fabsf(r.x), aa = fabsf(r.y), ab = fabsf(r.z);
+    return am >= 0x1p-100f && am <= 0x1p100f && aa >= 0x1p-100f && aa <= 0x1p100f && ab >= 0x1p-100f &&
+           ab <= 0x1p100f && fabsf(t.x) <= 0x1p120f && fabsf(t.y) <= 0x1p120f && fabsf(t.z) <= 0x1p120f;
+}
+
+// The slack of a record test's fused slab values on one axis (wide_test).
+PPM_DEV float slab_slack(float r, float t, float slab_eps) {
+    return __builtin_fmaf(fabsf(r), slab_eps, __builtin_fmaf(fabsf(t), 0x1p-22f, 0x1p-140f));
+}
+
+// A ray's reciprocals in the axis order of its face's node records; sh_a / sh_b rotate a record
+// word so that its low half is the near bound on that axis (16 when the direction is negative
+// there).
 struct WideRay {
-    float rm, tm, ra, ta, rb, tb;
+    float rm, ra, rb;
     unsigned int sh_a, sh_b;
 };
 
-PPM_DEV WideRay wide_ray(int face, f3 rcp, f3 t0) {
+PPM_DEV WideRay wide_ray(int face, f3 rcp) {
+    const f3 r = face_axes(face, rcp);
     WideRay w;
-    const int m = face >> 1;
-    w.rm = m == 0 ? rcp.x : m == 1 ? rcp.y : rcp.z;
-    w.tm = m == 0 ? t0.x : m == 1 ? t0.y : t0.z;
-    w.ra = m == 0 ? rcp.y : rcp.x;
-    w.ta = m == 0 ? t0.y : t0.x;
-    w.rb = m == 2 ? rcp.y : rcp.z;
-    w.tb = m == 2 ? t0.y : t0.z;
+    w.rm = r.x;
+    w.ra = r.y;
+    w.rb = r.z;
     w.sh_a = signbit(w.ra) ? 16u : 0u;
     w.sh_b = signbit(w.rb) ? 16u : 0u;
     return w;
 }
 
+// The offsets of the record tests for the near and far bound of each axis: t0 -+ e for a ray
+// the four-wide record tests serve (sign_order_ok, see wide_test), exactly t0 for any other.
+struct SlabOffsets {
+    float nm, fm, na, fa, nb, fb;
+};
+
+PPM_DEV SlabOffsets slab_offsets(const WideRay& w, int face, f3 org, float slab_eps, bool fused) {
+    const f3 o = face_axes(face, org);
+    const float tm = slab_t0(w.rm, o.x), ta = slab_t0(w.ra, o.y), tb = slab_t0(w.rb, o.z);
+    const float em = fused ? slab_slack(w.rm, tm, slab_eps) : 0.0f;
+    const float ea = fused ? slab_slack(w.ra, ta, slab_eps) : 0.0f;
+    const float eb = fused ? slab_slack(w.rb, tb, slab_eps) : 0.0f;
+    return {tm - em, tm + em, ta - ea, ta + ea, tb - eb, tb + eb};
+}
+
+PPM_DEV bool ray_fused(const WideRay& w, int face, f3 org) {
+    const f3 o = face_axes(face, org);
+    return sign_order_ok(mk3(w.rm, w.ra, w.rb), mk3(slab_t0(w.rm, o.x), slab_t0(w.ra, o.y), slab_t0(w.rb, o.z)));
+}
+
 // The reference's slab test of an exact fp32 box (eyeraytrace.fs:92-116, photontrace.fs:148-172:
 // per axis rcp * bound + t0, min/max, then tmin <= tmax, tmin <= closest hit, tmax >= 0) with the
-// slab terms in the face's axis order: the same products, sums and per-axis min/max; the axes
-// enter tmin and tmax in another order, which NaN-rejecting min and max do not see (up to the sign
-// of a zero, which no comparison sees).
-PPM_DEV bool box_test_face(const WideRay& w, int face, f3 a, f3 b, float ht) {
-    const int m = face >> 1;
-    const float am = m == 0 ? a.x : m == 1 ? a.y : a.z, bm = m == 0 ? b.x : m == 1 ? b.y : b.z;
-    const float aa = m == 0 ? a.y : a.x, ba = m == 0 ? b.y : b.x;
-    const float ab = m == 2 ? a.y : a.z, bb = m == 2 ? b.y : b.z;
-    const float lm = w.rm * am + w.tm, hm = w.rm * bm + w.tm;
-    const float la = w.ra * aa + w.ta, ha = w.ra * ba + w.ta;
-    const float lb = w.rb * ab + w.tb, hb = w.rb * bb + w.tb;
+// slab terms in the face's axis order: the same products, offsets, sums and per-axis min/max; the
+// axes enter tmin and tmax in another order, which NaN-rejecting min and max do not see (up to the
+// sign of a zero, which no comparison sees).
+PPM_DEV bool box_test_face(const WideRay& w, int face, f3 org, f3 a, f3 b, float ht) {
+    const f3 o = face_axes(face, org), lo = face_axes(face, a), hi = face_axes(face, b);
+    const float tm = slab_t0(w.rm, o.x), ta = slab_t0(w.ra, o.y), tb = slab_t0(w.rb, o.z);
+    const float lm = w.rm * lo.x + tm, hm = w.rm * hi.x + tm;
+    const float la = w.ra * lo.y + ta, ha = w.ra * hi.y + ta;
+    const float lb = w.rb * lo.z + tb, hb = w.rb * hi.z + tb;
     const float tmin = max1(max1(min1(lm, hm), min1(la, ha)), min1(lb, hb));
     const float tmax = min1(min1(max1(lm, hm), max1(la, ha)), max1(lb, hb));
     return (tmin <= tmax) && (tmin <= ht) && (tmax >= 0.0f);
 }
 
 // Slab test of a pre-order node record's outward-rounded binary16 box (axes in the face's order,
-// each as lo | hi << 16 or hi | lo << 16), in box_test_face's arithmetic: per-axis min/max whatever
-// the signs, so it passes exactly where the reference's test passes on the same bounds.
-PPM_DEV bool node_test(uint4 r, const WideRay& w, float ht) {
-    const float lm = mul_half_lo(r.x, w.rm) + w.tm, hm = mul_half_hi(r.x, w.rm) + w.tm;
-    const float la = mul_half_lo(r.y, w.ra) + w.ta, ha = mul_half_hi(r.y, w.ra) + w.ta;
-    const float lb = mul_half_lo(r.z, w.rb) + w.tb, hb = mul_half_hi(r.z, w.rb) + w.tb;
+// each as lo | hi << 16 or hi | lo << 16, rotated so that the low half is the near bound when the
+// direction is negative on a minor axis), in box_test_face's arithmetic with the near and far
+// offsets: per-axis min/max whatever the signs.  For a ray the record tests do not serve both
+// offsets are t0 (up to the sign of a zero t0 + 0, which no comparison sees) and this is the
+// reference's test on the rounded bounds; for one they serve, the near offset is <= t0 <= the far
+// one, which only moves each value outward.
+PPM_DEV bool node_test(uint4 r, const WideRay& w, const SlabOffsets& o, float ht) {
+    const unsigned int a = __builtin_rotateright32(r.y, w.sh_a), b = __builtin_rotateright32(r.z, w.sh_b);
+    const float lm = mul_half_lo(r.x, w.rm) + o.nm, hm = mul_half_hi(r.x, w.rm) + o.fm;
+    const float la = mul_half_lo(a, w.ra) + o.na, ha = mul_half_hi(a, w.ra) + o.fa;
+    const float lb = mul_half_lo(b, w.rb) + o.nb, hb = mul_half_hi(b, w.rb) + o.fb;
     const float tmin = max1(max1(min1(lm, hm), min1(la, ha)), min1(lb, hb));
     const float tmax = min1(min1(max1(lm, hm), max1(la, ha)), max1(lb, hb));
     return (tmin <= tmax) && (tmin <= ht) && (tmax >= 0.0f);
 }
 
-// Slab test of one member slot of a four-wide record.  The near and far bounds are taken by the
-// direction's signs instead of by min/max.  The walk uses it only for rays whose six slab terms are
-// finite with nonzero reciprocals (sign_order_ok): then every slab value is a number (a binary16
-// bound may be infinite, never a product of zero and infinity), the two values of an axis are
-// ordered by the sign of its reciprocal, and this is exactly node_test's min/max.  (The order in
-// which the axes enter tmin and tmax does not matter: NaN-rejecting min and max of a set do not
-// depend on it, up to the sign of a zero, which no comparison here sees.)  Other rays, those
-// parallel to an axis above all, keep the reference's NaN behaviour on the pre-order records:
-// there, taking bounds by sign would leave a NaN slab out where min/max rejects the box, and such
-// a ray would enter almost every box.
-PPM_DEV bool sign_order_ok(const WideRay& w) {
-    const float m = fminf(fminf(fabsf(w.rm), fabsf(w.ra)), fabsf(w.rb));
-    return m != 0.0f && __builtin_isfinite(w.rm) && __builtin_isfinite(w.ra) && __builtin_isfinite(w.rb) &&
-           __builtin_isfinite(w.tm) && __builtin_isfinite(w.ta) && __builtin_isfinite(w.tb);
-}
-
-PPM_DEV bool wide_test(uint4 r, const WideRay& w, float ht) {
+// Slab test of one member slot of a four-wide record, for rays that sign_order_ok accepts.  The
+// near and far bounds are taken by the direction's signs instead of by min/max, and each slab
+// value is one fused multiply-add, fma(b', rcp, t0 -+ e) on the binary16 bound b', with the
+// offset moved outward by the slack e = |rcp| slab_eps + 2^-22 |t0| + 2^-140 (fp32 fmas, to
+// nearest).  No slab value is NaN (a bound of magnitude over 65504 may pack as an infinity, whose
+// product is infinite and exact; every other product and sum stays finite) and the two values of
+// an axis are ordered by the sign of its reciprocal.  The fused value is never inward of the
+// reference's rcp * b + t0 (two roundings) on the exact bound b: b' is b rounded outward, so
+// rcp b' is on the outer side of rcp b; rounding rcp b errs by at most 2^-24 |rcp| B + 2^-150,
+// where B is the largest bound magnitude capped at 2^17 and slab_eps = 2^-22 B (a bound beyond
+// 2^17 packs as an infinity or as +-65504, which moves rcp b' outward by more than half of
+// |rcp b|, far more than that product's rounding error); rounding t0 -+ e errs by at most
+// 2^-24 (|t0| + e) + 2^-150; and e, rounded twice, is at least
+// 3.99 (2^-24 |rcp| B + 2^-24 |t0|) + 2^-141, more than all of these together.  Rounding to
+// nearest is monotone, so tmin is no larger and tmax no smaller than the reference's test on the
+// exact box.  (The order in which the axes enter tmin and tmax does not matter: NaN-rejecting min
+// and max of a set do not depend on it, up to the sign of a zero, which no comparison here sees.)
+// Other rays, those parallel to an axis above all, keep the reference's arithmetic and NaN
+// behaviour on the pre-order records: there, taking bounds by sign would leave a NaN slab out where
+// min/max rejects the box, and such a ray would enter almost every box.
+PPM_DEV bool wide_test(uint4 r, const WideRay& w, const SlabOffsets& o, float ht) {
     const unsigned int a = __builtin_rotateright32(r.y, w.sh_a), b = __builtin_rotateright32(r.z, w.sh_b);
-    const float nm = mul_half_lo(r.x, w.rm) + w.tm, fm = mul_half_hi(r.x, w.rm) + w.tm;
-    const float na = mul_half_lo(a, w.ra) + w.ta, fa = mul_half_hi(a, w.ra) + w.ta;
-    const float nb = mul_half_lo(b, w.rb) + w.tb, fb = mul_half_hi(b, w.rb) + w.tb;
+    const float nm = fma_half_lo(r.x, w.rm, o.nm), fm = fma_half_hi(r.x, w.rm, o.fm);
+    const float na = fma_half_lo(a, w.ra, o.na), fa = fma_half_hi(a, w.ra, o.fa);
+    const float nb = fma_half_lo(b, w.rb, o.nb), fb = fma_half_hi(b, w.rb, o.fb);
     const float tmin = max1(max1(nm, na), nb);
     const float tmax = min1(min1(fm, fa), fb);
     return (tmin <= tmax) && (tmin <= ht) && (tmax >= 0.0f);
@@ -202,7 +261,7 @@ extern __shared__ int walk_lds[];
 // Box-phase boxes are binary16 boxes rounded outward.  A slab value rcp * b + t0 is monotone in
 // b, and rounding outward keeps the sign of each bound (which decides the NaN axes of zero
 // direction components), so a rounded box passes whenever the exact one does, with a tmin no
-// larger.  Boxes are tested against a closest-hit distance no smaller than the reference's at
+// larger (the four-wide records' fused slab values with their slack too, see wide_test).  Boxes are tested against a closest-hit distance no smaller than the reference's at
 // the same point of the pre-order (members are tested before their turn, and leaves queued
 // earlier may still be untested).  So the lane may enter boxes the reference skips, never skips
 // one the reference enters, and reaches the reference's leaves in the reference's order.  A leaf
@@ -215,7 +274,7 @@ extern __shared__ int walk_lds[];
 constexpr int kDone = (int)0x80000000;  // no record or leaf left (never ~triangle: T < 2^31 - 1)
 
 template <bool PHOTON>
-PPM_DEV void walk(const DevScene& s, int face, f3 org, f3 dir, f3 rcp_in, f3 t0_in, bool cull, Hit& hit, int depth) {
+PPM_DEV void walk(const DevScene& s, int face, f3 org, f3 dir, f3 rcp_in, bool cull, Hit& hit, int depth) {
     Closest h = {hit.tri, hit.t};
     const int n = s.num_nodes;
     const uint4* __restrict__ nd = s.nodes + (size_t)face * (size_t)(n + 1);
@@ -223,17 +282,23 @@ PPM_DEV void walk(const DevScene& s, int face, f3 org, f3 dir, f3 rcp_in, f3 t0_
     // the lane's stack: entry k at walk_lds[k * blockDim.x + lane]; top points past the last one
     int* top = walk_lds + threadIdx.x;
     const int stride = blockDim.x;
-    const WideRay wr4 = wide_ray(face, rcp_in, t0_in);
+    const WideRay wr4 = wide_ray(face, rcp_in);
+    const bool fused = ray_fused(wr4, face, org);
     int cur = 0;  // >= 0: the record to visit; else ~triangle of a leaf to queue, or kDone
     int sp = 0;   // stack entries
     // >= 0: the lane walks the pre-order records from position i (from the start for a ray the
     // record tests do not cover)
-    int i = sign_order_ok(wr4) ? -1 : 0;
+    int i = fused ? -1 : 0;
     // queued leaves, newest first: q0, then q1 when two are queued
     int q0 = -1, q1 = -1;
     int nq = 0;
     for (;;) {
         const int live = __popcll(__ballot(cur != kDone));  // every queue is empty here
+        // the slab offsets, recomputed for each box phase: held through the leaf phase, whose
+        // two prefetched leaves are the walk's register peak, they would spill
+        f3 ob = org;
+        asm volatile("" : "+v"(ob.x), "+v"(ob.y), "+v"(ob.z));
+        const SlabOffsets so = slab_offsets(wr4, face, ob, s.slab_eps, fused);
         for (;;) {
             const bool can = cur != kDone && nq < kLeafQueue;
             if (__ballot(can) == 0ull) break;
@@ -241,7 +306,7 @@ PPM_DEV void walk(const DevScene& s, int face, f3 org, f3 dir, f3 rcp_in, f3 t0_
             if (can) {
                 if (i >= 0) {
                     const uint4 r0 = nd[i], r1 = nd[i + 1];
-                    bool hit = node_test(r0, wr4, h.t);
+                    bool hit = node_test(r0, wr4, so, h.t);
                     int link = (int)r0.w;
                     bool cont = hit || link < 0;
                     if (hit && link < 0) {
@@ -251,7 +316,7 @@ PPM_DEV void walk(const DevScene& s, int face, f3 org, f3 dir, f3 rcp_in, f3 t0_
                     }
                     int next = cont ? i + 1 : link;
                     if (cont && i + 1 < n && nq < kLeafQueue) {
-                        hit = node_test(r1, wr4, h.t);
+                        hit = node_test(r1, wr4, so, h.t);
                         link = (int)r1.w;
                         cont = hit || link < 0;
                         if (hit && link < 0) {
@@ -271,8 +336,8 @@ PPM_DEV void walk(const DevScene& s, int face, f3 org, f3 dir, f3 rcp_in, f3 t0_
                     unsigned int wa = a.w, wb = b.w, wc = c.w, wd = d.w;
                     asm volatile("" : "+v"(wa), "+v"(wb), "+v"(wc), "+v"(wd));
                     // an empty slot's infinite near bound fails it (see the record packing)
-                    const bool ha = wide_test(a, wr4, h.t), hb = wide_test(b, wr4, h.t);
-                    const bool hc = wide_test(c, wr4, h.t), hd = wide_test(d, wr4, h.t);
+                    const bool ha = wide_test(a, wr4, so, h.t), hb = wide_test(b, wr4, so, h.t);
+                    const bool hc = wide_test(c, wr4, so, h.t), hd = wide_test(d, wr4, so, h.t);
                     const int nh = (int)ha + (int)hb + (int)hc + (int)hd;
                     if (sp + nh - 1 > depth) {
                         const uint4 ps = s.wide_pos[(size_t)face * (size_t)s.wide_stride + cur];
@@ -359,9 +424,9 @@ PPM_DEV void walk(const DevScene& s, int face, f3 org, f3 dir, f3 rcp_in, f3 t0_
             y3 = T[3];
         }
         // the leaf's exact box: (x1.w, x2.w, x3.x) to (x3.y, x3.z, x3.w)
-        if (nq > 0 && box_test_face(wr4, face, mk3(x1.w, x2.w, x3.x), mk3(x3.y, x3.z, x3.w), h.t))
+        if (nq > 0 && box_test_face(wr4, face, org, mk3(x1.w, x2.w, x3.x), mk3(x3.y, x3.z, x3.w), h.t))
             tri_test<PHOTON>(x0, x1, x2, org, dir, cull, qa, h);
-        if (nq > 1 && box_test_face(wr4, face, mk3(y1.w, y2.w, y3.x), mk3(y3.y, y3.z, y3.w), h.t))
+        if (nq > 1 && box_test_face(wr4, face, org, mk3(y1.w, y2.w, y3.x), mk3(y3.y, y3.z, y3.w), h.t))
             tri_test<PHOTON>(y0, y1, y2, org, dir, cull, qb, h);
         nq = 0;
     }
@@ -373,12 +438,11 @@ PPM_DEV void walk(const DevScene& s, int face, f3 org, f3 dir, f3 rcp_in, f3 t0_
 template <bool PHOTON>
 PPM_DEV Hit trace(const DevScene& s, f3 org, f3 dir, bool cull, int depth) {
     const f3 rcp = mk3(1.0f / dir.x, 1.0f / dir.y, 1.0f / dir.z);
-    const f3 t0 = mk3(-rcp.x * org.x, -rcp.y * org.y, -rcp.z * org.z);
     Hit h;
     h.tri = -1;
     h.t = PHOTON ? 1.0e30f : 1.0e20f;
     h.bx = h.by = h.bz = 0.0f;
-    walk<PHOTON>(s, cube_face(dir), org, dir, rcp, t0, cull, h, depth);
+    walk<PHOTON>(s, cube_face(dir), org, dir, rcp, cull, h, depth);
     return h;
 }
 
@@ -1126,7 +1190,7 @@ PPM_DEV bool photon_gen(const DevScene& s, const PhotonParams& pp, const PhotonS
 }
 
 struct Walk {
-    f3 org, dir, rcp, t0;
+    f3 org, dir, rcp;
     int face;
     Hit h;
 };
@@ -1211,12 +1275,11 @@ PPM_DEV void photon_stage(const Job& J, int blk) {
     W.org = org;
     W.dir = dir;
     W.rcp = mk3(1.0f / dir.x, 1.0f / dir.y, 1.0f / dir.z);
-    W.t0 = mk3(-W.rcp.x * org.x, -W.rcp.y * org.y, -W.rcp.z * org.z);
     W.face = cube_face(dir);
     W.h.tri = -1;
     W.h.t = 1.0e30f;
     W.h.bx = W.h.by = W.h.bz = 0.0f;
-    walk<true>(s, W.face, W.org, W.dir, W.rcp, W.t0, false, W.h, J.walk_stack);
+    walk<true>(s, W.face, W.org, W.dir, W.rcp, false, W.h, J.walk_stack);
     photon_finish(s, pp, J.ps, J.h, g, p, W, flux, out_bits);
 }
 

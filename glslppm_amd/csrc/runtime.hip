@@ -773,6 +773,14 @@ int pack_scene(const ppm_scene* s, PackedScene* out) {
     out->light_cdf.assign(s->light_cdf, s->light_cdf + out->NL);
     out->light_area = s->light_area;
     for (int k = 0; k < 4; k++) out->bsphere[k] = s->bsphere[k];
+    // the largest node bound magnitude, capped at 2^17: a bound beyond it packs as an infinity or
+    // as +-65504, more than half its magnitude inward of it (outward for the box)
+    float bmax = 0.0f;
+    for (size_t k = 0; k < (size_t)s->num_nodes * 6; k++) {
+        const float a = std::fabs(s->node_bounds[k]);
+        if (a > bmax) bmax = std::fmin(a, 0x1p17f);
+    }
+    out->slab_eps = std::fmax(std::ldexp(bmax, -22), 0x1p-126f);
     out->camera = s->camera;
     out->tex.clear();
     out->volume.clear();
@@ -854,6 +862,7 @@ int upload_scene(const PackedScene& ps, Arena* arena, hipStream_t up, hipStream_
     sc.tex_size = ps.tex_size;
     sc.num_materials = ps.M;
     sc.material_rcp = 1.0f / (float)ps.M;
+    sc.slab_eps = ps.slab_eps;
     *out = sd;
     return PPM_OK;
 }
