@@ -101,9 +101,10 @@ def amdahl_bound(plain_s, plain_by_res, work_by_res):
     the mix (1024^2 here, a million photons per pass) already keep the whole GPU busy in the plain
     loop, so their photon segments per second there is the rate the GPU sustains on these scenes;
     batching hides the small jobs' idle launch gaps but must still do their work, so the batch takes
-    at least (all photon segments) / (that rate)."""
+    at least (all photon segments) / (that rate).  None when no job is that large: smaller jobs do
+    not fill the GPU alone, so their plain-loop rate bounds nothing."""
     big = max(plain_by_res)
-    if plain_by_res[big] <= 0 or work_by_res.get(big, 0) <= 0:
+    if int(big) < 1024 or plain_by_res[big] <= 0 or work_by_res.get(big, 0) <= 0:
         return None
     rate = work_by_res[big] / plain_by_res[big]
     return plain_s / (sum(work_by_res.values()) / rate)
