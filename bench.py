@@ -34,10 +34,12 @@ HBM_PEAK_GBS = 8000.0
 #   hit point per pass (k_gather, with the draw fused): gather and update 80 B (60 read, 20
 #     written) + draw 40 B = 120 B
 #   eye ray (eye stage, every L passes): 104 B
-#   hash slot (k_pass_begin): clear plus read of count and representative, 16 B
-BYTES_PER = {"photon": 216.0, "gather": 120.0, "eye": 104.0, "clear": 16.0}
+#   hash slot: clear plus read of count and representative, 16 B.  The photon stage clears the
+#   next pass's table (8 B per slot, written); the gather reads the slots it needs.
+BYTES_PER = {"photon": 216.0, "gather": 120.0, "eye": 104.0, "clear": 0.0}
+SLOT_BYTES_PER = {"photon": 8.0, "gather": 0.0, "eye": 0.0, "clear": 0.0}
 KERNEL_NAMES = {"photon": "photon stage (k_photon_stage)", "gather": "k_gather",
-                "eye": "eye stage (k_eye_begin + k_eye + k_grid)", "clear": "k_pass_begin (pass begin and hash clear)"}
+                "eye": "eye stage (k_eye_begin + k_eye + k_grid)", "clear": "k_pass_begin (the pass's uniforms)"}
 
 WORKLOADS = {
     "spheres-1024": dict(kind=1, seed=0x67736C70, count=40, detail=4, res=1024, L=10, camera=((0, 0, 1.667), (0, 0, 0), 45.0),
@@ -448,7 +450,7 @@ def main():
     # (over the stage passes with graph replay)
     dom_ms, dom_n = stages[dom] if args.graph else prof[dom]
     units = {"photon": P, "gather": Q, "eye": Q, "clear": H}[dom]
-    bytes_per_launch = BYTES_PER[dom] * units
+    bytes_per_launch = BYTES_PER[dom] * units + SLOT_BYTES_PER[dom] * H
     avg_ms = dom_ms / max(dom_n, 1)
     achieved = bytes_per_launch / (avg_ms * 1e-3) / 1e9
     traffic = None

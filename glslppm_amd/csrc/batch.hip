@@ -42,7 +42,7 @@ int berr(ppm_batch* b, int code, const std::string& msg) {
                         std::string(#call) + ": " + hipGetErrorString(e_));                  \
     } while (0)
 
-enum { kMapTile = 0, kMapPhoton = 1, kMapClear = 2, kMapTrace = 3, kMaps = 4 };
+enum { kMapTile = 0, kMapPhoton = 1, kMapBegin = 2, kMapTrace = 3, kMaps = 4 };
 
 // Destroys retired graphs whose last launch has finished (all of them when `wait`).
 void reap(ppm_batch* b, bool wait) {
@@ -86,8 +86,8 @@ void queue_batch_pass(ppm_batch* b, hipStream_t st, bool run_ahead) {
     const int n = b->n_jobs;
     const Job* J = b->d_jobs;
     const unsigned int pass = run_ahead ? b->pass_no : 0u;
-    hipLaunchKernelGGL(k_pass_begin_batch, dim3(b->n_blocks[kMapClear]), dim3(256), 0, st, J,
-                       (const BlockRef*)b->d_map[kMapClear], pass);
+    hipLaunchKernelGGL(k_pass_begin_batch, dim3(b->n_blocks[kMapBegin]), dim3(256), 0, st, J,
+                       (const BlockRef*)b->d_map[kMapBegin], pass);
     hipStream_t es = b->eye_overlap ? b->side : st;
     if (b->eye_overlap) {
         (void)hipEventRecord(b->ev_fork, st);
@@ -140,7 +140,7 @@ int rebuild(ppm_batch* b) {
         const int j = (int)jobs.size();
         jobs.push_back(c->job);
         b->table_ids.push_back((int32_t)id);
-        const int per[kMaps] = {c->job.tile_blocks, c->job.photon_blocks, c->job.clear_blocks, c->job.trace_blocks};
+        const int per[kMaps] = {c->job.tile_blocks, c->job.photon_blocks, c->job.begin_blocks, c->job.trace_blocks};
         for (int k = 0; k < kMaps; k++)
             for (int i = 0; i < per[k]; i++) maps[k].push_back(BlockRef{j, i});
     }

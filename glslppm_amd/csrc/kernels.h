@@ -160,8 +160,10 @@ struct PhotonState {
 };
 
 struct HashState {
-    unsigned int* slots;   // [H][2]: count, representative (0xffffffff = none), one 8-byte record
-                           // per slot so that a gather reads one line per cell
+    unsigned int* slots;   // [2][H][2]: count, representative (0xffffffff = none), one 8-byte record
+                           // per slot so that a gather reads one line per cell; two tables, used by
+                           // alternate passes, so that a pass's photon stage clears the table of the
+                           // next pass beside its walks (hash_table, next_table)
     int size;          // H
     int width;         // hash_width (power of two)
     int height;
@@ -225,7 +227,7 @@ struct Job {
     DevGlobals* g;
     int tile_blocks;     // pixel tiles (k_eye, k_gather)
     int photon_blocks;   // 256 photon slots per block (k_hash)
-    int clear_blocks;    // 256 hash slots per block (k_pass_begin)
+    int begin_blocks;    // blocks of k_pass_begin (one)
     int trace_blocks;    // photon slots per block = the launch's block size (k_photon_stage)
     int walk_stack;      // stack entries per lane of the walks (LDS: walk_stack x block x 4 bytes)
 };

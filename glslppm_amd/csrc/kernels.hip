@@ -1,12 +1,11 @@
 // Kernels of the SPPM pass loop for gfx950.  One pass (m_display, glslppm.cpp:214-536) is:
-//   k_pass_begin    frame count, eye cadence, NumPhotons of the last pass (block 0) and the clear
-//                   of the hash slots (every block)
+//   k_pass_begin    frame count, eye cadence, NumPhotons of the last pass (one block)
 //   k_eye_begin, k_eye, k_grid  every max_path passes: AAOffset, eyeraytrace.fs + per-block
 //                   max.fs/min.fs partials, the hit-point box and the grid uniforms of
 //                   glslppm.cpp:292-311 (one block)
 //   k_photon_stage  photontrace.fs emission/bounce, walk and deposit + hash.fs + scatter (atomic
 //                   min of the photon index) + correction (atomic count), one thread per slot;
-//                   sum.fs as per-block counts
+//                   sum.fs as per-wave counts; the clear of the next pass's hash table
 //   k_gather   progressive.fs (8-cell gather and the N/R/tau update) + re.fs draw to RGBA8
 // Arithmetic follows the GLSL statement by statement in strict fp32 (see fmath.h).
 //
@@ -602,6 +601,7 @@ __global__ void k_init_state(Job J) {
     }
     if (i < J.h.size) {
         reinterpret_cast<uint2*>(J.h.slots)[i] = make_uint2(0u, 0xffffffffu);
+        reinterpret_cast<uint2*>(J.h.slots)[J.h.size + i] = make_uint2(0u, 0xffffffffu);
     }
 }
 
@@ -683,11 +683,11 @@ PPM_DEV void eye_begin(const Job& J, int mode, unsigned int pass, int lag) {
 // The buffer of the latest eye pass, which the passes after it read.
 PPM_DEV int cur_buffer(const DevGlobals* g) { return (int)((g->eye_samples + 1ull) & 1ull); }
 
-// The start of a pass: block 0 folds the last pass's emitted counts into NumPhotons
-// (glslppm.cpp:396-397, `NumPhotons += int(sum)` once per pass, in float), clears the counters and
-// sets the pass's uniforms (pass_begin); every block clears its hash slots (the glClear of
-// glslppm.cpp:421, 450).  Whether the pass runs is the budget left, which no block changes here.
-PPM_DEV void begin_and_clear(const Job& J, int blk, unsigned int pass) {
+// The start of a pass (one block): its first wave folds the last pass's emitted counts into
+// NumPhotons (glslppm.cpp:396-397, `NumPhotons += int(sum)` once per pass, in float), clears the
+// counters and sets the pass's uniforms (pass_begin).  The pass's hash table was cleared by the
+// photon stage of the pass before (the glClear of glslppm.cpp:421, 450; see next_table).
+PPM_DEV void begin_pass(const Job& J, int blk, unsigned int pass) {
     DevGlobals* g = J.g;
     if (blk == 0 && threadIdx.x < 64) {
         unsigned int* sh = g->emit_shard + threadIdx.x * kEmitStride;
@@ -699,11 +699,18 @@ PPM_DEV void begin_and_clear(const Job& J, int blk, unsigned int pass) {
             pass_begin(J, pass);
         }
     }
-    if (g->passes_left == 0u) return;
-    const int i = blk * kBlock + threadIdx.x;
-    if (i < J.h.size) {
-        reinterpret_cast<uint2*>(J.h.slots)[i] = make_uint2(0u, 0xffffffffu);
-    }
+}
+
+// The hash table of the pass in progress (after its k_pass_begin, which counted it in `frame`):
+// pass f (0-based) uses table f & 1.  The readback of the hash takes the same table after the
+// last pass.
+PPM_DEV unsigned int* hash_table(const HashState& h, const DevGlobals* g) {
+    return h.slots + (size_t)2 * (size_t)h.size * (size_t)((g->frame + 1ull) & 1ull);
+}
+// The table of the next pass, which the photon stage of the pass in progress clears.  The pass
+// before used it, and its gather ran before this pass's k_pass_begin on the same stream.
+PPM_DEV unsigned int* next_table(const HashState& h, const DevGlobals* g) {
+    return h.slots + (size_t)2 * (size_t)h.size * (size_t)(g->frame & 1ull);
 }
 
 // NumPhotons after the pass in progress: the folded count plus the pass's counters (every
@@ -1204,8 +1211,9 @@ PPM_DEV int deposit(const PhotonParams& pp, const HashState& h, const DevGlobals
     const float sf = cell_hash(cell, gr.grid_scale, gr.hash_scale, h.num_f);
     if (!(sf < h.num_f)) return -1;
     const int slot = (int)sf;
-    atomicAdd(h.slots + 2 * slot, 1u);
-    if (p < pp.rep_limit) atomicMin(h.slots + 2 * slot + 1, (unsigned int)p);
+    unsigned int* t = hash_table(h, g);
+    atomicAdd(t + 2 * slot, 1u);
+    if (p < pp.rep_limit) atomicMin(t + 2 * slot + 1, (unsigned int)p);
     return slot;
 }
 
@@ -1260,6 +1268,13 @@ PPM_DEV void photon_stage(const Job& J, int blk) {
     if (!g->active) return;
     const DevScene& s = J.s;
     const PhotonParams& pp = J.pp;
+    // the next pass's hash table, cleared beside this pass's walks
+    {
+        uint2* nt = reinterpret_cast<uint2*>(next_table(J.h, g));
+        const int stride = J.trace_blocks * (int)blockDim.x;
+        for (int i = blk * (int)blockDim.x + (int)threadIdx.x; i < J.h.size; i += stride)
+            nt[i] = make_uint2(0u, 0xffffffffu);
+    }
     const int p = blk * (int)blockDim.x + (int)threadIdx.x;
     unsigned int emitted = 0;
     bool live = false;
@@ -1349,6 +1364,7 @@ PPM_DEV void gather(const Job& J, int blk) {
         f3 flux = mk3(0.0f, 0.0f, 0.0f);
         float m = 0.0f;
         const int clamp_slot = (h.height - 1) * h.width;
+        const unsigned int* table = hash_table(h, g);
         // Each cell's box is (cell / GridScale + BBoxMin, (cell + 1) / GridScale + BBoxMin)
         // (progressive.fs:79-80); along an axis the two cells' boxes share their middle bound, so the
         // block of eight has three bounds per axis, computed once with the same operations.
@@ -1379,7 +1395,7 @@ PPM_DEV void gather(const Job& J, int blk) {
                     // texel fetch clamps a slot past the last row to the start of that row; a
                     // NaN slot (zero-extent grid) clamps to texel (0, 0)
                     const int slot = sf < h.num_f ? (int)sf : (sf >= h.num_f ? clamp_slot : 0);
-                    const uint2 rec = reinterpret_cast<const uint2*>(h.slots)[slot];
+                    const uint2 rec = reinterpret_cast<const uint2*>(table)[slot];
                     const unsigned int cnt = rec.x;
                     if (cnt == 0u) continue;  // adds exactly +0 to flux and count
                     unsigned int rep = rec.y;
@@ -1438,12 +1454,12 @@ PPM_DEV void gather(const Job& J, int blk) {
         body(jobs[m.job], m.block);                                                                 \
     }
 __global__ __launch_bounds__(kBlock) void k_pass_begin(Job J, unsigned int pass) {
-    begin_and_clear(J, (int)blockIdx.x, pass);
+    begin_pass(J, (int)blockIdx.x, pass);
 }
 __global__ __launch_bounds__(kBlock) void k_pass_begin_batch(const Job* __restrict__ jobs,
                                                              const BlockRef* __restrict__ map, unsigned int pass) {
     const BlockRef m = map[blockIdx.x];
-    begin_and_clear(jobs[m.job], m.block, pass);
+    begin_pass(jobs[m.job], m.block, pass);
 }
 
 __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(6))) void k_eye(Job J, int mode) {

@@ -457,7 +457,7 @@ void queue_eye_stage(ppm_ctx* c, const Job& J, hipStream_t es, int ahead) {
 // the photon walk.  An eye pass whose stage ran ahead (kEyePassReady) deposits at once.
 // kEyeStageAhead is the stage of the next eye pass alone, for the side stream after an eye pass: the
 // passes in between read the other half of the double-buffered hit points.  `fork_ahead` starts
-// that stage from inside a photon pass instead, after the slot clear, beside the photon stage.
+// that stage from inside a photon pass instead, after k_pass_begin, beside the photon stage.
 void queue_pass(ppm_ctx* c, const Job& J, hipStream_t st, int kind, bool fork_ahead = false) {
     if (kind == kEyeStageAhead) {
         queue_eye_stage(c, J, st, 1);
@@ -465,7 +465,7 @@ void queue_pass(ppm_ctx* c, const Job& J, hipStream_t st, int kind, bool fork_ah
     }
     const bool stage = kind == kEyePass;
     hipEvent_t e1 = prof_begin(c, PPM_KERNEL_CLEAR, st);
-    hipLaunchKernelGGL(k_pass_begin, dim3(J.clear_blocks), dim3(256), 0, st, J, 0u);
+    hipLaunchKernelGGL(k_pass_begin, dim3(J.begin_blocks), dim3(256), 0, st, J, 0u);
     prof_end(c, PPM_KERNEL_CLEAR, e1, st);
     const bool fork = stage && c->side && c->eye_overlap;
     hipStream_t es = fork ? c->side : st;
@@ -709,9 +709,6 @@ void set_launch_shape(ppm_ctx* c, int trace_block, int wave_shift, int walk_stac
         }
 }
 int photon_blocks(const ppm_params* p) { return (p->photon_width * p->photon_height + 255) / 256; }
-int clear_blocks(const ppm_params* p) {
-    return (p->hash_width * p->hash_height + 255) / 256;
-}
 
 size_t state_footprint(const ppm_params* p) {
     const size_t Q = (size_t)p->image_width * p->image_height, P = (size_t)p->photon_width * p->photon_height;
@@ -723,8 +720,8 @@ size_t state_footprint(const ppm_params* p) {
         (size_t)max_tile_blocks(p) * 4, (size_t)max_tile_blocks(p) * 4,  // tile costs and order
         // photons: pos, flux, dir, rng, state, slot
         P * 16, P * 16, P * 16, P * 16, P * 8, P * 4,
-        // hash (count, representative) records; globals; resolve targets
-        H * 8, sizeof(DevGlobals), Q * 16, Q * 16};
+        // hash (count, representative) records, two tables; globals; resolve targets
+        H * 16, sizeof(DevGlobals), Q * 16, Q * 16};
     size_t tot = 0;
     for (size_t b : sizes) tot += Arena::round_size(b ? b : 16);
     return tot;
@@ -927,7 +924,7 @@ int create_context(ppm_ctx* c, std::shared_ptr<SceneDev> scene, const ppm_camera
     J.tl.width = p->image_width;
     J.tl.height = p->image_height;
     J.photon_blocks = photon_blocks(p);
-    J.clear_blocks = clear_blocks(p);
+    J.begin_blocks = 1;
     // one-wave blocks for the photon walk (measured: 1% faster than 256 at 1024^2, 3% on the 2.26M
     // triangle terrain), 8x8-pixel waves; the tuner may change them
     set_launch_shape(c, kTraceBlockDefault, 3, kWalkStack);
@@ -950,7 +947,7 @@ int create_context(ppm_ctx* c, std::shared_ptr<SceneDev> scene, const ppm_camera
         (rc = dev_alloc(c, &ps.flux, P)) || (rc = dev_alloc(c, &ps.dir, P)) || (rc = dev_alloc(c, &ps.rng, P)) ||
         (rc = dev_alloc(c, &ps.state, P)) || (rc = dev_alloc(c, &ps.slot, P)) ||
 
-        (rc = dev_alloc(c, &J.h.slots, 2 * H)) || (rc = dev_alloc(c, &J.g, 1)) ||
+        (rc = dev_alloc(c, &J.h.slots, 4 * H)) || (rc = dev_alloc(c, &J.g, 1)) ||
         (rc = dev_alloc(c, &c->resolve_lin, Q)) || (rc = dev_alloc(c, &c->resolve_map, Q)))
         return rc;
     J.h.size = c->H;
@@ -1225,8 +1222,10 @@ int ppm_readback(ppm_ctx* c, ppm_buf which, void* dst, size_t bytes) {
         case PPM_BUF_PHOTON_SLOT: return d2h(J.ps.slot, P * 4);
         case PPM_BUF_SLOT_COUNT:
         case PPM_BUF_SLOT_REP: {
+            // the table of the last pass that ran (passes alternate between the two)
             std::vector<uint32_t> rec(2 * H);
-            HIP_TRY(c, hipMemcpy(rec.data(), J.h.slots, H * 8, hipMemcpyDeviceToHost));
+            const size_t table = (size_t)((g.frame + 1ull) & 1ull);
+            HIP_TRY(c, hipMemcpy(rec.data(), J.h.slots + table * 2 * H, H * 8, hipMemcpyDeviceToHost));
             uint32_t* o = (uint32_t*)dst;
             const size_t k = which == PPM_BUF_SLOT_COUNT ? 0 : 1;
             for (size_t i = 0; i < H; i++) o[i] = rec[2 * i + k];

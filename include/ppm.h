@@ -211,7 +211,9 @@ float ppm_last_pass_ms(ppm_ctx* ctx);
  * waits for the streams and returns the accumulated milliseconds and launch counts per kernel kind.
  * No counterpart in the reference (it times whole frames with glutGet, glslppm.cpp:526-535). */
 #define PPM_KERNEL_EYE 0      /* eyeraytrace.fs + max/min */
-#define PPM_KERNEL_CLEAR 1    /* glClear of the hash and correction targets */
+#define PPM_KERNEL_CLEAR 1    /* k_pass_begin: the pass's uniforms (the glClear of the hash and
+                                 correction targets runs inside the photon stage of the pass
+                                 before, on the other of two hash tables) */
 #define PPM_KERNEL_PHOTON 2   /* photontrace.fs + sum + hash + scatter + correction */
 #define PPM_KERNEL_GATHER 3   /* progressive.fs + re.fs */
 #define PPM_KERNEL_KINDS 4
