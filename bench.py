@@ -204,7 +204,8 @@ def run_serve_workload(args):
     t = time.time()
     pool = batch_jobs("batch-mix64")
     build_s = time.time() - t
-    srv = g.Server(hbm_budget=args.hbm_budget << 30, passes_per_step=args.passes_per_step)
+    srv = g.Server(hbm_budget=args.hbm_budget << 30, passes_per_step=args.passes_per_step,
+                   passes_per_graph=args.passes_per_graph, max_jobs=args.max_jobs)
     sids = [srv.add_scene(m) for (m, _, _) in pool]
     rs = np.random.RandomState(0x5E2E)
     n_jobs = int(rs.poisson(args.rate * args.serve_seconds))
@@ -304,6 +305,7 @@ def run_serve_workload(args):
         "data": "synthetic (seeded scenes, cameras, arrivals and targets)",
         "config": {"workload": SERVE_DESC.replace("RATE", str(args.rate)).replace("SECONDS", str(args.serve_seconds)),
                    "jobs": n_jobs, "hbm_budget_gib": args.hbm_budget, "passes_per_step": args.passes_per_step,
+                   "passes_per_graph": args.passes_per_graph, "max_jobs": args.max_jobs,
                    "scene_build_s": round(build_s, 2)},
         "time_to_first_pass_s": {"p50": pct(first, 50), "p99": pct(first, 99), "max": float(first.max())},
         "time_to_target_s": {"p50": pct(target_t, 50), "p99": pct(target_t, 99), "max": float(target_t.max())},
@@ -364,9 +366,11 @@ def main():
     ap.add_argument("--rate", type=float, default=20.0, help="serve: job arrivals per second")
     ap.add_argument("--serve-seconds", type=float, default=20.0, help="serve: length of the arrival window")
     ap.add_argument("--hbm-budget", type=int, default=200, help="serve: device memory budget in GiB")
-    ap.add_argument("--passes-per-step", type=int, default=4, help="serve: batched passes between admissions")
+    ap.add_argument("--passes-per-step", type=int, default=8, help="serve: batched passes between admissions")
     ap.add_argument("--plain-sample", type=int, default=40, help="serve: jobs timed in the plain loop")
-    ap.add_argument("--passes-per-graph", type=int, default=1, help="batch workloads: passes per hipGraph")
+    ap.add_argument("--max-jobs", type=int, default=256, help="serve: resident jobs at most")
+    ap.add_argument("--passes-per-graph", type=int, default=0,
+                    help="batch and serve workloads: passes per hipGraph (0: plain launches)")
     ap.add_argument("--graph", action="store_true",
                     help="plain workloads: replay passes as hipGraphs (the timed region then carries no timing "
                          "events; the roofline's launch time comes from the stage passes before it)")

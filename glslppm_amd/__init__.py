@@ -528,7 +528,7 @@ class Batch:
     Each job is what a Renderer holds; after n of its passes its buffers equal those of a
     Renderer of the same mesh and parameters after run(n), bit for bit."""
 
-    def __init__(self, device=0, passes_per_graph=1):
+    def __init__(self, device=0, passes_per_graph=0):
         cfg = BatchConfig()
         library().ppm_batch_config_default(ctypes.byref(cfg))
         cfg.device = int(device)
@@ -646,7 +646,7 @@ class Server:
     A job's results equal those of a Renderer of the same scene, camera and parameters after
     run(target_passes), bit for bit."""
 
-    def __init__(self, device=0, hbm_budget=200 << 30, max_jobs=256, passes_per_step=4, passes_per_graph=1):
+    def __init__(self, device=0, hbm_budget=200 << 30, max_jobs=256, passes_per_step=8, passes_per_graph=0):
         cfg = ServerConfig()
         library().ppm_server_config_default(ctypes.byref(cfg))
         cfg.device, cfg.hbm_budget, cfg.max_jobs = int(device), int(hbm_budget), int(max_jobs)

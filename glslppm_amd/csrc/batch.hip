@@ -267,7 +267,7 @@ void ppm_batch_config_default(ppm_batch_config* cfg) {
     if (!cfg) return;
     std::memset(cfg, 0, sizeof(*cfg));
     cfg->device = 0;
-    cfg->passes_per_graph = 1;
+    cfg->passes_per_graph = 0;
 }
 
 int ppm_batch_create(const ppm_batch_config* cfg, ppm_batch** out) {

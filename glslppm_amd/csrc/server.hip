@@ -320,8 +320,8 @@ void ppm_server_config_default(ppm_server_config* cfg) {
     cfg->device = 0;
     cfg->hbm_budget = 200ull << 30;
     cfg->max_jobs = 256;
-    cfg->passes_per_step = 4;
-    cfg->passes_per_graph = 1;
+    cfg->passes_per_step = 8;
+    cfg->passes_per_graph = 0;
 }
 
 int ppm_server_create(const ppm_server_config* cfg, ppm_server** out) {

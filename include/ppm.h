@@ -257,8 +257,10 @@ const char* ppm_last_error(const ppm_ctx* ctx);
  * stage of m_display (glslppm.cpp:214-536).  Each job keeps its own frame count, eye cadence,
  * XORShift stream and pass budget on the device; a job whose budget is spent is skipped by the
  * kernels.  Every job's buffers after n of its passes are bit-identical to those of the same
- * scene and parameters after ppm_pass(ctx, n).  The launch sequence is captured as a hipGraph of
- * passes_per_graph passes and replayed; it is captured again after ppm_batch_add/remove.  Device
+ * scene and parameters after ppm_pass(ctx, n).  With plain launches (passes_per_graph 0, the
+ * default) each job's eye stage runs ahead of its eye pass on a second stream beside the photon
+ * passes; otherwise the launch sequence is captured as a hipGraph of passes_per_graph passes and
+ * replayed (captured again after ppm_batch_add/remove) and each eye stage runs in its eye pass.  Device
  * memory comes from a caching pool owned by the batch, so adding and removing jobs does not wait
  * for the passes in flight.  A batch is used by one thread at a time. */
 
@@ -266,7 +268,7 @@ typedef struct ppm_batch ppm_batch;
 
 typedef struct ppm_batch_config {
     int32_t device;            /* HIP device ordinal; every job's params.device must match */
-    uint32_t passes_per_graph; /* passes per captured hipGraph (1-64); 0: plain launches */
+    uint32_t passes_per_graph; /* passes per captured hipGraph (1-64); 0: plain launches (default) */
 } ppm_batch_config;
 
 #define PPM_PASSES_UNLIMITED 0xffffffffu
@@ -339,8 +341,8 @@ typedef struct ppm_server_config {
     int32_t device;
     uint64_t hbm_budget;       /* device bytes for job state and scenes (default 200 GB) */
     int32_t max_jobs;          /* resident jobs at most (default 256) */
-    uint32_t passes_per_step;  /* batched passes between admission points (default 4) */
-    uint32_t passes_per_graph; /* the batch's hipGraph length (default 1) */
+    uint32_t passes_per_step;  /* batched passes between admission points (default 8) */
+    uint32_t passes_per_graph; /* the batch's hipGraph length (default 0: plain launches) */
 } ppm_server_config;
 
 #define PPM_JOB_QUEUED 0

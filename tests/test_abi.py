@@ -146,7 +146,7 @@ def test_batch_create_without_gpu_reports_no_device():
         pytest.skip("a GPU is present")
     cfg = g.BatchConfig()
     g.library().ppm_batch_config_default(ctypes.byref(cfg))
-    assert cfg.passes_per_graph == 1
+    assert cfg.passes_per_graph == 0  # plain launches, the eye stages run ahead
     h = ctypes.c_void_p()
     assert g.library().ppm_batch_create(ctypes.byref(cfg), ctypes.byref(h)) == -5
     assert g.library().ppm_batch_last_error(None)
@@ -200,7 +200,8 @@ def test_null_server_calls_fail_cleanly():
 def test_server_config_defaults_and_no_device():
     cfg = g.ServerConfig()
     g.library().ppm_server_config_default(ctypes.byref(cfg))
-    assert cfg.hbm_budget == 200 << 30 and cfg.max_jobs == 256 and cfg.passes_per_step == 4
+    assert cfg.hbm_budget == 200 << 30 and cfg.max_jobs == 256 and cfg.passes_per_step == 8
+    assert cfg.passes_per_graph == 0
     bad = g.ServerConfig()
     g.library().ppm_server_config_default(ctypes.byref(bad))
     bad.passes_per_step = 0
