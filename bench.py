@@ -36,9 +36,14 @@ HBM_PEAK_GBS = 8000.0
 #   eye ray (eye stage, every L passes): 104 B
 #   hash slot: clear plus read of count and representative, 16 B.  The photon stage clears the
 #   next pass's table (8 B per slot, written); the gather reads the slots it needs.
+# The photon-stage launch of a pass also runs the gather of the pass before it (every launch of a
+# ppm_pass call but the first, whose pass before was gathered at the end of the call before), so
+# its bytes per launch are 216 P + 8 H + 120 Q x (launches with a gather) / launches; k_gather
+# runs the gather of the call's last pass.
 BYTES_PER = {"photon": 216.0, "gather": 120.0, "eye": 104.0, "clear": 0.0}
 SLOT_BYTES_PER = {"photon": 8.0, "gather": 0.0, "eye": 0.0, "clear": 0.0}
-KERNEL_NAMES = {"photon": "photon stage (k_photon_stage)", "gather": "k_gather",
+KERNEL_NAMES = {"photon": "photon stage with the gather of the pass before (k_photon_stage)",
+                "gather": "k_gather (the last pass of a call)",
                 "eye": "eye stage (k_eye_begin + k_eye + k_grid)", "clear": "k_pass_begin (the pass's uniforms)"}
 
 WORKLOADS = {
@@ -455,6 +460,9 @@ def main():
     dom_ms, dom_n = stages[dom] if args.graph else prof[dom]
     units = {"photon": P, "gather": Q, "eye": Q, "clear": H}[dom]
     bytes_per_launch = BYTES_PER[dom] * units + SLOT_BYTES_PER[dom] * H
+    if dom == "photon":
+        # one ppm_pass call of dom_n passes: all but its first launch carry a gather
+        bytes_per_launch += BYTES_PER["gather"] * Q * max(dom_n - 1, 0) / max(dom_n, 1)
     avg_ms = dom_ms / max(dom_n, 1)
     achieved = bytes_per_launch / (avg_ms * 1e-3) / 1e9
     traffic = None

@@ -73,7 +73,8 @@ void retire_graph(ppm_batch* b) {
     b->graph = nullptr;
 }
 
-// One batched pass over the current tables: m_display's stage order for every job.  With
+// One batched pass over the current tables: m_display's stage order for every job, each job's
+// gather of the pass before in the blocks the photon-stage launch runs after every trace block.  With
 // eye_overlap the eye stages run beside the photon stage: a job's stage that has not begun when its
 // eye pass comes runs on `side` and that pass's deposits wait for k_hash after the join (kEyeNow);
 // with `run_ahead` (plain launches), a job's next eye stage begins on `ahead` in the pass after
@@ -114,7 +115,13 @@ void queue_batch_pass(ppm_batch* b, hipStream_t st, bool run_ahead) {
     if (b->eye_overlap) (void)hipStreamWaitEvent(st, b->ev_join, 0);
     hipLaunchKernelGGL(k_hash_batch, dim3(b->n_blocks[kMapPhoton]), dim3(256), 0, st, J,
                        (const BlockRef*)b->d_map[kMapPhoton]);
-    hipLaunchKernelGGL(k_gather_batch, dim3(b->n_blocks[kMapTile]), dim3(256), 0, st, J, (const BlockRef*)b->d_map[kMapTile]);
+}
+
+// The gathers of the last batched pass of a sequence; every other pass's gathers run in the
+// photon-stage launch of the pass after it.
+void queue_batch_flush(ppm_batch* b, hipStream_t st) {
+    hipLaunchKernelGGL(k_gather_batch, dim3(b->n_blocks[kMapTile]), dim3(256), 0, st, (const Job*)b->d_jobs,
+                       (const BlockRef*)b->d_map[kMapTile]);
 }
 
 // Makes the pass stream wait for every eye stage queued on `ahead` so far, and starts a new ring
@@ -144,6 +151,11 @@ int rebuild(ppm_batch* b) {
         for (int k = 0; k < kMaps; k++)
             for (int i = 0; i < per[k]; i++) maps[k].push_back(BlockRef{j, i});
     }
+    // the photon-stage launch's gather blocks follow every job's trace blocks, so that they are
+    // dispatched last
+    for (size_t j = 0; j < jobs.size(); j++)
+        for (int i = 0; i < jobs[j].gather_blocks; i++)
+            maps[kMapTrace].push_back(BlockRef{(int)j, jobs[j].trace_blocks + i});
     // the previous tables may still be read by queued passes (eye stages on `ahead` included): hand
     // them back to the arena, which reuses them only after the stream has passed this point
     join_ahead(b);
@@ -369,6 +381,7 @@ int ppm_batch_pass(ppm_batch* b, uint32_t npasses) {
     }
     const bool run_ahead = ahead_mode(b);
     for (; left > 0; left--) queue_batch_pass(b, b->stream, run_ahead);
+    queue_batch_flush(b, b->stream);
     BHIP(b, hipGetLastError());
     BHIP(b, hipEventRecord(b->ev1, b->stream));
     b->timed = true;

@@ -226,6 +226,7 @@ const std::string& create_error();
 int tile_blocks(int width, int height, int wave_shift);
 int max_tile_blocks(const ppm_params* p);  // over every tile shape
 int photon_blocks(const ppm_params* p);
+int gather_blocks(int tiles, int block);
 // Checks a tuning and converts it to (trace block, wave shift); PPM_E_INVAL if out of range.
 int check_tuning(const ppm_tuning* t, int* wave_shift);
 // Sets the launch shape of a context's job (invalidates its captured graphs).

@@ -21,10 +21,13 @@
 //           volume[M][S][S] uchar4 RGB8 + pad of every material (S = 512)  only with textures
 //   hit points (Q): pos, refl, nrm, rng, (tau, R), (E, N) float4; tri int; (mat, t) float2;
 //                   per pixel tile: depth of its deepest eye path, the next eye pass's tile order
-//   photons (P):    pos, flux, dir, rng float4 (.w of pos/flux/dir = hit normal);
-//                   state int2 (triangle, bounce | emitted << 16); slot int
-//   hash (H):       (count u32, representative u32) per slot (0xffffffff = none); a photon's slot
-//                   is kSlotPending between an eye pass's photon trace and its k_hash
+//   photons (P):    pos, flux, dir [2][P] float4 (.w = hit normal; pass f writes half f & 1 from
+//                   half (f + 1) & 1, so the gather of pass f can read its representatives while
+//                   the photon stage of pass f + 1 runs), rng float4; state int2 (triangle,
+//                   bounce | emitted << 16); slot int
+//   hash (H):       three tables of (count u32, representative u32) per slot (0xffffffff = none);
+//                   pass f deposits into table f % 3; a photon's slot is kSlotPending between an
+//                   eye pass's photon trace and its k_hash
 #pragma once
 
 #include "fmath.h"
@@ -81,6 +84,26 @@ struct GridScalars {
     float pad;
 };
 
+// What the gather of a pass reads of that pass's uniforms: the gather of pass f runs in the launch
+// of pass f + 1's photon stage, after k_pass_begin of pass f + 1 has changed the live fields.
+// k_pass_begin of pass f writes snap[f & 1]; np (NumPhotons after pass f, for its draw) is filled
+// in by the k_pass_begin of pass f + 1, which folds pass f's emitted counts.
+struct GatherSnap {
+    int first_pass;  // pass f is frame 0
+    int eye_now;     // pass f traced eye rays: the gather copies the new emission into e_n
+    int cb;          // the hit-point buffer (and grid) of the latest eye pass at pass f
+    int table;       // pass f's hash table (0..2)
+    int half;        // the photon half pass f wrote (f & 1)
+    float np;        // NumPhotons after pass f
+    int pad[2];
+};
+
+// Gather modes: the gather of the pass before the one in progress, run by the gather blocks of the
+// photon-stage launch, or the gather of the last pass begun, run by k_gather at the end of a pass
+// sequence.
+constexpr int kGatherLagged = 1;
+constexpr int kGatherFlush = 2;
+
 // Launch modes of an eye stage (k_eye_begin, k_eye, k_grid): the stage of the eye pass in progress
 // (run before that pass's deposits), or the stage of the job's next eye pass, begun ahead of it.
 constexpr int kEyeNow = 1;
@@ -121,7 +144,12 @@ struct DevGlobals {
     unsigned int begin_pass;         // the launch sequence's pass number at this pass's k_pass_begin
     int ahead_frames;                // passes from this one to the next eye pass (k_pass_begin)
     int ahead_ok;                    // the job's budget reaches that eye pass (k_pass_begin)
-    int pad1;
+    int table;                       // this pass's hash table (frame f: f % 3)
+    int half;                        // the photon half this pass writes (f & 1)
+    int lag_pending;                 // the pass before this one is still to be gathered (k_pass_begin)
+    int lag_idx;                     // its snap index
+    unsigned long long gathered;     // `frame` when k_gather last ran: the pass before it is gathered
+    GatherSnap snap[2];              // per pass parity
     // emitted flags of the last begun pass, not yet in num_photons_f / photons_emitted
     unsigned int emit_shard[kEmitShards * kEmitStride];
 };
@@ -151,19 +179,20 @@ struct HitState {
 constexpr int kSlotPending = -2;
 
 struct PhotonState {
-    float4* pos;
-    float4* flux;
-    float4* dir;
+    float4* pos;   // [2][P]
+    float4* flux;  // [2][P]
+    float4* dir;   // [2][P]
     float4* rng;
     int2* state;
     int* slot;
 };
 
 struct HashState {
-    unsigned int* slots;   // [2][H][2]: count, representative (0xffffffff = none), one 8-byte record
-                           // per slot so that a gather reads one line per cell; two tables, used by
-                           // alternate passes, so that a pass's photon stage clears the table of the
-                           // next pass beside its walks (hash_table, next_table)
+    unsigned int* slots;   // [3][H][2]: count, representative (0xffffffff = none), one 8-byte record
+                           // per slot so that a gather reads one line per cell; three tables used in
+                           // turn: a pass's photon stage clears the table of the next pass beside its
+                           // walks while the gather of the pass before reads its own (hash_table,
+                           // next_table)
     int size;          // H
     int width;         // hash_width (power of two)
     int height;
@@ -229,6 +258,8 @@ struct Job {
     int photon_blocks;   // 256 photon slots per block (k_hash)
     int begin_blocks;    // blocks of k_pass_begin (one)
     int trace_blocks;    // photon slots per block = the launch's block size (k_photon_stage)
+    int gather_blocks;   // blocks of k_photon_stage after its trace blocks: the gather of the pass
+                         // before, one wave per quarter tile
     int walk_stack;      // stack entries per lane of the walks (LDS: walk_stack x block x 4 bytes)
 };
 
@@ -239,10 +270,12 @@ struct BlockRef {
 };
 
 // Kernels of one job: grid = that job's blocks.  The _batch forms take the job array and, for
-// the kernels with several blocks per job, the block map of that kernel kind (k_pass_begin the
-// hash-clear map); k_eye_begin_batch runs one thread per job, k_grid_batch one block per job.
-// k_photon_stage runs with any block size up to kTraceBlockMax (a multiple of 64); the others with
-// 256 (1024 for k_grid).
+// the kernels with several blocks per job, the block map of that kernel kind; k_eye_begin_batch
+// runs one thread per job, k_grid_batch one block per job.  k_photon_stage runs with any block size
+// up to kTraceBlockMax (a multiple of 64) over trace_blocks + gather_blocks blocks per job (its
+// batch map lists every job's trace blocks, then every job's gather blocks, so that the gathers are
+// dispatched last and fill the tail of the walks); the others with 256 (1024 for k_grid).  k_gather
+// is the gather of the last pass begun, launched once at the end of a pass sequence.
 __global__ void k_pass_begin(Job J, unsigned int pass);
 __global__ void k_eye_begin(Job J, int mode, unsigned int pass, int lag);
 __global__ void k_eye(Job J, int mode);

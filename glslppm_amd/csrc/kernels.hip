@@ -5,8 +5,10 @@
 //                   glslppm.cpp:292-311 (one block)
 //   k_photon_stage  photontrace.fs emission/bounce, walk and deposit + hash.fs + scatter (atomic
 //                   min of the photon index) + correction (atomic count), one thread per slot;
-//                   sum.fs as per-wave counts; the clear of the next pass's hash table
-//   k_gather   progressive.fs (8-cell gather and the N/R/tau update) + re.fs draw to RGBA8
+//                   sum.fs as per-wave counts; the clear of the next pass's hash table; then, in
+//                   blocks of its own dispatched after the photon blocks, the gather of the pass
+//                   before: progressive.fs (8-cell gather and the N/R/tau update) + re.fs draw
+//   k_gather   the same gather for the last pass of a sequence (launched once at its end)
 // Arithmetic follows the GLSL statement by statement in strict fp32 (see fmath.h).
 //
 // Each kernel is written once as a body over (job, block of that job).  The plain form takes the
@@ -570,11 +572,11 @@ PPM_DEV unsigned int wave_sum(unsigned int v) {
 constexpr int kBlock = 256;
 constexpr int kWaves = kBlock / 64;
 
-// Pixel of this thread: block = tile of 2x2 waves, wave = ww x wh pixels (ww = 2^wave_shift).
-PPM_DEV void tile_pixel(const Tiles& tl, int tile, int& i, int& j) {
+// Pixel of this thread: tile of 2x2 waves, wave (0..3) = ww x wh pixels (ww = 2^wave_shift).
+PPM_DEV void tile_pixel(const Tiles& tl, int tile, int wave, int& i, int& j) {
     const int sh = tl.wave_shift, ww = 1 << sh, wh = 64 >> sh;
     const int tx = tile % tl.tiles_x, ty = tile / tl.tiles_x;
-    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int lane = threadIdx.x & 63;
     i = tx * 2 * ww + (wave & 1) * ww + (lane & (ww - 1));
     j = ty * 2 * wh + (wave >> 1) * wh + (lane >> sh);
 }
@@ -593,16 +595,18 @@ __global__ void k_init_state(Job J) {
         J.hs.em[J.hs.q + i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
     }
     if (i < J.pp.count) {
-        J.ps.pos[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-        J.ps.flux[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-        J.ps.dir[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        for (int k = 0; k < 2; k++) {
+            const size_t o = (size_t)k * (size_t)J.pp.count + (size_t)i;
+            J.ps.pos[o] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+            J.ps.flux[o] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+            J.ps.dir[o] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        }
         J.ps.state[i] = make_int2(-1, 0);  // (-1, -1, 0, 1e30): every photon starts new
         J.ps.slot[i] = -1;
     }
-    if (i < J.h.size) {
-        reinterpret_cast<uint2*>(J.h.slots)[i] = make_uint2(0u, 0xffffffffu);
-        reinterpret_cast<uint2*>(J.h.slots)[J.h.size + i] = make_uint2(0u, 0xffffffffu);
-    }
+    if (i < J.h.size)
+        for (int k = 0; k < 3; k++)
+            reinterpret_cast<uint2*>(J.h.slots)[(size_t)k * (size_t)J.h.size + (size_t)i] = make_uint2(0u, 0xffffffffu);
 }
 
 namespace {
@@ -620,7 +624,8 @@ PPM_DEV float xs_frand(unsigned int* s) {
 // Per-pass uniforms of m_display (glslppm.cpp:214-287): eye pass when FrameCount % MaxPathLength
 // is 0, with two AAOffset draws from the job's XORShift stream and NumEyeSamples = count + 1;
 // FrameCount 0 is the first pass.  A job whose pass budget is spent stops advancing; the budget
-// itself is counted down by the pass's gather (k_pass_begin's other blocks read it meanwhile).
+// counts this pass.  The pass's hash table, photon half and the uniforms its gather reads (which
+// runs one launch later) are fixed here.
 PPM_DEV void pass_begin(const Job& J, unsigned int pass) {
     DevGlobals* g = J.g;
     const unsigned int left = g->passes_left;
@@ -628,6 +633,7 @@ PPM_DEV void pass_begin(const Job& J, unsigned int pass) {
         g->active = 0;
         return;
     }
+    if (left != 0xffffffffu) g->passes_left = left - 1u;
     const unsigned long long frame = g->frame;
     const unsigned long long L = (unsigned long long)J.ep.max_path;
     const int eye = (frame % L) == 0ull;
@@ -635,6 +641,15 @@ PPM_DEV void pass_begin(const Job& J, unsigned int pass) {
     g->eye_now = eye;
     g->first_pass = frame == 0ull;
     if (eye) g->eye_samples = g->eye_samples + 1ull;
+    const int half = (int)(frame & 1ull);
+    g->table = (int)(frame % 3ull);
+    g->half = half;
+    GatherSnap& sn = g->snap[half];
+    sn.first_pass = frame == 0ull;
+    sn.eye_now = eye;
+    sn.cb = (int)((g->eye_samples + 1ull) & 1ull);
+    sn.table = g->table;
+    sn.half = half;
     // the deposits of an eye pass wait for k_hash unless its eye stage has already set the grid up
     g->defer = eye && (unsigned long long)g->eye_stages_done < g->eye_samples;
     // for an eye stage begun ahead in this pass: the passes until the next eye pass, and whether
@@ -651,9 +666,11 @@ PPM_DEV void pass_begin(const Job& J, unsigned int pass) {
 // in progress unless it has already begun.  kEyeAhead begins the stage of the job's next eye pass
 // (after an eye pass, beside the photon passes that follow it) once the stage before it is done,
 // if the job's budget reaches that eye pass and, for a batch (pass != kAnyPass), only in the pass
-// whose k_pass_begin last ran (a launch that runs late leaves it to a later one) and with at least
-// `lag` passes left before that eye pass: the batch's passes wait for the ahead stages of the pass
-// `lag` passes back, which then covers this stage.  Stages begin in eye-pass order and draw the
+// whose k_pass_begin last ran (a launch that runs late leaves it to a later one), not in an eye pass
+// (the gather of the pass before it, which runs in that eye pass's photon-stage launch, still reads
+// the buffer the stage would write; the plain loop starts its ahead stages after that launch) and
+// with at least `lag` passes left before that eye pass: the batch's passes wait for the ahead
+// stages of the pass `lag` passes back, which then covers this stage.  Stages begin in eye-pass order and draw the
 // same numbers wherever they run, so the results do not depend on where a stage ran.  A job has one
 // stage in progress at most: the two conditions exclude each other in any interleaving of their
 // launches, and eye_go (which tags the stage's launches) is written only here and by k_grid.
@@ -664,7 +681,7 @@ PPM_DEV void eye_begin(const Job& J, int mode, unsigned int pass, int lag) {
     bool go;
     if (mode == kEyeAhead) {
         go = begun == g->eye_samples && (unsigned long long)g->eye_stages_done == begun && g->ahead_ok &&
-             (pass == kAnyPass || (g->begin_pass == pass && g->ahead_frames >= lag));
+             (pass == kAnyPass || (g->begin_pass == pass && !g->eye_now && g->ahead_frames >= lag));
     } else {
         go = g->eye_now && begun < g->eye_samples;
     }
@@ -685,8 +702,10 @@ PPM_DEV int cur_buffer(const DevGlobals* g) { return (int)((g->eye_samples + 1ul
 
 // The start of a pass (one block): its first wave folds the last pass's emitted counts into
 // NumPhotons (glslppm.cpp:396-397, `NumPhotons += int(sum)` once per pass, in float), clears the
-// counters and sets the pass's uniforms (pass_begin).  The pass's hash table was cleared by the
-// photon stage of the pass before (the glClear of glslppm.cpp:421, 450; see next_table).
+// counters, marks the pass before for the gather blocks of this pass's photon-stage launch (unless
+// k_gather already ran it), with NumPhotons after it for its draw, and sets the pass's uniforms
+// (pass_begin).  The pass's hash table was cleared by the photon stage of the pass before (the
+// glClear of glslppm.cpp:421, 450; see next_table).
 PPM_DEV void begin_pass(const Job& J, int blk, unsigned int pass) {
     DevGlobals* g = J.g;
     if (blk == 0 && threadIdx.x < 64) {
@@ -696,21 +715,35 @@ PPM_DEV void begin_pass(const Job& J, int blk, unsigned int pass) {
         if (threadIdx.x == 0) {
             g->num_photons_f = g->num_photons_f + (float)(int)v;
             g->photons_emitted += v;
+            // `active` is still that of the pass before, begun by the last k_pass_begin
+            const int lag = g->active && g->gathered != g->frame;
+            g->lag_pending = lag;
+            g->lag_idx = g->half;
+            if (lag) g->snap[g->half].np = g->num_photons_f;
             pass_begin(J, pass);
         }
     }
 }
 
-// The hash table of the pass in progress (after its k_pass_begin, which counted it in `frame`):
-// pass f (0-based) uses table f & 1.  The readback of the hash takes the same table after the
-// last pass.
-PPM_DEV unsigned int* hash_table(const HashState& h, const DevGlobals* g) {
-    return h.slots + (size_t)2 * (size_t)h.size * (size_t)((g->frame + 1ull) & 1ull);
-}
-// The table of the next pass, which the photon stage of the pass in progress clears.  The pass
-// before used it, and its gather ran before this pass's k_pass_begin on the same stream.
+// Hash table k of three.  Pass f (0-based) deposits into table f % 3 (set by its k_pass_begin);
+// the readback of the hash takes the same table after the last pass.
+PPM_DEV unsigned int* table_at(const HashState& h, int k) { return h.slots + (size_t)2 * (size_t)h.size * (size_t)k; }
+PPM_DEV unsigned int* hash_table(const HashState& h, const DevGlobals* g) { return table_at(h, g->table); }
+// The table of the next pass, which the photon stage of the pass in progress clears.  The pass two
+// before used it; its gather ran in the launch of the pass before, which has finished.
 PPM_DEV unsigned int* next_table(const HashState& h, const DevGlobals* g) {
-    return h.slots + (size_t)2 * (size_t)h.size * (size_t)(g->frame & 1ull);
+    return table_at(h, g->table == 2 ? 0 : g->table + 1);
+}
+
+// A view of the photon records with position, flux and direction in half k (rng, state and slot
+// are single).
+PPM_DEV PhotonState photon_half(const PhotonState& ps, int count, int k) {
+    PhotonState v = ps;
+    const size_t o = (size_t)count * (size_t)k;
+    v.pos += o;
+    v.flux += o;
+    v.dir += o;
+    return v;
 }
 
 // NumPhotons after the pass in progress: the folded count plus the pass's counters (every
@@ -743,7 +776,7 @@ PPM_DEV void eye(const Job& J, int blk, int mode) {
     // start first and the short ones fill in around them
     const int tile = g->order_tiles == J.tile_blocks ? hs.tile_order[blk] : blk;
     int px_i, px_j;
-    tile_pixel(J.tl, tile, px_i, px_j);
+    tile_pixel(J.tl, tile, (int)(threadIdx.x >> 6), px_i, px_j);
     const int p = px_j * ep.width + px_i;
     f3 pos = mk3(0.0f, 0.0f, 0.0f);
     int segs = 0;
@@ -1062,12 +1095,13 @@ PPM_DEV void ibl_sample(const DevScene& s, float4& rndv, f3& flux, f3& dir, f3& 
 
 namespace {
 
-// Emission or bounce of slot p.  Returns false when Russian roulette ends the photon, after
-// writing the terminated state (photontrace.fs:711-719).
-PPM_DEV bool photon_gen(const DevScene& s, const PhotonParams& pp, const PhotonState& ps, int p, f3& org_out,
-                        f3& dir_out, f3& flux_out, int& out_bits, unsigned int& emitted_out) {
+// Emission or bounce of slot p from its record in ps_in (the pass before's half).  Returns false
+// when Russian roulette ends the photon, after writing the terminated state into ps (this pass's
+// half; photontrace.fs:711-719).
+PPM_DEV bool photon_gen(const DevScene& s, const PhotonParams& pp, const PhotonState& ps_in, const PhotonState& ps,
+                        int p, f3& org_out, f3& dir_out, f3& flux_out, int& out_bits, unsigned int& emitted_out) {
     float4 rndv = ps.rng[p];
-    const float4 ppos = ps.pos[p], pflux = ps.flux[p], pdir = ps.dir[p];
+    const float4 ppos = ps_in.pos[p], pflux = ps_in.flux[p], pdir = ps_in.dir[p];
     const f3 prev_nrm = mk3(ppos.w, pflux.w, pdir.w);
     f3 flux = mk3(fabsf(pflux.x), fabsf(pflux.y), fabsf(pflux.z));
     const int2 st = ps.state[p];
@@ -1256,7 +1290,8 @@ PPM_DEV void hash_deferred(const Job& J, int blk) {
     if (!g->active || !g->defer) return;
     const int p = blk * kBlock + threadIdx.x;
     if (p >= J.pp.count || J.ps.slot[p] != kSlotPending) return;
-    J.ps.slot[p] = deposit(J.pp, J.h, g, p, xyz(J.ps.pos[p]));
+    const float4* pos = J.ps.pos + (size_t)J.pp.count * (size_t)g->half;
+    J.ps.slot[p] = deposit(J.pp, J.h, g, p, xyz(pos[p]));
 }
 
 // The photon stage in one launch: each thread takes one photon slot through emission or bounce
@@ -1280,7 +1315,9 @@ PPM_DEV void photon_stage(const Job& J, int blk) {
     bool live = false;
     f3 org, dir, flux;
     int out_bits = 0;
-    if (p < pp.count) live = photon_gen(s, pp, J.ps, p, org, dir, flux, out_bits, emitted);
+    const PhotonState ps_out = photon_half(J.ps, pp.count, g->half);
+    if (p < pp.count) live = photon_gen(s, pp, photon_half(J.ps, pp.count, g->half ^ 1), ps_out, p, org, dir, flux,
+                                        out_bits, emitted);
     // sum.fs: each wave's emitted flags go to its block's counter (folded in by the next pass's
     // k_pass_begin; the gather's draw adds them itself)
     const unsigned int es = wave_sum(emitted);
@@ -1295,7 +1332,7 @@ PPM_DEV void photon_stage(const Job& J, int blk) {
     W.h.t = 1.0e30f;
     W.h.bx = W.h.by = W.h.bz = 0.0f;
     walk<true>(s, W.face, W.org, W.dir, W.rcp, false, W.h, J.walk_stack);
-    photon_finish(s, pp, J.ps, J.h, g, p, W, flux, out_bits);
+    photon_finish(s, pp, ps_out, J.h, g, p, W, flux, out_bits);
 }
 
 // re.fs: tone map of one channel
@@ -1321,29 +1358,45 @@ PPM_DEV unsigned char unorm8(float c) {
     return (unsigned char)(int)(c * 255.0f + 0.5f);
 }
 
-// progressive.fs:98-142 for one hit point, then the re.fs draw of that pixel.
-PPM_DEV void gather(const Job& J, int blk) {
+// progressive.fs:98-142 for one hit point, then the re.fs draw of that pixel: the gather of pass
+// f, for wave `wave` (0..3) of pixel tile `tile`.  kGatherLagged: pass f is the pass before the one
+// in progress (the gather blocks of the photon-stage launch), and the gather reads the uniforms
+// pass f's k_pass_begin saved (GatherSnap); kGatherFlush: pass f is the last one begun (k_gather at
+// the end of a pass sequence; every thread of the block calls it, since the draw's NumPhotons is a
+// block-wide sum).  The gather reads pass f's hash table and photon half, which the photon stage
+// running beside it does not write, and its hit points, which no eye stage writes before the
+// gather is done (an eye stage writes the buffer the next eye pass reads).
+PPM_DEV void gather(const Job& J, int tile, int wave, int mode) {
     const DevGlobals* g = J.g;
-    if (!g->active) return;
+    GatherSnap sn;
+    float np = 0.0f;
     const GatherParams& gp = J.gp;
-    const float np = gp.draw ? num_photons_now(g) : 0.0f;
-    // the pass is done with the budget: count it down (k_pass_begin read it)
-    if (blk == 0 && threadIdx.x == 0 && g->passes_left != 0xffffffffu) J.g->passes_left = g->passes_left - 1u;
+    if (mode == kGatherLagged) {
+        if (!g->lag_pending) return;
+        sn = g->snap[g->lag_idx];
+        np = sn.np;
+    } else {
+        if (!g->active) return;
+        sn = g->snap[g->half];
+        if (gp.draw) np = num_photons_now(g);
+        // the pass is gathered: the next k_pass_begin leaves it out of the lagged gather
+        if (tile == 0 && threadIdx.x == 0) J.g->gathered = g->frame;
+    }
     const Tiles& tl = J.tl;
     const HitState& hs = J.hs;
-    const PhotonState& ps = J.ps;
+    const PhotonState ps = photon_half(J.ps, J.pp.count, sn.half);
     const HashState& h = J.h;
     int px_i, px_j;
-    tile_pixel(tl, blk, px_i, px_j);
+    tile_pixel(tl, tile, wave, px_i, px_j);
     if (px_i >= tl.width || px_j >= tl.height) return;
     const int p = px_j * tl.width + px_i;
-    // the hit points, emission and grid of the latest eye pass
-    const int cb = cur_buffer(g);
+    // the hit points, emission and grid of the latest eye pass at pass f
+    const int cb = sn.cb;
     const GridScalars& gr = g->grid[cb];
     const size_t hp = (size_t)cb * (size_t)hs.q + (size_t)p;
-    float4 tr = g->first_pass ? make_float4(0.0f, 0.0f, 0.0f, gr.initial_radius) : hs.tau_r[p];
+    float4 tr = sn.first_pass ? make_float4(0.0f, 0.0f, 0.0f, gr.initial_radius) : hs.tau_r[p];
     float4 en = hs.e_n[p];
-    if (g->eye_now) {
+    if (sn.eye_now) {
         // the emission changes with the eye stage only: between eye passes e_n holds it already
         const float4 em = hs.em[hp];
         en.x = em.x;
@@ -1364,7 +1417,7 @@ PPM_DEV void gather(const Job& J, int blk) {
         f3 flux = mk3(0.0f, 0.0f, 0.0f);
         float m = 0.0f;
         const int clamp_slot = (h.height - 1) * h.width;
-        const unsigned int* table = hash_table(h, g);
+        const unsigned int* table = table_at(h, sn.table);
         // Each cell's box is (cell / GridScale + BBoxMin, (cell + 1) / GridScale + BBoxMin)
         // (progressive.fs:79-80); along an axis the two cells' boxes share their middle bound, so the
         // block of eight has three bounds per axis, computed once with the same operations.
@@ -1429,6 +1482,21 @@ PPM_DEV void gather(const Job& J, int blk) {
     }
 }
 
+// The photon-stage launch: trace blocks [0, trace_blocks) run the photon stage of the pass in
+// progress; the blocks after them run the gather of the pass before, one wave per quarter of a
+// pixel tile.  Being dispatched last, the gather waves fill the GPU while the last walks finish.
+PPM_DEV void photon_launch(const Job& J, int blk) {
+    if (blk < J.trace_blocks) {
+        photon_stage(J, blk);
+        return;
+    }
+    const int w = (blk - J.trace_blocks) * (int)(blockDim.x >> 6) + (int)(threadIdx.x >> 6);
+    if (w < 4 * J.tile_blocks) gather(J, w >> 2, w & 3, kGatherLagged);
+}
+
+// k_gather: the gather of the last pass begun, in 256-thread tiles.
+PPM_DEV void gather_flush(const Job& J, int blk) { gather(J, blk, (int)(threadIdx.x >> 6), kGatherFlush); }
+
 }  // namespace
 
 // ------------------------------------------------------------------------------------------
@@ -1483,9 +1551,9 @@ __global__ __launch_bounds__(1024) void k_grid(Job J, int mode) { grid(J, mode);
 __global__ __launch_bounds__(1024) void k_grid_batch(const Job* __restrict__ jobs, int mode) {
     grid(jobs[blockIdx.x], mode);
 }
-PPM_JOB_KERNEL_WAVES(k_photon_stage, photon_stage, kTraceBlockMax, 7)
+PPM_JOB_KERNEL_WAVES(k_photon_stage, photon_launch, kTraceBlockMax, 7)
 PPM_JOB_KERNEL(k_hash, hash_deferred, kBlock)
-PPM_JOB_KERNEL(k_gather, gather, kBlock)
+PPM_JOB_KERNEL(k_gather, gather_flush, kBlock)
 
 
 // (frame, passes_left) of every job of a batch, for the host's progress snapshot.

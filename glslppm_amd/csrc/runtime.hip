@@ -449,7 +449,11 @@ void queue_eye_stage(ppm_ctx* c, const Job& J, hipStream_t es, int ahead) {
     prof_end(c, PPM_KERNEL_EYE, e0, es);
 }
 
-// Launches one pass of job J on stream st: m_display's stage order (glslppm.cpp:214-536).
+// Launches one pass of job J on stream st: m_display's stage order (glslppm.cpp:214-536), except
+// that the pass's gather (progressive.fs, re.fs) runs in the photon-stage launch of the pass after
+// it, beside that pass's walks, or at the end of the sequence (queue_flush); it reads only what
+// the pass it gathers left (its hash table of three, its half of the photon records, the uniforms
+// k_pass_begin saved for it) and nothing the passes after it read, so every result is unchanged.
 // The eye stage (eye trace and grid set-up) and the photon trace do not read each other's
 // results: only the deposits need the grid.  So in an eye pass the eye stage runs on the side
 // stream beside the photon stage, whose deposits wait (kSlotPending) for k_hash after the join;
@@ -482,14 +486,19 @@ void queue_pass(ppm_ctx* c, const Job& J, hipStream_t st, int kind, bool fork_ah
         queue_eye_stage(c, J, c->side, 1);
         (void)hipEventRecord(c->ev_ahead, c->side);
     }
-    hipLaunchKernelGGL(k_photon_stage, dim3(J.trace_blocks), dim3(c->trace_block),
+    hipLaunchKernelGGL(k_photon_stage, dim3(J.trace_blocks + J.gather_blocks), dim3(c->trace_block),
                        walk_lds_bytes(J.walk_stack, c->trace_block), st, J);
     prof_end(c, PPM_KERNEL_PHOTON, e2, st);
     if (fork) (void)hipStreamWaitEvent(st, c->ev_join, 0);
-    hipEvent_t e3 = prof_begin(c, PPM_KERNEL_GATHER, st);
     if (stage) hipLaunchKernelGGL(k_hash, dim3(J.photon_blocks), dim3(256), 0, st, J);
+}
+
+// The gather of the last pass of a sequence (the other passes' gathers run in the photon-stage
+// launch of the pass after them).
+void queue_flush(ppm_ctx* c, const Job& J, hipStream_t st) {
+    hipEvent_t e = prof_begin(c, PPM_KERNEL_GATHER, st);
     hipLaunchKernelGGL(k_gather, dim3(J.tile_blocks), dim3(256), 0, st, J);
-    prof_end(c, PPM_KERNEL_GATHER, e3, st);
+    prof_end(c, PPM_KERNEL_GATHER, e, st);
 }
 
 // One captured pass of a kind (queue_pass) for PPM_FLAG_GRAPH.
@@ -565,6 +574,7 @@ int run_passes(ppm_ctx* c, const Job& J, uint32_t npasses, bool allow_graph) {
         }
         c->frame++;
     }
+    queue_flush(c, J, c->stream);
     HIP_TRY(c, hipGetLastError());
     HIP_TRY(c, hipEventRecord(c->ev1, c->stream));
     c->timed = true;
@@ -700,6 +710,7 @@ void set_launch_shape(ppm_ctx* c, int trace_block, int wave_shift, int walk_stac
     J.tile_blocks = tile_blocks(J.tl.width, J.tl.height, wave_shift);
     c->trace_block = trace_block;
     J.trace_blocks = (c->P + trace_block - 1) / trace_block;
+    J.gather_blocks = gather_blocks(J.tile_blocks, trace_block);
     for (int k = 0; k < kPassKinds; k++)
         if (c->graph_pass[k]) {
             // a graph in flight keeps running; destroy it once the streams are idle
@@ -710,6 +721,13 @@ void set_launch_shape(ppm_ctx* c, int trace_block, int wave_shift, int walk_stac
 }
 int photon_blocks(const ppm_params* p) { return (p->photon_width * p->photon_height + 255) / 256; }
 
+// Blocks of `block` threads that the photon-stage launch adds for the gather of the pass before:
+// one wave per quarter of each of the tiles.
+int gather_blocks(int tiles, int block) {
+    const int waves = block / 64;
+    return (4 * tiles + waves - 1) / waves;
+}
+
 size_t state_footprint(const ppm_params* p) {
     const size_t Q = (size_t)p->image_width * p->image_height, P = (size_t)p->photon_width * p->photon_height;
     const size_t H = (size_t)p->hash_width * p->hash_height;
@@ -718,10 +736,10 @@ size_t state_footprint(const ppm_params* p) {
         // (two buffers each), rgba8, eye partials
         Q * 32, Q * 32, Q * 32, Q * 32, Q * 32, Q * 16, Q * 16, Q * 8, Q * 16, Q * 4, (size_t)max_tile_blocks(p) * 24,
         (size_t)max_tile_blocks(p) * 4, (size_t)max_tile_blocks(p) * 4,  // tile costs and order
-        // photons: pos, flux, dir, rng, state, slot
-        P * 16, P * 16, P * 16, P * 16, P * 8, P * 4,
-        // hash (count, representative) records, two tables; globals; resolve targets
-        H * 16, sizeof(DevGlobals), Q * 16, Q * 16};
+        // photons: pos, flux, dir (two halves each), rng, state, slot
+        P * 32, P * 32, P * 32, P * 16, P * 8, P * 4,
+        // hash (count, representative) records, three tables; globals; resolve targets
+        H * 24, sizeof(DevGlobals), Q * 16, Q * 16};
     size_t tot = 0;
     for (size_t b : sizes) tot += Arena::round_size(b ? b : 16);
     return tot;
@@ -943,11 +961,11 @@ int create_context(ppm_ctx* c, std::shared_ptr<SceneDev> scene, const ppm_camera
         (rc = dev_alloc(c, &hs.mat_t, 2 * Q)) || (rc = dev_alloc(c, &hs.rgba8, Q)) ||
         (rc = dev_alloc(c, &hs.partial, (size_t)max_tile_blocks(p) * 6)) ||
         (rc = dev_alloc(c, &hs.tile_cost, (size_t)max_tile_blocks(p))) ||
-        (rc = dev_alloc(c, &hs.tile_order, (size_t)max_tile_blocks(p))) || (rc = dev_alloc(c, &ps.pos, P)) ||
-        (rc = dev_alloc(c, &ps.flux, P)) || (rc = dev_alloc(c, &ps.dir, P)) || (rc = dev_alloc(c, &ps.rng, P)) ||
+        (rc = dev_alloc(c, &hs.tile_order, (size_t)max_tile_blocks(p))) || (rc = dev_alloc(c, &ps.pos, 2 * P)) ||
+        (rc = dev_alloc(c, &ps.flux, 2 * P)) || (rc = dev_alloc(c, &ps.dir, 2 * P)) || (rc = dev_alloc(c, &ps.rng, P)) ||
         (rc = dev_alloc(c, &ps.state, P)) || (rc = dev_alloc(c, &ps.slot, P)) ||
 
-        (rc = dev_alloc(c, &J.h.slots, 4 * H)) || (rc = dev_alloc(c, &J.g, 1)) ||
+        (rc = dev_alloc(c, &J.h.slots, 6 * H)) || (rc = dev_alloc(c, &J.g, 1)) ||
         (rc = dev_alloc(c, &c->resolve_lin, Q)) || (rc = dev_alloc(c, &c->resolve_map, Q)))
         return rc;
     J.h.size = c->H;
@@ -1204,9 +1222,10 @@ int ppm_readback(ppm_ctx* c, ppm_buf which, void* dst, size_t bytes) {
             }
             return PPM_OK;
         }
-        case PPM_BUF_PHOTON_POSITION: return d2h(J.ps.pos, P * 16);
-        case PPM_BUF_PHOTON_FLUX: return d2h(J.ps.flux, P * 16);
-        case PPM_BUF_PHOTON_DIRECTION: return d2h(J.ps.dir, P * 16);
+        // the photon half of the last pass that ran (both hold the initial records before any)
+        case PPM_BUF_PHOTON_POSITION: return d2h(J.ps.pos + (size_t)g.half * P, P * 16);
+        case PPM_BUF_PHOTON_FLUX: return d2h(J.ps.flux + (size_t)g.half * P, P * 16);
+        case PPM_BUF_PHOTON_DIRECTION: return d2h(J.ps.dir + (size_t)g.half * P, P * 16);
         case PPM_BUF_PHOTON_RNG: return d2h(J.ps.rng, P * 16);
         case PPM_BUF_PHOTON_STATE: {
             std::vector<int2> st(P);
@@ -1222,9 +1241,9 @@ int ppm_readback(ppm_ctx* c, ppm_buf which, void* dst, size_t bytes) {
         case PPM_BUF_PHOTON_SLOT: return d2h(J.ps.slot, P * 4);
         case PPM_BUF_SLOT_COUNT:
         case PPM_BUF_SLOT_REP: {
-            // the table of the last pass that ran (passes alternate between the two)
+            // the table of the last pass that ran (passes take the three in turn)
             std::vector<uint32_t> rec(2 * H);
-            const size_t table = (size_t)((g.frame + 1ull) & 1ull);
+            const size_t table = (size_t)g.table;
             HIP_TRY(c, hipMemcpy(rec.data(), J.h.slots + table * 2 * H, H * 8, hipMemcpyDeviceToHost));
             uint32_t* o = (uint32_t*)dst;
             const size_t k = which == PPM_BUF_SLOT_COUNT ? 0 : 1;

@@ -176,7 +176,9 @@ void ppm_params_default(ppm_params* p, int32_t resolution);
 int ppm_create(const ppm_scene* scene, const ppm_params* params, ppm_ctx** out);
 
 /* Replaces m_display() (glslppm.cpp:214-536), run npasses times back to back on the context's
- * stream.  Returns once the work is queued. */
+ * stream.  Returns once the work is queued.  Each pass's gather and draw (progressive.fs, re.fs)
+ * run in the photon-stage launch of the pass after it, the last pass's at the end of the call, so
+ * once the call's work is done every one of its passes is complete. */
 int ppm_pass(ppm_ctx* ctx, uint32_t npasses);
 
 /* ppm_pass for callers that cannot allocate device memory (code under HIP graph capture, tight
@@ -213,9 +215,10 @@ float ppm_last_pass_ms(ppm_ctx* ctx);
 #define PPM_KERNEL_EYE 0      /* eyeraytrace.fs + max/min */
 #define PPM_KERNEL_CLEAR 1    /* k_pass_begin: the pass's uniforms (the glClear of the hash and
                                  correction targets runs inside the photon stage of the pass
-                                 before, on the other of two hash tables) */
-#define PPM_KERNEL_PHOTON 2   /* photontrace.fs + sum + hash + scatter + correction */
-#define PPM_KERNEL_GATHER 3   /* progressive.fs + re.fs */
+                                 before, on the next of three hash tables) */
+#define PPM_KERNEL_PHOTON 2   /* photontrace.fs + sum + hash + scatter + correction, with the
+                                 progressive.fs + re.fs of the pass before in the same launch */
+#define PPM_KERNEL_GATHER 3   /* progressive.fs + re.fs of the last pass of a ppm_pass call */
 #define PPM_KERNEL_KINDS 4
 #define PPM_PROFILE_KINDS 0x100  /* ppm_profile_enable: the low bits select kernel kinds */
 int ppm_profile_enable(ppm_ctx* ctx, int enable);

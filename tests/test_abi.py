@@ -154,16 +154,16 @@ def test_batch_create_without_gpu_reports_no_device():
 
 def test_scene_footprint_counts_every_state_array():
     """Per-job HBM: 252 B per pixel (the eye pass's outputs twice: 2 x (5 x 16 + 4 + 8); the
-    progressive statistics 32; RGBA8 4; resolve targets 32), 76 B per photon slot (position, flux,
-    direction, generator 4 x 16; state 8; slot 4), 16 B per hash slot (two tables of count and
-    representative, used by alternate passes), plus the scene and small tables, each array rounded up to the pool's block size (at most
-    1/8 above)."""
+    progressive statistics 32; RGBA8 4; resolve targets 32), 124 B per photon slot (position, flux,
+    direction twice, one half per pass parity, 6 x 16; generator 16; state 8; slot 4), 24 B per
+    hash slot (three tables of count and representative, used in turn), plus the scene and small
+    tables, each array rounded up to the pool's block size (at most 1/8 above)."""
     m = _scene()
     small = g.scene_footprint(m, g.default_params(32))
     p = g.default_params(1024)
     big = g.scene_footprint(m, p)
     Q = P = H = 1024 * 1024
-    exact = 252 * Q + 76 * P + 16 * H
+    exact = 252 * Q + 124 * P + 24 * H
     assert exact <= big <= exact * 1.125 + 64 * 65536
     assert small < big
 

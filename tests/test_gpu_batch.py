@@ -209,7 +209,8 @@ def test_eye_stage_ahead_at_the_shortest_cadences(max_path, passes_per_graph):
 def test_profiling_times_the_chosen_kernel_kinds_and_keeps_results():
     """ppm_profile_enable: 1 times every kind, PPM_PROFILE_KINDS | bits only the chosen ones; the
     launch counts follow the pass cadence (L=3: eye passes 0, 3 and 6 in 7 passes, and their 3 eye
-    stages; the stage of eye pass 9 would start inside pass 7)."""
+    stages; the stage of eye pass 9 would start inside pass 7); the gathers of passes 0-5 run in the
+    photon-stage launches of passes 1-6, so k_gather runs once, for pass 6, at the end of the call."""
     m = spheres(5, 4, 2, 50, 37)
     p = g.default_params(64, image_width=50, image_height=37, photon_width=40, photon_height=33, max_path_length=3)
     a = g.Renderer(m, p)
@@ -218,7 +219,7 @@ def test_profiling_times_the_chosen_kernel_kinds_and_keeps_results():
     b.profile(True)
     b.run(7)
     every = b.profile_read()
-    assert [every[k][1] for k in g.KERNELS] == [3, 7, 7, 7], every
+    assert [every[k][1] for k in g.KERNELS] == [3, 7, 7, 1], every
     assert all(every[k][0] > 0.0 for k in g.KERNELS)
     b.profile(True, kinds=["photon"])
     b.run(5)
