@@ -139,6 +139,7 @@ bool ahead_mode(const ppm_batch* b) { return b->eye_overlap && b->passes_per_gra
 int rebuild(ppm_batch* b) {
     if (!b->dirty) return PPM_OK;
     std::vector<Job> jobs;
+    std::vector<int> tris;
     std::vector<BlockRef> maps[kMaps];
     b->table_ids.clear();
     for (size_t id = 0; id < b->slots.size(); id++) {
@@ -146,16 +147,26 @@ int rebuild(ppm_batch* b) {
         if (!c) continue;
         const int j = (int)jobs.size();
         jobs.push_back(c->job);
+        tris.push_back(c->T);
         b->table_ids.push_back((int32_t)id);
-        const int per[kMaps] = {c->job.tile_blocks, c->job.photon_blocks, c->job.begin_blocks, c->job.trace_blocks};
+        const int per[kMaps] = {0, c->job.photon_blocks, c->job.begin_blocks, 0};
         for (int k = 0; k < kMaps; k++)
             for (int i = 0; i < per[k]; i++) maps[k].push_back(BlockRef{j, i});
     }
-    // the photon-stage launch's gather blocks follow every job's trace blocks, so that they are
-    // dispatched last
-    for (size_t j = 0; j < jobs.size(); j++)
+    // The walk launches (eye tiles, photon trace blocks) take the jobs of the largest scenes first:
+    // their walks are the longest, and a launch ends with the blocks dispatched last.  The
+    // photon-stage launch's gather blocks follow every job's trace blocks, so that they are
+    // dispatched last of all.
+    std::vector<int> order(jobs.size());
+    for (size_t j = 0; j < jobs.size(); j++) order[j] = (int)j;
+    std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return tris[x] > tris[y]; });
+    for (int j : order) {
+        for (int i = 0; i < jobs[j].tile_blocks; i++) maps[kMapTile].push_back(BlockRef{j, i});
+        for (int i = 0; i < jobs[j].trace_blocks; i++) maps[kMapTrace].push_back(BlockRef{j, i});
+    }
+    for (int j : order)
         for (int i = 0; i < jobs[j].gather_blocks; i++)
-            maps[kMapTrace].push_back(BlockRef{(int)j, jobs[j].trace_blocks + i});
+            maps[kMapTrace].push_back(BlockRef{j, jobs[j].trace_blocks + i});
     // the previous tables may still be read by queued passes (eye stages on `ahead` included): hand
     // them back to the arena, which reuses them only after the stream has passed this point
     join_ahead(b);
