@@ -144,6 +144,27 @@ def test_plain_loop_graph_replay_equals_launches():
     assert not format_report(rep), format_report(rep)
 
 
+@pytest.mark.parametrize("splits,graph", [((1,) * 9, 0), ((2, 3, 4), 0), ((5, 4), 1), ((1, 1, 7), 1)])
+def test_pass_calls_split_anywhere_equal_the_oracle(splits, graph):
+    """Each pass's gather runs in the photon-stage launch of the pass after it, and a call's last
+    pass is gathered at the end of the call: the same passes split over calls in any way, with
+    every buffer read back between the calls, equal the oracle after each call (L = 3: eye passes
+    first, inside and last in a call; one-wave photon blocks and four-wave ones, with and without
+    graph replay)."""
+    m = spheres(3, 5, 2, 50, 37)
+    p = g.default_params(64, image_width=50, image_height=37, photon_width=40, photon_height=33, max_path_length=3)
+    gpu = g.Renderer(m, p)
+    cpu = oracle.OracleRenderer(m, p)
+    gpu.set_tuning(eye_overlap=1, passes_per_graph=graph, trace_block=64 if graph else 256)
+    for n in splits:
+        gpu.run(n)
+        cpu.run(n)
+        rep = compare_all(gpu, cpu)
+        assert not format_report(rep), (n, format_report(rep))
+    gpu.close()
+    cpu.close()
+
+
 @pytest.mark.parametrize("shape", [dict(trace_block=64), dict(trace_block=512), dict(tile_wave_width=1),
                                    dict(tile_wave_width=4), dict(tile_wave_width=64),
                                    dict(trace_block=128, tile_wave_width=16, passes_per_graph=1),
