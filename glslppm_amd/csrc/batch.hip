@@ -83,28 +83,63 @@ void retire_graph(ppm_batch* b) {
 // lengthening the eye pass.  Pass p waits for the ahead launches of pass p - lag before its photon
 // stage, which covers every ahead stage an eye pass needs (eye_begin starts one only with at least
 // `lag` passes to go).  Without eye_overlap everything runs in order on one stream.
-void queue_batch_pass(ppm_batch* b, hipStream_t st, bool run_ahead) {
+// The eye-stage work a batched pass can have: an eye pass of some job (its stage, if not begun
+// ahead, and its held-back deposits), or some job that may begin its next eye stage ahead.
+struct PassNeeds {
+    bool eye_pass;
+    bool ahead;
+};
+
+// Advances every job's frame and budget mirror by one queued batched pass, by k_pass_begin's rule
+// (a job with budget left runs the pass, which is an eye pass when its frame count is a multiple
+// of its cadence), and returns what the pass can need: eye_begin begins a kEyeNow stage only in a
+// job's eye pass, k_hash deposits only there, and a kEyeAhead stage begins only in a job's photon
+// pass with at least `lag` passes to its next eye pass and the budget to reach it.  A launch the
+// pass cannot need would exit in every block, so leaving it out changes nothing.
+PassNeeds advance_mirrors(ppm_batch* b) {
+    PassNeeds n{false, false};
+    for (ppm_ctx* c : b->slots) {
+        if (!c || c->batch_left == 0u) continue;
+        const uint64_t L = (uint64_t)c->prm.max_path_length;
+        const uint64_t in = c->batch_frame % L;
+        if (in == 0) {
+            n.eye_pass = true;
+        } else {
+            const uint64_t to_next = L - in;
+            const bool budget = c->batch_left == 0xffffffffu || (uint64_t)(c->batch_left - 1u) >= to_next;
+            if (budget && to_next >= (uint64_t)b->lag) n.ahead = true;
+        }
+        c->batch_frame++;
+        if (c->batch_left != 0xffffffffu) c->batch_left--;
+    }
+    return n;
+}
+
+void queue_batch_pass(ppm_batch* b, hipStream_t st, bool run_ahead, PassNeeds need) {
     const int n = b->n_jobs;
     const Job* J = b->d_jobs;
     const unsigned int pass = run_ahead ? b->pass_no : 0u;
     hipLaunchKernelGGL(k_pass_begin_batch, dim3(b->n_blocks[kMapBegin]), dim3(256), 0, st, J,
                        (const BlockRef*)b->d_map[kMapBegin], pass);
     hipStream_t es = b->eye_overlap ? b->side : st;
-    if (b->eye_overlap) {
-        (void)hipEventRecord(b->ev_fork, st);
-        (void)hipStreamWaitEvent(es, b->ev_fork, 0);
-    }
+    const bool now = need.eye_pass, ahead = run_ahead && need.ahead;
+    if (b->eye_overlap && (now || ahead)) (void)hipEventRecord(b->ev_fork, st);
     auto eye_stage = [&](hipStream_t s, int mode, int lag) {
         hipLaunchKernelGGL(k_eye_begin_batch, dim3((n + 63) / 64), dim3(64), 0, s, J, n, mode, pass, lag);
         hipLaunchKernelGGL(k_eye_batch, dim3(b->n_blocks[kMapTile]), dim3(256), walk_lds_bytes(b->walk_stack, 256), s,
                            J, (const BlockRef*)b->d_map[kMapTile], mode);
         hipLaunchKernelGGL(k_grid_batch, dim3(n), dim3(1024), 0, s, J, mode);
     };
-    eye_stage(es, kEyeNow, 0);
-    if (b->eye_overlap) (void)hipEventRecord(b->ev_join, es);
+    if (now) {
+        if (b->eye_overlap) (void)hipStreamWaitEvent(es, b->ev_fork, 0);
+        eye_stage(es, kEyeNow, 0);
+        if (b->eye_overlap) (void)hipEventRecord(b->ev_join, es);
+    }
     if (run_ahead) {
-        (void)hipStreamWaitEvent(b->ahead, b->ev_fork, 0);
-        eye_stage(b->ahead, kEyeAhead, b->lag);
+        if (ahead) {
+            (void)hipStreamWaitEvent(b->ahead, b->ev_fork, 0);
+            eye_stage(b->ahead, kEyeAhead, b->lag);
+        }
         (void)hipEventRecord(b->ring[pass % ppm_batch::kAheadRing], b->ahead);
         if (pass >= b->ring_from + (uint32_t)b->lag)
             (void)hipStreamWaitEvent(st, b->ring[(pass - (uint32_t)b->lag) % ppm_batch::kAheadRing], 0);
@@ -112,9 +147,11 @@ void queue_batch_pass(ppm_batch* b, hipStream_t st, bool run_ahead) {
     }
     hipLaunchKernelGGL(k_photon_stage_batch, dim3(b->n_blocks[kMapTrace]), dim3(b->trace_block),
                        walk_lds_bytes(b->walk_stack, b->trace_block), st, J, (const BlockRef*)b->d_map[kMapTrace]);
-    if (b->eye_overlap) (void)hipStreamWaitEvent(st, b->ev_join, 0);
-    hipLaunchKernelGGL(k_hash_batch, dim3(b->n_blocks[kMapPhoton]), dim3(256), 0, st, J,
-                       (const BlockRef*)b->d_map[kMapPhoton]);
+    if (now) {
+        if (b->eye_overlap) (void)hipStreamWaitEvent(st, b->ev_join, 0);
+        hipLaunchKernelGGL(k_hash_batch, dim3(b->n_blocks[kMapPhoton]), dim3(256), 0, st, J,
+                           (const BlockRef*)b->d_map[kMapPhoton]);
+    }
 }
 
 // The gathers of the last batched pass of a sequence; every other pass's gathers run in the
@@ -196,7 +233,8 @@ int rebuild(ppm_batch* b) {
     if (b->passes_per_graph > 0) {
         hipGraph_t g = nullptr;
         BHIP(b, hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
-        for (uint32_t k = 0; k < b->passes_per_graph; k++) queue_batch_pass(b, b->stream, false);
+        // a captured pass is replayed whatever the jobs' phases: it carries every launch
+        for (uint32_t k = 0; k < b->passes_per_graph; k++) queue_batch_pass(b, b->stream, false, PassNeeds{true, true});
         BHIP(b, hipStreamEndCapture(b->stream, &g));
         const hipError_t e = hipGraphInstantiate(&b->graph, g, nullptr, nullptr, 0);
         (void)hipGraphDestroy(g);
@@ -228,6 +266,8 @@ int batch_add_resident(ppm_batch* b, std::shared_ptr<SceneDev> scene, const ppm_
         return berr(b, rc, m);
     }
     set_launch_shape(c, b->trace_block, b->wave_shift, b->walk_stack);
+    c->batch_frame = 0;
+    c->batch_left = passes;
     b->slots.push_back(c);
     *job = (int32_t)(b->slots.size() - 1);
     b->dirty = true;
@@ -372,6 +412,7 @@ int ppm_batch_set_passes(ppm_batch* b, int32_t job, uint32_t passes) {
     BHIP(b, hipMemcpyAsync((char*)c->job.g + offsetof(DevGlobals, passes_left), &passes, sizeof(passes),
                            hipMemcpyHostToDevice, b->stream));
     BHIP(b, hipStreamSynchronize(b->stream));
+    c->batch_left = passes;
     return PPM_OK;
 }
 
@@ -387,11 +428,15 @@ int ppm_batch_pass(ppm_batch* b, uint32_t npasses) {
     if (b->graph) {
         while (left >= b->passes_per_graph) {
             BHIP(b, hipGraphLaunch(b->graph, b->stream));
+            for (uint32_t k = 0; k < b->passes_per_graph; k++) (void)advance_mirrors(b);
             left -= b->passes_per_graph;
         }
     }
     const bool run_ahead = ahead_mode(b);
-    for (; left > 0; left--) queue_batch_pass(b, b->stream, run_ahead);
+    for (; left > 0; left--) {
+        const PassNeeds need = advance_mirrors(b);
+        queue_batch_pass(b, b->stream, run_ahead, need);
+    }
     queue_batch_flush(b, b->stream);
     BHIP(b, hipGetLastError());
     BHIP(b, hipEventRecord(b->ev1, b->stream));

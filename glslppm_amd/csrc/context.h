@@ -113,6 +113,10 @@ struct ppm_ctx {
     double prof_ms[PPM_KERNEL_KINDS] = {0, 0, 0, 0};
     uint64_t prof_n[PPM_KERNEL_KINDS] = {0, 0, 0, 0};
     uint64_t frame = 0;             // passes queued by ppm_pass (the plain loop's eye cadence)
+    // a batch job's frame count and pass budget as its k_pass_begin advances them, pass by queued
+    // pass (the batch launches eye-stage work only in passes where some job can have it)
+    uint64_t batch_frame = 0;
+    uint32_t batch_left = 0;
     int trace_block = kTraceBlockDefault;  // launch shape (ppm_set_tuning)
     bool use_graph = false;         // replay passes as hipGraphs (PPM_FLAG_GRAPH)
     bool eye_overlap = false;       // eye stages on `side`, ahead of their eye passes (ppm_set_tuning)
