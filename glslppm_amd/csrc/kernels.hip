@@ -23,6 +23,36 @@ namespace {
 
 constexpr float kInf = __builtin_huge_valf();
 
+// Streaming accesses: the per-slot photon records, read once and written once per pass (600 MB a
+// launch at 4M photons), and the hash clear.  As nontemporal loads and stores their lines are the
+// first to leave L2, so they evict fewer of the walk's tree and triangle records (measured: photon
+// stage 1.370 -> 1.350 ms at 4M photons, 0.446 -> 0.436 ms at 1M).
+typedef float v4f_stream __attribute__((ext_vector_type(4)));
+typedef int v2i_stream __attribute__((ext_vector_type(2)));
+PPM_DEV float4 ld_stream(const float4* p) {
+    const v4f_stream v = __builtin_nontemporal_load(reinterpret_cast<const v4f_stream*>(p));
+    return make_float4(v.x, v.y, v.z, v.w);
+}
+PPM_DEV int2 ld_stream(const int2* p) {
+    const v2i_stream v = __builtin_nontemporal_load(reinterpret_cast<const v2i_stream*>(p));
+    return make_int2(v.x, v.y);
+}
+PPM_DEV void st_stream(float4* p, float4 x) {
+    v4f_stream v;
+    v.x = x.x;
+    v.y = x.y;
+    v.z = x.z;
+    v.w = x.w;
+    __builtin_nontemporal_store(v, reinterpret_cast<v4f_stream*>(p));
+}
+PPM_DEV void st_stream(int2* p, int2 x) {
+    v2i_stream v;
+    v.x = x.x;
+    v.y = x.y;
+    __builtin_nontemporal_store(v, reinterpret_cast<v2i_stream*>(p));
+}
+PPM_DEV void st_stream(int* p, int x) { __builtin_nontemporal_store(x, p); }
+
 PPM_DEV int cube_face(f3 d) {
     const float ax = fabsf(d.x), ay = fabsf(d.y), az = fabsf(d.z);
     if (ax >= ay && ax >= az) return d.x >= 0.0f ? 0 : 1;
@@ -1100,11 +1130,11 @@ namespace {
 // half; photontrace.fs:711-719).
 PPM_DEV bool photon_gen(const DevScene& s, const PhotonParams& pp, const PhotonState& ps_in, const PhotonState& ps,
                         int p, f3& org_out, f3& dir_out, f3& flux_out, int& out_bits, unsigned int& emitted_out) {
-    float4 rndv = ps.rng[p];
-    const float4 ppos = ps_in.pos[p], pflux = ps_in.flux[p], pdir = ps_in.dir[p];
+    float4 rndv = ld_stream(ps.rng + p);
+    const float4 ppos = ld_stream(ps_in.pos + p), pflux = ld_stream(ps_in.flux + p), pdir = ld_stream(ps_in.dir + p);
     const f3 prev_nrm = mk3(ppos.w, pflux.w, pdir.w);
     f3 flux = mk3(fabsf(pflux.x), fabsf(pflux.y), fabsf(pflux.z));
-    const int2 st = ps.state[p];
+    const int2 st = ld_stream(ps.state + p);
     const int bounce = (st.y & 0xffff) + 1;
     f3 org = mk3(0.0f, 0.0f, 0.0f), dir = mk3(0.0f, 0.0f, 0.0f);
     bool cont;
@@ -1214,13 +1244,13 @@ PPM_DEV bool photon_gen(const DevScene& s, const PhotonParams& pp, const PhotonS
         }
     }
     emitted_out = emitted;
-    ps.rng[p] = rndv;
+    st_stream(ps.rng + p, rndv);
     if (!cont) {
-        ps.pos[p] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-        ps.flux[p] = make_float4(-1.0f, -1.0f, -1.0f, 0.0f);
-        ps.dir[p] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-        ps.state[p] = make_int2(-1, (int)(emitted << 16));
-        ps.slot[p] = -1;
+        st_stream(ps.pos + p, make_float4(0.0f, 0.0f, 0.0f, 0.0f));
+        st_stream(ps.flux + p, make_float4(-1.0f, -1.0f, -1.0f, 0.0f));
+        st_stream(ps.dir + p, make_float4(0.0f, 0.0f, 0.0f, 0.0f));
+        st_stream(ps.state + p, make_int2(-1, (int)(emitted << 16)));
+        st_stream(ps.slot + p, -1);
         return false;
     }
     org_out = org;
@@ -1273,15 +1303,15 @@ PPM_DEV void photon_finish(const DevScene& s, const PhotonParams& pp, const Phot
     // a miss has no geometric normal: 0/0 is NaN and the NaN-rejecting min gives sqrt(5)
     const float gf = min1(fabsf(dot3(dir, hnrm) / dot3(dir, hgn)), sqrtf(5.0f));
     flux = flux * gf;
-    ps.pos[p] = make_float4(hpos.x, hpos.y, hpos.z, hnrm.x);
-    ps.flux[p] = make_float4(flux.x, flux.y, flux.z, hnrm.y);
-    ps.dir[p] = make_float4(dir.x, dir.y, dir.z, hnrm.z);
-    ps.state[p] = make_int2(W.h.tri, out_bits);
+    st_stream(ps.pos + p, make_float4(hpos.x, hpos.y, hpos.z, hnrm.x));
+    st_stream(ps.flux + p, make_float4(flux.x, flux.y, flux.z, hnrm.y));
+    st_stream(ps.dir + p, make_float4(dir.x, dir.y, dir.z, hnrm.z));
+    st_stream(ps.state + p, make_int2(W.h.tri, out_bits));
     // hash.fs: deposits with flux.x >= 0 (including -0) go to the grid.  In an eye pass the grid
     // is still being set up while the photons fly, so the deposit waits for k_hash (kSlotPending).
     int slot = -1;
     if (!(flux.x < 0.0f)) slot = g->defer ? kSlotPending : deposit(pp, h, g, p, hpos);
-    ps.slot[p] = slot;
+    st_stream(ps.slot + p, slot);
 }
 
 // The deposits an eye pass held back, once k_grid has set the grid up.
@@ -1308,7 +1338,7 @@ PPM_DEV void photon_stage(const Job& J, int blk) {
         uint2* nt = reinterpret_cast<uint2*>(next_table(J.h, g));
         const int stride = J.trace_blocks * (int)blockDim.x;
         for (int i = blk * (int)blockDim.x + (int)threadIdx.x; i < J.h.size; i += stride)
-            nt[i] = make_uint2(0u, 0xffffffffu);
+            st_stream(reinterpret_cast<int2*>(nt + i), make_int2(0, -1));
     }
     const int p = blk * (int)blockDim.x + (int)threadIdx.x;
     unsigned int emitted = 0;
