@@ -80,9 +80,11 @@ void retire_graph(ppm_batch* b) {
 // with `run_ahead` (plain launches), a job's next eye stage begins on `ahead` in the pass after
 // each of its eye passes and runs beside the photon passes until the next one (kEyeAhead), so the
 // eye stages of many jobs whose eye passes coincide spread over the passes in between instead of
-// lengthening the eye pass.  Pass p waits for the ahead launches of pass p - lag before its photon
-// stage, which covers every ahead stage an eye pass needs (eye_begin starts one only with at least
-// `lag` passes to go).  Without eye_overlap everything runs in order on one stream.
+// lengthening the eye pass.  An eye pass p then holds its deposits back (defer_all) and waits for
+// the ahead launches of pass p - lag after its photon stage, before k_hash, which covers every
+// ahead stage it needs (eye_begin starts one only with at least `lag` passes to go); the photon
+// stage itself waits for nothing on `ahead`.  Without eye_overlap everything runs in order on one
+// stream.
 // The eye-stage work a batched pass can have: an eye pass of some job (its stage, if not begun
 // ahead, and its held-back deposits), or some job that may begin its next eye stage ahead.
 struct PassNeeds {
@@ -119,10 +121,12 @@ void queue_batch_pass(ppm_batch* b, hipStream_t st, bool run_ahead, PassNeeds ne
     const int n = b->n_jobs;
     const Job* J = b->d_jobs;
     const unsigned int pass = run_ahead ? b->pass_no : 0u;
-    hipLaunchKernelGGL(k_pass_begin_batch, dim3(b->n_blocks[kMapBegin]), dim3(256), 0, st, J,
-                       (const BlockRef*)b->d_map[kMapBegin], pass);
-    hipStream_t es = b->eye_overlap ? b->side : st;
     const bool now = need.eye_pass, ahead = run_ahead && need.ahead;
+    // with stages begun ahead, an eye pass's deposits always wait for k_hash, which runs after the
+    // pass has waited for those stages: the photon stage need not wait for them
+    hipLaunchKernelGGL(k_pass_begin_batch, dim3(b->n_blocks[kMapBegin]), dim3(256), 0, st, J,
+                       (const BlockRef*)b->d_map[kMapBegin], pass, run_ahead ? 1 : 0);
+    hipStream_t es = b->eye_overlap ? b->side : st;
     if (b->eye_overlap && (now || ahead)) (void)hipEventRecord(b->ev_fork, st);
     auto eye_stage = [&](hipStream_t s, int mode, int lag) {
         hipLaunchKernelGGL(k_eye_begin_batch, dim3((n + 63) / 64), dim3(64), 0, s, J, n, mode, pass, lag);
@@ -141,14 +145,16 @@ void queue_batch_pass(ppm_batch* b, hipStream_t st, bool run_ahead, PassNeeds ne
             eye_stage(b->ahead, kEyeAhead, b->lag);
         }
         (void)hipEventRecord(b->ring[pass % ppm_batch::kAheadRing], b->ahead);
-        if (pass >= b->ring_from + (uint32_t)b->lag)
-            (void)hipStreamWaitEvent(st, b->ring[(pass - (uint32_t)b->lag) % ppm_batch::kAheadRing], 0);
         b->pass_no++;
     }
     hipLaunchKernelGGL(k_photon_stage_batch, dim3(b->n_blocks[kMapTrace]), dim3(b->trace_block),
                        walk_lds_bytes(b->walk_stack, b->trace_block), st, J, (const BlockRef*)b->d_map[kMapTrace]);
     if (now) {
         if (b->eye_overlap) (void)hipStreamWaitEvent(st, b->ev_join, 0);
+        // the ahead launches of pass - lag and before: every stage an eye pass of this pass needs
+        // began there (eye_begin), so its grid and hit points are complete before k_hash
+        if (run_ahead && pass >= b->ring_from + (uint32_t)b->lag)
+            (void)hipStreamWaitEvent(st, b->ring[(pass - (uint32_t)b->lag) % ppm_batch::kAheadRing], 0);
         hipLaunchKernelGGL(k_hash_batch, dim3(b->n_blocks[kMapPhoton]), dim3(256), 0, st, J,
                            (const BlockRef*)b->d_map[kMapPhoton]);
     }

@@ -149,8 +149,9 @@ struct ppm_batch {
     hipStream_t side = nullptr;     // the eye stage of every pass, beside the photon stage
     hipEvent_t ev_fork = nullptr, ev_join = nullptr;
     // plain launches with eye_overlap: eye stages begun ahead of their eye passes run on `ahead`;
-    // ring[p % kAheadRing] is recorded after the ahead launches of pass p, and pass p waits (before
-    // its photon stage) for those of pass p - lag (lag + 1 = the shortest eye cadence of the jobs)
+    // ring[p % kAheadRing] is recorded after the ahead launches of pass p, and a pass p with eye
+    // passes waits (before its k_hash) for those of pass p - lag (lag + 1 = the shortest eye cadence
+    // of the jobs)
     hipStream_t ahead = nullptr;
     static constexpr int kAheadRing = 32;
     hipEvent_t ring[kAheadRing] = {};

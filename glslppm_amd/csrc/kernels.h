@@ -283,7 +283,7 @@ __global__ void k_grid(Job J, int mode);
 __global__ void k_photon_stage(Job J);
 __global__ void k_hash(Job J);
 __global__ void k_gather(Job J);
-__global__ void k_pass_begin_batch(const Job* jobs, const BlockRef* map, unsigned int pass);
+__global__ void k_pass_begin_batch(const Job* jobs, const BlockRef* map, unsigned int pass, int defer_all);
 __global__ void k_eye_begin_batch(const Job* jobs, int n_jobs, int mode, unsigned int pass, int lag);
 __global__ void k_eye_batch(const Job* jobs, const BlockRef* map, int mode);
 __global__ void k_grid_batch(const Job* jobs, int mode);

@@ -656,7 +656,7 @@ PPM_DEV float xs_frand(unsigned int* s) {
 // FrameCount 0 is the first pass.  A job whose pass budget is spent stops advancing; the budget
 // counts this pass.  The pass's hash table, photon half and the uniforms its gather reads (which
 // runs one launch later) are fixed here.
-PPM_DEV void pass_begin(const Job& J, unsigned int pass) {
+PPM_DEV void pass_begin(const Job& J, unsigned int pass, int defer_all) {
     DevGlobals* g = J.g;
     const unsigned int left = g->passes_left;
     if (left == 0u) {
@@ -681,7 +681,8 @@ PPM_DEV void pass_begin(const Job& J, unsigned int pass) {
     sn.table = g->table;
     sn.half = half;
     // the deposits of an eye pass wait for k_hash unless its eye stage has already set the grid up
-    g->defer = eye && (unsigned long long)g->eye_stages_done < g->eye_samples;
+    // (or, with defer_all, always: a batch that has not yet waited for the stages begun ahead)
+    g->defer = eye && (defer_all || (unsigned long long)g->eye_stages_done < g->eye_samples);
     // for an eye stage begun ahead in this pass: the passes until the next eye pass, and whether
     // the budget lasts until then (left counts this pass)
     const unsigned long long to_next = L - frame % L;
@@ -736,7 +737,7 @@ PPM_DEV int cur_buffer(const DevGlobals* g) { return (int)((g->eye_samples + 1ul
 // k_gather already ran it), with NumPhotons after it for its draw, and sets the pass's uniforms
 // (pass_begin).  The pass's hash table was cleared by the photon stage of the pass before (the
 // glClear of glslppm.cpp:421, 450; see next_table).
-PPM_DEV void begin_pass(const Job& J, int blk, unsigned int pass) {
+PPM_DEV void begin_pass(const Job& J, int blk, unsigned int pass, int defer_all) {
     DevGlobals* g = J.g;
     if (blk == 0 && threadIdx.x < 64) {
         unsigned int* sh = g->emit_shard + threadIdx.x * kEmitStride;
@@ -750,7 +751,7 @@ PPM_DEV void begin_pass(const Job& J, int blk, unsigned int pass) {
             g->lag_pending = lag;
             g->lag_idx = g->half;
             if (lag) g->snap[g->half].np = g->num_photons_f;
-            pass_begin(J, pass);
+            pass_begin(J, pass, defer_all);
         }
     }
 }
@@ -1552,12 +1553,13 @@ PPM_DEV void gather_flush(const Job& J, int blk) { gather(J, blk, (int)(threadId
         body(jobs[m.job], m.block);                                                                 \
     }
 __global__ __launch_bounds__(kBlock) void k_pass_begin(Job J, unsigned int pass) {
-    begin_pass(J, (int)blockIdx.x, pass);
+    begin_pass(J, (int)blockIdx.x, pass, 0);
 }
 __global__ __launch_bounds__(kBlock) void k_pass_begin_batch(const Job* __restrict__ jobs,
-                                                             const BlockRef* __restrict__ map, unsigned int pass) {
+                                                             const BlockRef* __restrict__ map, unsigned int pass,
+                                                             int defer_all) {
     const BlockRef m = map[blockIdx.x];
-    begin_pass(jobs[m.job], m.block, pass);
+    begin_pass(jobs[m.job], m.block, pass, defer_all);
 }
 
 __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(6))) void k_eye(Job J, int mode) {
