@@ -306,6 +306,9 @@ int batch_fetch(ppm_batch* b, int32_t job, float4* radiance, float4* flux_radius
     const Job& J = c->job;
     const size_t Q = (size_t)c->Q;
     hipStream_t up = b->upload;
+    // the job has stopped, but the first pass queued after its last one folds its last emitted
+    // counts into NumPhotons, which the resolve and the counters read: wait for that fold
+    BHIP(b, hipStreamWaitEvent(up, b->ev_begun, 0));
     if (radiance || rgba8) {
         hipLaunchKernelGGL(k_resolve, dim3((c->Q + 255) / 256), dim3(256), 0, up, c->Q, J.hs, (const DevGlobals*)J.g,
                            c->resolve_lin, c->resolve_map);
@@ -364,6 +367,7 @@ int ppm_batch_create(const ppm_batch_config* cfg, ppm_batch** out) {
     if (e == hipSuccess) e = hipEventCreateWithFlags(&b->ev_join, hipEventDisableTiming);
     if (e == hipSuccess) e = hipEventCreate(&b->ev0);
     if (e == hipSuccess) e = hipEventCreate(&b->ev1);
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&b->ev_begun, hipEventDisableTiming);
     if (e != hipSuccess) {
         g_batch_error = std::string("batch streams: ") + hipGetErrorString(e);
         ppm_batch_destroy(b);
@@ -431,17 +435,22 @@ int ppm_batch_pass(ppm_batch* b, uint32_t npasses) {
     if (npasses == 0 || b->n_jobs == 0) return PPM_OK;
     BHIP(b, hipEventRecord(b->ev0, b->stream));
     uint32_t left = npasses;
+    bool begun = false;
     if (b->graph) {
         while (left >= b->passes_per_graph) {
             BHIP(b, hipGraphLaunch(b->graph, b->stream));
             for (uint32_t k = 0; k < b->passes_per_graph; k++) (void)advance_mirrors(b);
             left -= b->passes_per_graph;
+            if (!begun) BHIP(b, hipEventRecord(b->ev_begun, b->stream));
+            begun = true;
         }
     }
     const bool run_ahead = ahead_mode(b);
     for (; left > 0; left--) {
         const PassNeeds need = advance_mirrors(b);
         queue_batch_pass(b, b->stream, run_ahead, need);
+        if (!begun) BHIP(b, hipEventRecord(b->ev_begun, b->stream));
+        begun = true;
     }
     queue_batch_flush(b, b->stream);
     BHIP(b, hipGetLastError());
@@ -592,6 +601,7 @@ void ppm_batch_destroy(ppm_batch* b) {
     if (b->side) (void)hipStreamSynchronize(b->side);
     if (b->ev0) (void)hipEventDestroy(b->ev0);
     if (b->ev1) (void)hipEventDestroy(b->ev1);
+    if (b->ev_begun) (void)hipEventDestroy(b->ev_begun);
     if (b->ev_fork) (void)hipEventDestroy(b->ev_fork);
     if (b->ev_join) (void)hipEventDestroy(b->ev_join);
     for (int k = 0; k < ppm_batch::kAheadRing; k++)

@@ -180,6 +180,9 @@ struct ppm_batch {
     std::vector<Retired> retired;
     std::vector<int32_t> table_ids;  // job id of each entry of d_jobs
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    // recorded after the first pass of the latest ppm_batch_pass call: its k_pass_begin_batch has
+    // folded every job's last emitted counts, so a job that has stopped no longer changes them
+    hipEvent_t ev_begun = nullptr;
     bool timed = false;
     std::string err;
 };

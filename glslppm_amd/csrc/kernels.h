@@ -293,3 +293,4 @@ __global__ void k_gather_batch(const Job* jobs, const BlockRef* map);
 __global__ void k_batch_status(const Job* jobs, int n_jobs, unsigned long long* out);
 __global__ void k_resolve(int count, HitState hs, const DevGlobals* g, float4* linear, float4* mapped);
 __global__ void k_init_state(Job J);
+__global__ void k_seed(float4* eye, int q, float4* photon, int p, const uint4* starts, int chunk);

@@ -651,6 +651,34 @@ PPM_DEV float xs_frand(unsigned int* s) {
     return (float)s[3] * (1.0f / 4294967295.0f);
 }
 
+}  // namespace
+
+// The seed grids (RandomizeTextures, glslppm.cpp:113-140): the eye grid's q texels, then the photon
+// grid's p, four draws of the XORShift stream each, scaled by 2^22; g++ evaluates the TVector4
+// constructor arguments last to first, so w takes the first draw and x the fourth.  Thread c
+// draws texels [c chunk, (c + 1) chunk) of the two grids in sequence from starts[c], the state of
+// the stream at that chunk.
+__global__ void k_seed(float4* eye, int q, float4* photon, int p, const uint4* starts, int chunk) {
+    const long long n = (long long)q + (long long)p;
+    const long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    long long k = c * chunk;
+    if (k >= n) return;
+    const uint4 st = starts[c];
+    unsigned int s[4] = {st.x, st.y, st.z, st.w};
+    const long long end = k + chunk < n ? k + chunk : n;
+    for (; k < end; k++) {
+        const float w = xs_frand(s) * 4194304.0f;
+        const float z = xs_frand(s) * 4194304.0f;
+        const float y = xs_frand(s) * 4194304.0f;
+        const float x = xs_frand(s) * 4194304.0f;
+        const float4 v = make_float4(x, y, z, w);
+        if (k < q) eye[k] = v;
+        else photon[k - q] = v;
+    }
+}
+
+namespace {
+
 // Per-pass uniforms of m_display (glslppm.cpp:214-287): eye pass when FrameCount % MaxPathLength
 // is 0, with two AAOffset draws from the job's XORShift stream and NumEyeSamples = count + 1;
 // FrameCount 0 is the first pass.  A job whose pass budget is spent stops advancing; the budget

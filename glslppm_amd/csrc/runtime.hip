@@ -108,17 +108,95 @@ inline float xs_frand(uint32_t* s) {
     return (float)s[3] * (1.0f / 4294967295.0f);
 }
 
-// RandomizeTextures (glslppm.cpp:113-140): four draws per texel scaled by 2^22.  g++ evaluates
-// the TVector4 constructor arguments last to first, so w takes the first draw and x the fourth.
-void seed_grid(uint32_t* s, std::vector<float4>& out, size_t n) {
-    out.resize(n);
-    for (size_t i = 0; i < n; i++) {
-        const float w = xs_frand(s) * 4194304.0f;
-        const float z = xs_frand(s) * 4194304.0f;
-        const float y = xs_frand(s) * 4194304.0f;
-        const float x = xs_frand(s) * 4194304.0f;
-        out[i] = make_float4(x, y, z, w);
+// The seed grids of RandomizeTextures (glslppm.cpp:113-140) take four draws of the XORShift
+// stream per texel, the eye grid's Q texels and then the photon grid's P.  k_seed draws them on the
+// device, one thread per chunk of kSeedChunk texels, each from its chunk's start state.  The state
+// update of xs_frand is linear over GF(2): the state after n steps is M^n s for a 128 x 128 bit
+// matrix M, so the start of chunk c + 1 is J s_c with J = M^(4 kSeedChunk), without stepping
+// through the draws between (drawing a 1024^2 job's 2M texels one after another took 39 ms of host
+// time, most of the server's admission work).
+constexpr int kSeedChunk = 1024;
+
+struct Bits128 {
+    uint32_t w[4];
+};
+
+// A 128 x 128 bit matrix by columns: column k is the image of the state with only bit k set
+// (bit k is bit k % 32 of word k / 32, words in XORShift state order).
+struct BitMatrix {
+    Bits128 col[128];
+};
+
+Bits128 xs_step_bits(Bits128 s) {
+    const uint32_t t = s.w[0] ^ (s.w[0] << 11);
+    return Bits128{{s.w[1], s.w[2], s.w[3], (s.w[3] ^ (s.w[3] >> 19)) ^ (t ^ (t >> 8))}};
+}
+
+Bits128 apply_bits(const BitMatrix& a, const Bits128& v) {
+    Bits128 r = {{0, 0, 0, 0}};
+    for (int k = 0; k < 128; k++)
+        if ((v.w[k >> 5] >> (k & 31)) & 1u)
+            for (int i = 0; i < 4; i++) r.w[i] ^= a.col[k].w[i];
+    return r;
+}
+
+// J = M^(4 kSeedChunk), by squaring M (4 kSeedChunk is a power of two), as 16 tables of 256
+// entries: entry v of table b is J applied to byte b of the state being v, so that J s is the XOR
+// of 16 entries; built once.
+struct JumpTable {
+    Bits128 t[16][256];
+};
+
+const JumpTable& seed_chunk_jump() {
+    static const JumpTable* table = [] {
+        static_assert((4 * kSeedChunk & (4 * kSeedChunk - 1)) == 0, "the chunk's draws must be a power of two");
+        static BitMatrix m;
+        for (int k = 0; k < 128; k++) {
+            Bits128 e = {{0, 0, 0, 0}};
+            e.w[k >> 5] = 1u << (k & 31);
+            m.col[k] = xs_step_bits(e);
+        }
+        for (int d = 1; d < 4 * kSeedChunk; d *= 2) {
+            static BitMatrix sq;
+            for (int k = 0; k < 128; k++) sq.col[k] = apply_bits(m, m.col[k]);
+            m = sq;
+        }
+        static JumpTable jt;
+        for (int b = 0; b < 16; b++) {
+            jt.t[b][0] = Bits128{{0, 0, 0, 0}};
+            for (int v = 1; v < 256; v++) {
+                const int low = __builtin_ctz((unsigned)v);
+                for (int i = 0; i < 4; i++) jt.t[b][v].w[i] = jt.t[b][v & (v - 1)].w[i] ^ m.col[8 * b + low].w[i];
+            }
+        }
+        return &jt;
+    }();
+    return *table;
+}
+
+Bits128 jump_bits(const JumpTable& jt, const Bits128& v) {
+    Bits128 r = {{0, 0, 0, 0}};
+    for (int b = 0; b < 16; b++) {
+        const Bits128& e = jt.t[b][(v.w[b >> 2] >> ((b & 3) * 8)) & 255u];
+        for (int i = 0; i < 4; i++) r.w[i] ^= e.w[i];
     }
+    return r;
+}
+
+// The start state of every chunk of n texels drawn from state s, and s advanced past all of them.
+void seed_starts(uint32_t* s, size_t n, std::vector<uint4>& starts) {
+    const JumpTable& jump = seed_chunk_jump();
+    const size_t chunks = (n + kSeedChunk - 1) / kSeedChunk;
+    starts.resize(chunks);
+    Bits128 b = {{s[0], s[1], s[2], s[3]}};
+    for (size_t c = 0; c < chunks; c++) {
+        starts[c] = make_uint4(b.w[0], b.w[1], b.w[2], b.w[3]);
+        if (c + 1 < chunks) b = jump_bits(jump, b);
+    }
+    // the last chunk's draws, one by one
+    for (size_t i = (chunks - 1) * kSeedChunk; i < n; i++)
+        for (int d = 0; d < 4; d++) b = xs_step_bits(b);
+    for (int i = 0; i < 4; i++) s[i] = b.w[i];
 }
 
 bool is_pow2(int v) { return v > 0 && (v & (v - 1)) == 0; }
@@ -738,8 +816,9 @@ size_t state_footprint(const ppm_params* p) {
         (size_t)max_tile_blocks(p) * 4, (size_t)max_tile_blocks(p) * 4,  // tile costs and order
         // photons: pos, flux, dir (two halves each), rng, state, slot
         P * 32, P * 32, P * 32, P * 16, P * 8, P * 4,
-        // hash (count, representative) records, three tables; globals; resolve targets
-        H * 24, sizeof(DevGlobals), Q * 16, Q * 16};
+        // hash (count, representative) records, three tables; globals; resolve targets; the seed
+        // chunks' start states
+        H * 24, sizeof(DevGlobals), Q * 16, Q * 16, (Q + P + 1023) / 1024 * 16};
     size_t tot = 0;
     for (size_t b : sizes) tot += Arena::round_size(b ? b : 16);
     return tot;
@@ -973,13 +1052,16 @@ int create_context(ppm_ctx* c, std::shared_ptr<SceneDev> scene, const ppm_camera
     J.h.height = p->hash_height;
     J.h.num_f = (float)c->H;
 
-    // per-pixel and per-photon generator states, then the stream's state for the AAOffset draws
-    std::vector<float4> eye_seeds, photon_seeds;
-    seed_grid(xs, eye_seeds, Q);
-    seed_grid(xs, photon_seeds, P);
-    // the first eye stage (buffer 0) reads its generator states from buffer 1
-    HIP_TRY(c, hipMemcpyAsync(hs.rng + Q, eye_seeds.data(), Q * sizeof(float4), hipMemcpyHostToDevice, up));
-    HIP_TRY(c, hipMemcpyAsync(ps.rng, photon_seeds.data(), P * sizeof(float4), hipMemcpyHostToDevice, up));
+    // per-pixel and per-photon generator states (the first eye stage, buffer 0, reads its states
+    // from buffer 1), then the stream's state for the AAOffset draws
+    std::vector<uint4> starts;
+    seed_starts(xs, Q + P, starts);
+    uint4* d_starts = nullptr;
+    if ((rc = dev_alloc(c, &d_starts, starts.size()))) return rc;
+    HIP_TRY(c, hipMemcpyAsync(d_starts, starts.data(), starts.size() * sizeof(uint4), hipMemcpyHostToDevice, up));
+    hipLaunchKernelGGL(k_seed, dim3((unsigned)((starts.size() + 255) / 256)), dim3(256), 0, up, hs.rng + Q, (int)Q,
+                       ps.rng, (int)P, (const uint4*)d_starts, kSeedChunk);
+    HIP_TRY(c, hipGetLastError());
     DevGlobals g0;
     std::memset(&g0, 0, sizeof(g0));
     std::memcpy(g0.xs, xs, sizeof(xs));

@@ -9,7 +9,8 @@
 //   3. queue the next step (passes_per_step batched passes and the next snapshot) at once, so the
 //      device stays busy;
 //   4. copy out the results of the jobs that reached their target (on the upload stream, while
-//      the new step runs; a finished job's kernels do nothing, so its state is final) and remove
+//      the new step runs; a finished job's kernels do nothing once the new step's first pass has
+//      folded its last emitted counts, which the copy waits for, so its state is final) and remove
 //      them from the batch, which frees their memory for later admissions.
 // A job's target pass count is its device-side pass budget, so a step may run past it for other
 // jobs without changing it.  Jobs never share state, so a job's results are those of the same
