@@ -280,6 +280,11 @@ int batch_add_resident(ppm_batch* b, std::shared_ptr<SceneDev> scene, const ppm_
     return PPM_OK;
 }
 
+bool batch_job_has_passes(const ppm_batch* b, int32_t job) {
+    if (!b || job < 0 || (size_t)job >= b->slots.size() || !b->slots[(size_t)job]) return false;
+    return b->slots[(size_t)job]->batch_left != 0u;
+}
+
 int batch_prepare(ppm_batch* b) {
     BHIP(b, hipSetDevice(b->device));
     return rebuild(b);

@@ -195,6 +195,8 @@ int batch_add_resident(ppm_batch* b, std::shared_ptr<SceneDev> scene, const ppm_
                        uint32_t passes, int32_t* job);
 // Rebuilds the batch tables if jobs were added or removed.
 int batch_prepare(ppm_batch* b);
+// Whether a job still has passes of its budget left after the passes queued so far.
+bool batch_job_has_passes(const ppm_batch* b, int32_t job);
 // Queues, after the passes queued so far, a copy of every table job's (frame, passes_left) into
 // host_out[2 i], host_out[2 i + 1] (pinned, >= 2 x jobs entries; dev_tmp the same in HBM) and
 // records `done` after it.  *ids receives the job id of each entry.

@@ -4,10 +4,11 @@
 //   1. admit queued jobs, first come first served, skipping a job that does not fit the HBM
 //      budget or the job limit (its scene is uploaded with its first job; the uploads and the
 //      job's initialisation run on the batch's upload stream, overlapping the step in flight);
-//   2. wait for the step in flight and read its status snapshot (frame and pass budget of every
-//      job, copied to pinned memory behind the step's passes);
-//   3. queue the next step (passes_per_step batched passes and the next snapshot) at once, so the
-//      device stays busy;
+//   2. queue the next step (passes_per_step batched passes and its status snapshot: frame and pass
+//      budget of every job, copied to pinned memory behind the step's passes) while fewer than two
+//      steps are in flight and some job has passes left, so that one step always waits behind the
+//      one running;
+//   3. read the oldest step's status once two are in flight (or nothing more was queued);
 //   4. copy out the results of the jobs that reached their target (on the upload stream, while
 //      the new step runs; a finished job's kernels do nothing once the new step's first pass has
 //      folded its last emitted counts, which the copy waits for, so its state is final) and remove
@@ -79,9 +80,11 @@ struct ppm_server {
     std::vector<int64_t> resident;
     std::map<int32_t, int64_t> by_bjob;
     size_t device_bytes = 0;
-    unsigned long long* d_status = nullptr;
-    unsigned long long* h_status = nullptr;
-    hipEvent_t ev_status = nullptr;
+    // status snapshots of the steps in flight (kSteps at most), one slot per step
+    static constexpr int kSteps = 2;
+    unsigned long long* d_status[kSteps] = {nullptr, nullptr};
+    unsigned long long* h_status[kSteps] = {nullptr, nullptr};
+    hipEvent_t ev_status[kSteps] = {nullptr, nullptr};
     ppm_server_stats st;
     std::string err;
 
@@ -253,13 +256,14 @@ void fail_all(ppm_server* S, int code, const std::string& msg) {
 
 void worker_main(ppm_server* S) {
     (void)hipSetDevice(S->cfg.device);
-    bool in_flight = false;
-    std::vector<int32_t> snap_ids;
+    constexpr int kSteps = ppm_server::kSteps;
+    int head = 0, in_flight = 0;  // the oldest step's status slot, steps queued and not yet read
+    std::vector<int32_t> snap_ids[kSteps];
     double busy_mark = -1.0;
     for (;;) {
         {
             std::unique_lock<std::mutex> lk(S->mu);
-            while (!S->stop && S->queue.empty() && S->resident.empty() && !in_flight) S->cv_work.wait(lk);
+            while (!S->stop && S->queue.empty() && S->resident.empty() && in_flight == 0) S->cv_work.wait(lk);
             if (S->stop) break;
         }
         {
@@ -267,48 +271,65 @@ void worker_main(ppm_server* S) {
             drop_scenes_locked(S, 0);
         }
         admit(S);
+        // queue the next step while a status slot is free and some resident job has passes left
+        // (the batch's mirror of the queued passes), so that a step is always queued behind the
+        // one running while the host admits, reads statuses and copies results out
+        bool queued = false;
+        if (in_flight < kSteps) {
+            bool work_left = false;
+            {
+                std::lock_guard<std::mutex> lk(S->mu);
+                for (int64_t id : S->resident)
+                    if (ppm_internal::batch_job_has_passes(S->b, S->jobs[id]->bjob)) {
+                        work_left = true;
+                        break;
+                    }
+                const double t = S->now();
+                if (busy_mark >= 0.0) S->st.busy_seconds += t - busy_mark;
+                busy_mark = S->resident.empty() ? -1.0 : t;
+            }
+            if (work_left) {
+                const int slot = (head + in_flight) % kSteps;
+                int rc = ppm_batch_pass(S->b, S->cfg.passes_per_step);
+                if (rc == PPM_OK)
+                    rc = ppm_internal::batch_snapshot(S->b, S->d_status[slot], S->h_status[slot], S->ev_status[slot],
+                                                      &snap_ids[slot]);
+                if (rc != PPM_OK) {
+                    fail_all(S, rc, S->b->err);
+                    break;
+                }
+                in_flight++;
+                queued = true;
+                std::lock_guard<std::mutex> lk(S->mu);
+                S->st.steps++;
+            }
+        }
+        // read the oldest step's status once the pipeline is full or nothing more was queued
         std::vector<int64_t> finished;
-        if (in_flight) {
-            if (hipEventSynchronize(S->ev_status) != hipSuccess) {
+        if (in_flight > 0 && (in_flight == kSteps || !queued)) {
+            if (hipEventSynchronize(S->ev_status[head]) != hipSuccess) {
                 fail_all(S, PPM_E_HIP, "device error while running the batch");
                 break;
             }
-            in_flight = false;
             std::lock_guard<std::mutex> lk(S->mu);
             const double t = S->now();
-            for (size_t i = 0; i < snap_ids.size(); i++) {
-                auto it = S->by_bjob.find(snap_ids[i]);
+            const std::vector<int32_t>& ids = snap_ids[head];
+            const unsigned long long* st = S->h_status[head];
+            for (size_t i = 0; i < ids.size(); i++) {
+                auto it = S->by_bjob.find(ids[i]);
                 if (it == S->by_bjob.end()) continue;
                 SJob& J = *S->jobs[it->second];
-                const unsigned long long frames = S->h_status[2 * i], left = S->h_status[2 * i + 1];
+                const unsigned long long frames = st[2 * i], left = st[2 * i + 1];
                 J.passes = frames;
                 if (frames >= 1 && J.t_first < 0.0) J.t_first = t;
-                if (left == 0ull && frames >= J.target) finished.push_back(J.id);
+                if (left == 0ull && frames >= J.target && J.state == PPM_JOB_RUNNING) finished.push_back(J.id);
             }
-        }
-        bool work_left;
-        {
-            std::lock_guard<std::mutex> lk(S->mu);
-            work_left = S->resident.size() > finished.size();
-            const double t = S->now();
-            if (busy_mark >= 0.0) S->st.busy_seconds += t - busy_mark;
-            busy_mark = S->resident.empty() ? -1.0 : t;
-        }
-        if (work_left) {
-            int rc = ppm_batch_pass(S->b, S->cfg.passes_per_step);
-            if (rc == PPM_OK)
-                rc = ppm_internal::batch_snapshot(S->b, S->d_status, S->h_status, S->ev_status, &snap_ids);
-            if (rc != PPM_OK) {
-                fail_all(S, rc, S->b->err);
-                break;
-            }
-            in_flight = true;
-            std::lock_guard<std::mutex> lk(S->mu);
-            S->st.steps++;
+            head = (head + 1) % kSteps;
+            in_flight--;
         }
         for (int64_t id : finished) retire(S, id);
     }
-    if (in_flight) (void)hipEventSynchronize(S->ev_status);
+    for (int k = 0; k < in_flight; k++) (void)hipEventSynchronize(S->ev_status[(head + k) % kSteps]);
 }
 
 }  // namespace
@@ -349,13 +370,19 @@ int ppm_server_create(const ppm_server_config* cfg, ppm_server** out) {
     S->b = b;
     std::memset(&S->st, 0, sizeof(S->st));
     S->t0 = std::chrono::steady_clock::now();
-    hipError_t e = hipMalloc(&S->d_status, (size_t)cfg->max_jobs * 16);
-    if (e == hipSuccess) e = hipHostMalloc(&S->h_status, (size_t)cfg->max_jobs * 16, hipHostMallocDefault);
-    if (e == hipSuccess) e = hipEventCreateWithFlags(&S->ev_status, hipEventDisableTiming);
+    hipError_t e = hipSuccess;
+    for (int k = 0; k < ppm_server::kSteps && e == hipSuccess; k++) {
+        e = hipMalloc(&S->d_status[k], (size_t)cfg->max_jobs * 16);
+        if (e == hipSuccess) e = hipHostMalloc(&S->h_status[k], (size_t)cfg->max_jobs * 16, hipHostMallocDefault);
+        if (e == hipSuccess) e = hipEventCreateWithFlags(&S->ev_status[k], hipEventDisableTiming);
+    }
     if (e != hipSuccess) {
         g_server_error = std::string("server buffers: ") + hipGetErrorString(e);
-        if (S->d_status) (void)hipFree(S->d_status);
-        if (S->h_status) (void)hipHostFree(S->h_status);
+        for (int k = 0; k < ppm_server::kSteps; k++) {
+            if (S->ev_status[k]) (void)hipEventDestroy(S->ev_status[k]);
+            if (S->d_status[k]) (void)hipFree(S->d_status[k]);
+            if (S->h_status[k]) (void)hipHostFree(S->h_status[k]);
+        }
         ppm_batch_destroy(b);
         delete S;
         return PPM_E_HIP;
@@ -535,9 +562,11 @@ void ppm_server_destroy(ppm_server* S) {
         for (auto& kv : S->scenes) kv.second.dev.reset();
     }
     ppm_batch_destroy(S->b);
-    if (S->ev_status) (void)hipEventDestroy(S->ev_status);
-    if (S->d_status) (void)hipFree(S->d_status);
-    if (S->h_status) (void)hipHostFree(S->h_status);
+    for (int k = 0; k < ppm_server::kSteps; k++) {
+        if (S->ev_status[k]) (void)hipEventDestroy(S->ev_status[k]);
+        if (S->d_status[k]) (void)hipFree(S->d_status[k]);
+        if (S->h_status[k]) (void)hipHostFree(S->h_status[k]);
+    }
     delete S;
 }
 
