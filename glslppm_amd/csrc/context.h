@@ -60,6 +60,7 @@ struct PackedScene {
     float light_area = 0.0f;
     float bsphere[4] = {0, 0, 0, 0};
     float slab_eps = 0x1p-126f;  // DevScene::slab_eps
+    int leaf_box_stored = 0;     // DevScene::leaf_box_stored
     ppm_camera camera;
     size_t device_bytes() const;
 };

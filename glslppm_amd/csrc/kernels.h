@@ -11,9 +11,11 @@
 //                                  member's box as above, then its record, ~triangle of a leaf, or
 //                                  kWideEmpty), members in pre-order; record 0 is the top
 //           wide_pos[6][R] uint4   the members' pre-order positions (N: empty)
-//           tri_isect[T][4] float4 (v0, brdf), (v1 - v0, leaf min.x), (v2 - v0, leaf min.y),
+//           tri_isect[T][4] float4 (v0, brdf), (v1, leaf min.x), (v2, leaf min.y),
 //                                  (leaf min.z, leaf max.xyz): one 64-byte record per triangle
-//                                  with the exact box of its leaf
+//                                  with the exact box of its leaf, whose last 16 bytes a walk
+//                                  reads only when some leaf's box is not the box of its
+//                                  triangle's vertices (DevScene::leaf_box_stored)
 //           tri_shade[T][6] float4 (v0, n0.x) (v1, n0.y) (v2, n0.z) (n1, n2.x)
 //                                  (n2.y, n2.z, gn.x, gn.y) (gn.z, material, 0, 0)
 //           mats[M][2] float4      (rgb, eta), (brdf, g, 0, 0)
@@ -60,6 +62,10 @@ struct DevScene {
     // 2^-22 times the largest node bound magnitude capped at 2^17 (at least 2^-126): scales the
     // slack of the four-wide record test's fused slab values
     float slab_eps;
+    // 0: every leaf's box is the box of its triangle's three vertices (min and max per axis, as
+    // the reference's build makes it, bvh.cpp:32-57 from mesh.cpp:439-446), taken from the
+    // vertices; 1: the leaf boxes are read from the records
+    int leaf_box_stored;
 };
 
 struct DevCamera {

@@ -72,10 +72,11 @@ struct Closest {
     float t;
 };
 
-// The Moller-Trumbore terms of a triangle record for a ray (eyeraytrace.fs:118-147,
-// photontrace.fs:174-205), in the shaders' operation order.
+// The Moller-Trumbore terms of a triangle record (vertices v0, v1, v2 in q0, q1, q2) for a ray
+// (eyeraytrace.fs:118-147, photontrace.fs:174-205), in the shaders' operation order, the edges
+// included (eyeraytrace.fs:125-126).
 PPM_DEV void tri_terms(float4 q0, float4 q1, float4 q2, f3 org, f3 dir, float& det, float& u, float& v, float& t) {
-    const f3 e0 = xyz(q1), e1 = xyz(q2);
+    const f3 e0 = xyz(q1) - xyz(q0), e1 = xyz(q2) - xyz(q0);
     const f3 pv = cross3(dir, e1);
     det = dot3(e0, pv);
     const f3 tv = org - xyz(q0);
@@ -115,6 +116,20 @@ PPM_DEV void hit_barycentrics(const DevScene& s, f3 org, f3 dir, Hit& h) {
 
 // Leaves a lane may hold before its triangle tests run.
 constexpr int kLeafQueue = 2;
+
+// The exact box of a leaf: read from its triangle record (q1.w, q2.w, q3) or, when every leaf's
+// box is its triangle's vertex box, the smallest and largest vertex coordinate per axis (exact, no
+// rounding; min and max of the same three values are the reference's box up to the sign of a
+// zero, which no slab comparison sees).
+PPM_DEV void leaf_box(bool stored, float4 q0, float4 q1, float4 q2, float4 q3, f3& lo, f3& hi) {
+    if (stored) {
+        lo = mk3(q1.w, q2.w, q3.x);
+        hi = mk3(q3.y, q3.z, q3.w);
+    } else {
+        lo = mk3(fminf(fminf(q0.x, q1.x), q2.x), fminf(fminf(q0.y, q1.y), q2.y), fminf(fminf(q0.z, q1.z), q2.z));
+        hi = mk3(fmaxf(fmaxf(q0.x, q1.x), q2.x), fmaxf(fmaxf(q0.y, q1.y), q2.y), fmaxf(fmaxf(q0.z, q1.z), q2.z));
+    }
+}
 
 // float(binary16 in the low / high half of w) * r with one rounding: v_fma_mix_f32 widens the
 // half exactly and adds -0, which leaves every product (signed zeros included) unchanged, so the
@@ -440,25 +455,30 @@ PPM_DEV void walk(const DevScene& s, int face, f3 org, f3 dir, f3 rcp_in, bool c
         // first test's result, its data does not
         float4 x0 = make_float4(0.0f, 0.0f, 0.0f, 0.0f), x1 = x0, x2 = x0, x3 = x0;
         float4 y0 = x0, y1 = x0, y2 = x0, y3 = x0;
+        const bool stored = s.leaf_box_stored != 0;
         if (nq > 0) {
             const float4* T = s.tri_isect + 4 * (size_t)qa;
             x0 = T[0];
             x1 = T[1];
             x2 = T[2];
-            x3 = T[3];
+            if (stored) x3 = T[3];
         }
         if (nq > 1) {
             const float4* T = s.tri_isect + 4 * (size_t)qb;
             y0 = T[0];
             y1 = T[1];
             y2 = T[2];
-            y3 = T[3];
+            if (stored) y3 = T[3];
         }
-        // the leaf's exact box: (x1.w, x2.w, x3.x) to (x3.y, x3.z, x3.w)
-        if (nq > 0 && box_test_face(wr4, face, org, mk3(x1.w, x2.w, x3.x), mk3(x3.y, x3.z, x3.w), h.t))
-            tri_test<PHOTON>(x0, x1, x2, org, dir, cull, qa, h);
-        if (nq > 1 && box_test_face(wr4, face, org, mk3(y1.w, y2.w, y3.x), mk3(y3.y, y3.z, y3.w), h.t))
-            tri_test<PHOTON>(y0, y1, y2, org, dir, cull, qb, h);
+        f3 lo, hi;
+        if (nq > 0) {
+            leaf_box(stored, x0, x1, x2, x3, lo, hi);
+            if (box_test_face(wr4, face, org, lo, hi, h.t)) tri_test<PHOTON>(x0, x1, x2, org, dir, cull, qa, h);
+        }
+        if (nq > 1) {
+            leaf_box(stored, y0, y1, y2, y3, lo, hi);
+            if (box_test_face(wr4, face, org, lo, hi, h.t)) tri_test<PHOTON>(y0, y1, y2, org, dir, cull, qb, h);
+        }
         nq = 0;
     }
     hit.tri = h.tri;

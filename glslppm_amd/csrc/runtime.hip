@@ -401,17 +401,25 @@ int pack_nodes(const ppm_scene* s, std::vector<uint4>& out, std::vector<uint4>& 
     return PPM_OK;
 }
 
+// The smallest and largest of three coordinates, as the walk takes them (fminf / fmaxf).
+float min3f(float a, float b, float c) { return std::fmin(std::fmin(a, b), c); }
+float max3f(float a, float b, float c) { return std::fmax(std::fmax(a, b), c); }
+
 // Triangle records of textureTriangles (gpurt.cpp:115-162) in the form the kernels read:
 // normal z as the shaders rebuild it, sign(sgn(n.z)(mat+1)) sqrt(|1 - x^2 - y^2|)
 // (eyeraytrace.fs:268-270), and the geometric normal normalize(cross(v1-v0, v2-v0)).  The
-// intersection record also holds the exact box of the triangle's leaf.
-void pack_triangles(const ppm_scene* s, std::vector<float4>& isect, std::vector<float4>& shade) {
+// intersection record holds the three vertices (the walk forms the edges as the shaders do,
+// eyeraytrace.fs:125-126) and the exact box of the triangle's leaf.  Returns 1 when some leaf's
+// box is not the box of its triangle's vertices, so that the walk must read the stored boxes
+// (equal as floats: bounds that differ only in the sign of a zero give the same comparisons).
+int pack_triangles(const ppm_scene* s, std::vector<float4>& isect, std::vector<float4>& shade) {
     const int T = s->num_triangles;
     isect.resize((size_t)T * 4);
     shade.resize((size_t)T * 6);
     std::vector<int> leaf(T, -1);
     for (int k = 0; k < s->num_nodes; k++)
         if (s->node_triangle[k] >= 0) leaf[s->node_triangle[k]] = k;
+    int stored = 0;
     for (int t = 0; t < T; t++) {
         const float* p = s->positions + (size_t)t * 9;
         const float* n = s->normals + (size_t)t * 9;
@@ -432,9 +440,12 @@ void pack_triangles(const ppm_scene* s, std::vector<float4>& isect, std::vector<
         const float inv = 1.0f / std::sqrt(c[0] * c[0] + c[1] * c[1] + c[2] * c[2]);
         const float gn[3] = {c[0] * inv, c[1] * inv, c[2] * inv};
         const float* lb = s->node_bounds + (size_t)leaf[t] * 6;
+        for (int k = 0; k < 3; k++)
+            if (!(lb[k] == min3f(p[k], p[3 + k], p[6 + k]) && lb[3 + k] == max3f(p[k], p[3 + k], p[6 + k])))
+                stored = 1;
         isect[(size_t)t * 4 + 0] = make_float4(p[0], p[1], p[2], as_float(brdf));
-        isect[(size_t)t * 4 + 1] = make_float4(e0[0], e0[1], e0[2], lb[0]);
-        isect[(size_t)t * 4 + 2] = make_float4(e1[0], e1[1], e1[2], lb[1]);
+        isect[(size_t)t * 4 + 1] = make_float4(p[3], p[4], p[5], lb[0]);
+        isect[(size_t)t * 4 + 2] = make_float4(p[6], p[7], p[8], lb[1]);
         isect[(size_t)t * 4 + 3] = make_float4(lb[2], lb[3], lb[4], lb[5]);
         float4* o = &shade[(size_t)t * 6];
         o[0] = make_float4(p[0], p[1], p[2], rn[0][0]);
@@ -444,6 +455,7 @@ void pack_triangles(const ppm_scene* s, std::vector<float4>& isect, std::vector<
         o[4] = make_float4(rn[2][1], rn[2][2], gn[0], gn[1]);
         o[5] = make_float4(gn[2], as_float(mat), 0.0f, 0.0f);
     }
+    return stored;
 }
 
 hipEvent_t pooled_event(ppm_ctx* c) {
@@ -852,7 +864,7 @@ SceneDev::~SceneDev() {
 int pack_scene(const ppm_scene* s, PackedScene* out) {
     int rc;
     if ((rc = pack_nodes(s, out->nodes, out->wide, out->wide_pos, &out->wide_stride))) return rc;
-    pack_triangles(s, out->isect, out->shade);
+    out->leaf_box_stored = pack_triangles(s, out->isect, out->shade);
     out->T = s->num_triangles;
     out->N = s->num_nodes;
     out->M = s->num_materials;
@@ -957,6 +969,7 @@ int upload_scene(const PackedScene& ps, Arena* arena, hipStream_t up, hipStream_
     sc.num_materials = ps.M;
     sc.material_rcp = 1.0f / (float)ps.M;
     sc.slab_eps = ps.slab_eps;
+    sc.leaf_box_stored = ps.leaf_box_stored;
     *out = sd;
     return PPM_OK;
 }
