@@ -16,8 +16,9 @@
 //                                  with the exact box of its leaf, whose last 16 bytes a walk
 //                                  reads only when some leaf's box is not the box of its
 //                                  triangle's vertices (DevScene::leaf_box_stored)
-//           tri_shade[T][6] float4 (v0, n0.x) (v1, n0.y) (v2, n0.z) (n1, n2.x)
-//                                  (n2.y, n2.z, gn.x, gn.y) (gn.z, material, 0, 0)
+//           tri_shade[T][4] float4 (n0, 0) (n1, 0) (n2, 0) (gn, material): the shading normals,
+//                                  the geometric normal and the material (the vertices are
+//                                  tri_isect's)
 //           mats[M][2] float4      (rgb, eta), (brdf, g, 0, 0)
 //           tri_tex[T][2] float4   (u0, v0, u1, v1), (u2, v2, 0, 0)     only with textures
 //           volume[M][S][S] uchar4 RGB8 + pad of every material (S = 512)  only with textures

@@ -64,6 +64,7 @@ struct Hit {
     int tri;    // -1: no hit
     float t;
     float bx, by, bz;  // Barycentric = uvt.ywx
+    f3 pos;            // Result.pos = V2 B.x + V0 B.y + V1 B.z (eyeraytrace.fs:273)
 };
 
 // The closest hit while a walk runs; the barycentrics of the final one are computed at the end.
@@ -107,11 +108,13 @@ PPM_DEV void tri_test(float4 q0, float4 q1, float4 q2, f3 org, f3 dir, bool cull
 PPM_DEV void hit_barycentrics(const DevScene& s, f3 org, f3 dir, Hit& h) {
     if (h.tri < 0) return;
     const float4* T = s.tri_isect + 4 * (size_t)h.tri;
+    const float4 q0 = T[0], q1 = T[1], q2 = T[2];
     float det, u, v, t;
-    tri_terms(T[0], T[1], T[2], org, dir, det, u, v, t);
+    tri_terms(q0, q1, q2, org, dir, det, u, v, t);
     h.bx = v;
     h.by = (1.0f - u) - v;
     h.bz = u;
+    h.pos = (xyz(q2) * h.bx + xyz(q0) * h.by) + xyz(q1) * h.bz;
 }
 
 // Leaves a lane may hold before its triangle tests run.
@@ -507,14 +510,13 @@ struct Surface {
 // Shading fetch after the walk (eyeraytrace.fs:256-287).
 PPM_DEV Surface shade(const DevScene& s, const Hit& h) {
     Surface r;
-    const float4* t = s.tri_shade + (size_t)6 * h.tri;
-    const float4 a = t[0], b = t[1], c = t[2], d = t[3], e = t[4], f = t[5];
-    const f3 V0 = xyz(a), V1 = xyz(b), V2 = xyz(c);
-    const f3 N0 = mk3(a.w, b.w, c.w), N1 = xyz(d), N2 = mk3(d.w, e.x, e.y);
-    r.pos = (V2 * h.bx + V0 * h.by) + V1 * h.bz;
+    const float4* t = s.tri_shade + (size_t)4 * h.tri;
+    const float4 a = t[0], b = t[1], c = t[2], d = t[3];
+    const f3 N0 = xyz(a), N1 = xyz(b), N2 = xyz(c);
+    r.pos = h.pos;
     r.nrm = normalize3((N2 * h.bx + N0 * h.by) + N1 * h.bz);
-    r.gnrm = mk3(e.z, e.w, f.x);
-    r.mat = __float_as_int(f.y);
+    r.gnrm = xyz(d);
+    r.mat = __float_as_int(d.w);
     const float4 m0 = s.mats[2 * r.mat], m1 = s.mats[2 * r.mat + 1];
     r.col = xyz(m0);
     r.eta = m0.w;
@@ -1099,11 +1101,12 @@ PPM_DEV void light_sample(const DevScene& s, float4& rndv, f3& flux, f3& dir, f3
     const float a = 1.0f - t;
     const float c = sv * t;
     const float b = t - c;
-    const float4* tr = s.tri_shade + (size_t)6 * li;
-    const float4 q0 = tr[0], q1 = tr[1], q2 = tr[2], q3 = tr[3], q4 = tr[4], q5 = tr[5];
-    const f3 V0 = xyz(q0), V1 = xyz(q1), V2 = xyz(q2);
-    const f3 N0 = mk3(q0.w, q1.w, q2.w), N1 = xyz(q3), N2 = mk3(q3.w, q4.x, q4.y);
-    const int mat = __float_as_int(q5.y);
+    const float4* tv = s.tri_isect + (size_t)4 * li;
+    const float4* tr = s.tri_shade + (size_t)4 * li;
+    const f3 V0 = xyz(tv[0]), V1 = xyz(tv[1]), V2 = xyz(tv[2]);
+    const float4 q3 = tr[3];
+    const f3 N0 = xyz(tr[0]), N1 = xyz(tr[1]), N2 = xyz(tr[2]);
+    const int mat = __float_as_int(q3.w);
     const float4 m0 = s.mats[2 * mat];
     const float le = s.light_area * 3.141592f;
     flux = mk3(m0.x * le, m0.y * le, m0.z * le);
@@ -1116,7 +1119,7 @@ PPM_DEV void light_sample(const DevScene& s, float4& rndv, f3& flux, f3& dir, f3
     }
     org = (V0 * a + V1 * b) + V2 * c;
     const f3 nrm = normalize3((N0 * a + N1 * b) + N2 * c);
-    const f3 gnrm = mk3(q4.z, q4.w, q5.x);
+    const f3 gnrm = xyz(q3);
     org = org + gnrm * 1.0e-5f;
     float rx = gpu_rnd(rndv);
     float ry = gpu_rnd(rndv);
@@ -1196,10 +1199,9 @@ PPM_DEV bool photon_gen(const DevScene& s, const PhotonParams& pp, const PhotonS
         out_bounce = 0;
         cont = true;
     } else {
-        const float4* tr = s.tri_shade + (size_t)6 * st.x;
-        const float4 q4 = tr[4], q5 = tr[5];
-        const f3 gnrm = mk3(q4.z, q4.w, q5.x);
-        const int mat = __float_as_int(q5.y);
+        const float4 q3 = s.tri_shade[(size_t)4 * st.x + 3];
+        const f3 gnrm = xyz(q3);
+        const int mat = __float_as_int(q3.w);
         const float4 m0 = s.mats[2 * mat], m1 = s.mats[2 * mat + 1];
         f3 col = xyz(m0);
         const int brdf = (int)m1.x;
@@ -1208,7 +1210,8 @@ PPM_DEV bool photon_gen(const DevScene& s, const PhotonParams& pp, const PhotonS
         if (s.tri_tex) {
             // barycentrics of the stored hit from the three sub-triangle areas, then
             // i.tex = T2 B.x + T0 B.y + T1 B.z (photontrace.fs:519-544, 562-565)
-            const f3 V0 = xyz(tr[0]), V1 = xyz(tr[1]), V2 = xyz(tr[2]);
+            const float4* tvx = s.tri_isect + (size_t)4 * st.x;
+            const f3 V0 = xyz(tvx[0]), V1 = xyz(tvx[1]), V2 = xyz(tvx[2]);
             const float bx = length3(cross3(V1 - V0, ipos - V0));
             const float by = length3(cross3(V2 - V1, ipos - V1));
             const float bz = length3(cross3(V0 - V2, ipos - V2));

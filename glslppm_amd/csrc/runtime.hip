@@ -415,7 +415,7 @@ float max3f(float a, float b, float c) { return std::fmax(std::fmax(a, b), c); }
 int pack_triangles(const ppm_scene* s, std::vector<float4>& isect, std::vector<float4>& shade) {
     const int T = s->num_triangles;
     isect.resize((size_t)T * 4);
-    shade.resize((size_t)T * 6);
+    shade.resize((size_t)T * 4);
     std::vector<int> leaf(T, -1);
     for (int k = 0; k < s->num_nodes; k++)
         if (s->node_triangle[k] >= 0) leaf[s->node_triangle[k]] = k;
@@ -447,13 +447,11 @@ int pack_triangles(const ppm_scene* s, std::vector<float4>& isect, std::vector<f
         isect[(size_t)t * 4 + 1] = make_float4(p[3], p[4], p[5], lb[0]);
         isect[(size_t)t * 4 + 2] = make_float4(p[6], p[7], p[8], lb[1]);
         isect[(size_t)t * 4 + 3] = make_float4(lb[2], lb[3], lb[4], lb[5]);
-        float4* o = &shade[(size_t)t * 6];
-        o[0] = make_float4(p[0], p[1], p[2], rn[0][0]);
-        o[1] = make_float4(p[3], p[4], p[5], rn[0][1]);
-        o[2] = make_float4(p[6], p[7], p[8], rn[0][2]);
-        o[3] = make_float4(rn[1][0], rn[1][1], rn[1][2], rn[2][0]);
-        o[4] = make_float4(rn[2][1], rn[2][2], gn[0], gn[1]);
-        o[5] = make_float4(gn[2], as_float(mat), 0.0f, 0.0f);
+        float4* o = &shade[(size_t)t * 4];
+        o[0] = make_float4(rn[0][0], rn[0][1], rn[0][2], 0.0f);
+        o[1] = make_float4(rn[1][0], rn[1][1], rn[1][2], 0.0f);
+        o[2] = make_float4(rn[2][0], rn[2][1], rn[2][2], 0.0f);
+        o[3] = make_float4(gn[0], gn[1], gn[2], as_float(mat));
     }
     return stored;
 }
@@ -838,7 +836,7 @@ size_t state_footprint(const ppm_params* p) {
 
 size_t scene_footprint(int T, int N, int M, int NL, int tex_size, int wide_stride) {
     const size_t S = (size_t)tex_size, R = (size_t)wide_stride;
-    const size_t sizes[] = {(size_t)6 * (N + 1) * 16, 6 * R * 64, 6 * R * 16, (size_t)T * 64, (size_t)T * 96,
+    const size_t sizes[] = {(size_t)6 * (N + 1) * 16, 6 * R * 64, 6 * R * 16, (size_t)T * 64, (size_t)T * 64,
                             (size_t)M * 32, (size_t)NL * 4, (size_t)NL * 4, S ? (size_t)T * 32 : 0, (size_t)M * S * S * 4};
     size_t tot = 0;
     for (size_t b : sizes) tot += Arena::round_size(b ? b : 16);
