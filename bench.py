@@ -383,6 +383,9 @@ def main():
                     help="plain workloads: pick launch shapes with ppm_autotune during the warm-up")
     ap.add_argument("--tuning", default=None,
                     help='launch shapes to set first (plain and batch workloads), as JSON, e.g. \'{"walk_stack": 8}\'')
+    ap.add_argument("--readback-each-pass", default=None, choices=["rgba8", "radiance"],
+                    help="plain workloads: also time --steps passes each followed by a readback of this image "
+                         "(ppm_readback synchronises and copies over PCIe); reported as pcie_inclusive")
     ap.add_argument("--cpu-passes", type=int, default=None,
                     help="passes of the CPU baseline sample (0: skip; default: the workload's, about 10-20 s of "
                          "oracle time on 16 cores)")
@@ -473,6 +476,19 @@ def main():
         except Exception:
             traffic = None
 
+    # the rate a caller sees when it reads the image back after every pass: each ppm_pass(1) call
+    # ends with its own gather, and the readback waits for it and copies the image to the host
+    pcie = None
+    if args.readback_each_pass:
+        img = r.readback(args.readback_each_pass)
+        t2 = time.perf_counter()
+        for _ in range(args.steps):
+            r.run(1)
+            img = r.readback(args.readback_each_pass)
+        t3 = time.perf_counter()
+        pcie = {"value": P * args.steps / (t3 - t2), "unit": "photons/s", "readback": args.readback_each_pass,
+                "bytes_per_pass": int(img.nbytes), "ms_per_step": (t3 - t2) * 1e3 / args.steps}
+
     cpu = None
     cpu_passes = w["cpu_passes"] if args.cpu_passes is None else args.cpu_passes
     if cpu_passes > 0:
@@ -507,6 +523,8 @@ def main():
                      else "timed region"},
         "cpu_baseline": cpu,
     }
+    if pcie is not None:
+        line["pcie_inclusive"] = pcie
     print(json.dumps(line), flush=True)
 
 
