@@ -1,4 +1,6 @@
 """Helpers that compare the HIP renderer with the CPU oracle buffer by buffer."""
+import ctypes
+
 import numpy as np
 
 import glslppm_amd as g
@@ -65,3 +67,25 @@ def check_hash_properties(r, p):
     first = np.r_[True, s_sorted[1:] != s_sorted[:-1]]
     best[s_sorted[first]] = i_sorted[first]
     assert np.array_equal(best.astype(np.int32), rep)
+
+
+class ShrunkLeafBoxes:
+    """A built mesh's scene with every leaf box shrunk by 1/8 of its extent on each side, as a
+    caller may hand in its own tree: the boxes stay nested, and a ray that passes near a
+    triangle's edge misses its leaf box, so the reference skips that triangle.  No leaf box is its
+    triangle's vertex box, so the walks must read the leaf boxes from the triangle records; the
+    vertex boxes would find hits the reference does not."""
+
+    def __init__(self, mesh):
+        self._mesh = mesh
+        a = mesh.arrays()
+        s = g.Scene.from_buffer_copy(mesh.scene.contents)
+        b = a["node_bounds"].astype(np.float32)
+        leaf = a["node_triangle"] >= 0
+        ext = (b[:, 3:] - b[:, :3]) * np.float32(0.125)
+        b[leaf, :3] += ext[leaf]
+        b[leaf, 3:] -= ext[leaf]
+        self._bounds = np.ascontiguousarray(b.reshape(-1))
+        s.node_bounds = self._bounds.ctypes.data_as(ctypes.POINTER(ctypes.c_float))
+        self._scene = s
+        self.scene = ctypes.pointer(s)

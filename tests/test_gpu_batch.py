@@ -10,7 +10,7 @@ import pytest
 
 import glslppm_amd as g
 import oracle
-from parity import STATE_BUFFERS, compare_all, format_report
+from parity import STATE_BUFFERS, ShrunkLeafBoxes, compare_all, format_report
 
 pytestmark = pytest.mark.gpu
 
@@ -106,6 +106,24 @@ def test_batch_jobs_equal_alone(passes_per_graph, eye_overlap):
     for (label, m, p, _), j in zip(specs[:3], ids[:3]):
         check_equal_alone(JobView(b, j), m, p, expect[j], label)
     check_equal_alone(JobView(b, late), late_mesh, late_params, 20, "late-40x24")
+    b.close()
+
+
+def test_batch_mixes_vertex_and_stored_leaf_boxes():
+    """One batched launch with a scene whose leaf boxes are its triangles' vertex boxes (the walk
+    takes them from the vertices) and the same scene with every leaf box shrunk (the walk reads
+    them from the triangle records): each job equals the same job alone and the oracle."""
+    p = g.default_params(64, image_width=48, image_height=40, photon_width=64, photon_height=32, hash_width=128,
+                         hash_height=16, max_path_length=6)
+    plain = spheres(7, 6, 2, 48, 40)
+    shrunk = ShrunkLeafBoxes(spheres(7, 6, 2, 48, 40))
+    b = g.Batch(passes_per_graph=0)
+    ja = b.add(plain, p, 13)
+    jb = b.add(shrunk, p, 13)
+    b.run(13)
+    b.sync()
+    check_equal_alone(JobView(b, ja), plain, p, 13, "vertex-leaf-boxes")
+    check_equal_alone(JobView(b, jb), shrunk, p, 13, "stored-leaf-boxes")
     b.close()
 
 

@@ -6,13 +6,11 @@ progressive statistics (tau, R, E, N), the photon state (position, flux, directi
 state, triangle, bounce, emitted flag), the hash (slot per photon, count and representative per
 slot), the linear and tone-mapped radiance, the counters and the grid scalars.
 """
-import ctypes
-
 import numpy as np
 import pytest
 
 import glslppm_amd as g
-from parity import check_hash_properties, compare_all, format_report, run_pair
+from parity import ShrunkLeafBoxes, check_hash_properties, compare_all, format_report, run_pair
 
 pytestmark = pytest.mark.gpu
 
@@ -46,32 +44,10 @@ def test_spheres_glass_mirror():
     _check(m, g.default_params(64, max_path_length=10), 23, "spheres")
 
 
-class _ShrunkLeafBoxes:
-    """A built mesh's scene with every leaf box shrunk by 1/8 of its extent on each side, as a
-    caller may hand in its own tree: the boxes stay nested, and a ray that passes near a
-    triangle's edge misses its leaf box, so the reference skips that triangle.  No leaf box is its
-    triangle's vertex box, so the walks must read the leaf boxes from the triangle records; the
-    vertex boxes would find hits the reference does not."""
-
-    def __init__(self, mesh):
-        self._mesh = mesh
-        a = mesh.arrays()
-        s = g.Scene.from_buffer_copy(mesh.scene.contents)
-        b = a["node_bounds"].astype(np.float32)
-        leaf = a["node_triangle"] >= 0
-        ext = (b[:, 3:] - b[:, :3]) * np.float32(0.125)
-        b[leaf, :3] += ext[leaf]
-        b[leaf, 3:] -= ext[leaf]
-        self._bounds = np.ascontiguousarray(b.reshape(-1))
-        s.node_bounds = self._bounds.ctypes.data_as(ctypes.POINTER(ctypes.c_float))
-        self._scene = s
-        self.scene = ctypes.pointer(s)
-
-
 def test_leaf_boxes_that_are_not_their_triangles_boxes():
     m = g.Mesh.generate(g.SCENE_SPHERES, seed=7, count=10, detail=2)
     m.set_camera((0, 0, 1.667), (0, 0, 0), 64, 64, 45.0).build()
-    _check(_ShrunkLeafBoxes(m), g.default_params(64, max_path_length=10), 23, "shrunk-leaf-boxes")
+    _check(ShrunkLeafBoxes(m), g.default_params(64, max_path_length=10), 23, "shrunk-leaf-boxes")
 
 
 def test_decoupled_sizes():
